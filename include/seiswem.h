@@ -1,0 +1,188 @@
+/*
+ * seiswem.h -- C ABI of libseiswem.so, the MI355X (gfx950) shot-profile wave-equation
+ * migration / demigration library (split-step Fourier and PSPI, 2-D and 3-D).
+ *
+ * Every entry point returns an int status (SWEM_OK == 0) and never calls exit(); the text of
+ * the last error on the calling thread is available from swem_last_error().  Plain pointers
+ * and sizes only.  Array layouts follow the reference's alloc2float convention
+ * (src/API/c/seismic.c:92-106): a [n2][n1] array is contiguous with n1 fastest, so
+ *   d   [nm][nt]  traces of one shot, time fastest      (wem.c:3, d[ix][it])
+ *   m   [nm][nz]  image, depth fastest                  (wem.c:3, m[ix][iz])
+ *   vel [nm][nz]  velocity, depth fastest               (wem.c:9, vel[ix][iz])
+ *   wav [nt]      source wavelet                         (wem.c:3, wav[0][it])
+ * with trace index ix = imx*nmy + imy (y fastest) and nm = nmx*nmy.
+ */
+#ifndef SEISWEM_H
+#define SEISWEM_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWEM_VERSION 100
+
+/* Status codes. */
+#define SWEM_OK 0
+#define SWEM_EINVAL 1        /* invalid argument (missing file name, bad size, nref < 2, ...) */
+#define SWEM_ENOMEM 2        /* host or device allocation failed, or budget exceeded */
+#define SWEM_EDEVICE 3       /* HIP runtime error, or no usable gfx950 device */
+#define SWEM_EIO 4           /* .seis file could not be opened / read / written */
+#define SWEM_EUNSUPPORTED 5  /* valid input this build does not handle yet (named in the message) */
+#define SWEM_EBUSY 6         /* serving queue full */
+
+/* pspi_mode */
+#define SWEM_PSPI_COMPAT 0   /* the reference: real-part cut + nearest lower vref, both directions
+                                (wem.c:450-459, linear_interp wem.c:473-478); not an adjoint pair */
+#define SWEM_PSPI_LINEAR 1   /* linear interpolation between vref1 and vref2 (the formula at
+                                wem.c:476); the forward operator is the exact adjoint */
+
+/* precision */
+#define SWEM_PREC_KZ64 0     /* FP32 propagation, kz phase built from FP64 s = w^2 p^2 - k^2,
+                                compensated image sums (default) */
+#define SWEM_PREC_FP32 1     /* naive FP32: s, sqrt and phase in float like wem.c:341-343 */
+
+/* Geometry of one migration problem: the scalar arguments of wem() (wem.c:3-11). */
+typedef struct swem_geometry {
+    int nt;   float ot, dt;
+    int nmx;  float omx, dmx;
+    int nmy;  float omy, dmy;
+    int nz;   float oz, dz;
+    float gz, sz;
+    int nref; float fmin, fmax;
+    int padt, padx;
+    int pspi;               /* 0 = SSF (ssop, wem.c:289), 1 = PSPI (pspiop, wem.c:380) */
+} swem_geometry;
+
+typedef struct swem_options {
+    int pspi_mode;          /* SWEM_PSPI_* (default COMPAT) */
+    int precision;          /* SWEM_PREC_* (default KZ64) */
+    int device;             /* HIP device ordinal (default 0) */
+    int freqs_per_block;    /* frequencies marched together by one workgroup; 0 = autotuned */
+    int verbose;            /* progress to stderr like progress_msg (wem.c:516-520) */
+    size_t scratch_bytes;   /* device scratch budget for one launch chunk; 0 = default */
+} swem_options;
+
+/* Per-call timing of the last plan call (HIP events on the plan's stream). */
+typedef struct swem_timing {
+    double total_ms;        /* first event to last event */
+    double spectra_ms;      /* time -> frequency transforms */
+    double march_ms;        /* sum of depth-march kernel durations */
+    double reduce_ms;       /* image reduction / frequency -> time */
+    int march_launches;     /* number of depth-march kernel launches */
+    double march_flops;     /* algorithmic FP32 flops of the march launches (5 N log2 N per FFT) */
+} swem_timing;
+
+/* ---------------------------------------------------------------- library status */
+int swem_version(void);
+const char* swem_last_error(void);
+void swem_options_init(swem_options* opt);
+int swem_device_count(int* count);
+
+/* ------------------------------------------------------------ reference drop-ins */
+
+/* Replaces shotwem.so's `int main(int argc, char** argv)` (shotwem.c:12-140): the same
+ * key=value argv protocol (seismic.c:2-71; keys adj, pspi, verbose, d, m, vel, wav, sz, gz,
+ * sx, sy, fmin, fmax, padt, padx, nref), the same .seis files in and out (seismic.c:270-461),
+ * and the same header rewrites (shotwem.c:86-98, 113-117).  Unlike the reference it returns
+ * SWEM_EINVAL instead of calling exit(1) when a required file name is missing.  The library
+ * shotwem.so built beside libseiswem.so exports this function under the name `main`, so the
+ * Julia ccall of ShotProfileWEM.jl:347 binds to it unchanged. */
+int swem_shot_main(int argc, char** argv);
+
+/* Replaces wem() (wem.c:3-11) with host arrays: adj=1 migrates d [nm][nt] into m [nm][nz]
+ * (overwritten), adj=0 models m into d (overwritten).  pspi selects PSPI (compat mode) or SSF. */
+int swem_wem(float* d, float* m, const float* wav,
+             int nt, float ot, float dt,
+             int nmx, float omx, float dmx,
+             int nmy, float omy, float dmy,
+             float sx, float sy,
+             int nz, float oz, float dz, float gz, float sz,
+             const float* vel, int nref, float fmin, float fmax,
+             int padt, int padx,
+             int adj, int pspi, int verbose);
+
+/* wem() with explicit options (pspi_mode, precision, device...). */
+int swem_wem_ex(float* d, float* m, const float* wav, const swem_geometry* g,
+                float sx, float sy, const float* vel, int adj, const swem_options* opt);
+
+/* ShotProfileWEM (ShotProfileWEM.jl:39-310) in memory with nangx = nangy = 1: several shots
+ * over one velocity grid.  adj=1: m [nm][nz] = sum over shots of wem_adj(d[shot]) (the
+ * stacking of ShotProfileWEM.jl:140-201); adj=0: d[shot] = wem_fwd(m) for every shot
+ * (ShotProfileWEM.jl:288-308).  d [nshot][nm][nt]; sx, sy [nshot] (sy may be NULL). */
+int swem_shots(const swem_geometry* g, const float* vel, const float* wav,
+               const float* sx, const float* sy, int nshot,
+               float* d, float* m, int adj, const swem_options* opt);
+
+/* ---------------------------------------------------------- device-resident plans */
+typedef struct swem_plan swem_plan;
+
+/* Uploads velocity-derived tables and the source spectrum once; vel and wav are host arrays. */
+int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav,
+                     const swem_options* opt, swem_plan** out);
+int swem_plan_destroy(swem_plan* plan);
+
+/* Migration of nshot shots whose records d_dev [nshot][nm][nt] are already in device memory.
+ * Writes the stacked image m_dev [nm][nz] (device), adding to it when accumulate != 0.
+ * sx, sy are host arrays [nshot] (sy may be NULL).  stream is a hipStream_t (NULL = default).
+ * Returns after the work is enqueued; synchronise the stream before reading m_dev. */
+int swem_plan_migrate(swem_plan* plan, const float* d_dev, const float* sx, const float* sy,
+                      int nshot, float* m_dev, int accumulate, void* stream);
+/* As swem_plan_migrate but one image per shot: m_dev [nshot][nm][nz], no stacking. */
+int swem_plan_migrate_each(swem_plan* plan, const float* d_dev, const float* sx, const float* sy,
+                           int nshot, float* m_dev, void* stream);
+/* Demigration (modelling) of the image m_dev [nm][nz] into d_dev [nshot][nm][nt]. */
+int swem_plan_model(swem_plan* plan, const float* m_dev, const float* sx, const float* sy,
+                    int nshot, float* d_dev, void* stream);
+/* Timing of the last migrate/model call on this plan (synchronises its stream). */
+int swem_plan_timing(swem_plan* plan, swem_timing* out);
+/* Device bytes held by the plan's tables and its largest work chunk. */
+int swem_plan_bytes(const swem_plan* plan, size_t* table_bytes, size_t* work_bytes);
+
+/* ---------------------------------------------------------- device memory helpers */
+int swem_malloc(void** ptr, size_t bytes);
+int swem_free(void* ptr);
+int swem_memcpy_htod(void* dst, const void* src, size_t bytes);
+int swem_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int swem_memset(void* ptr, int value, size_t bytes);
+int swem_synchronize(void);
+
+/* ------------------------------------------- discrete decisions (host only, no GPU) */
+
+/* ntfft, nw, nkx, nky, ifmin, ifmax, lmx, lmy with the reference's casts (wem.c:79-89,
+ * 305-308): out[8] in that order. */
+int swem_sampling(const swem_geometry* g, int* out);
+/* igx = (int)(sx-omx)/dmx (wem.c:112-113); SWEM_EINVAL outside the grid. */
+int swem_source_index(const swem_geometry* g, float sx, float sy, int* igx, int* igy);
+/* Reference velocities vref [nref][nz], indices iref1/iref2 [nm][nz] and the PSPI branch
+ * flag cut [nm][nz] = (vref2 - vref1 > 10) with the reference's float arithmetic
+ * (wem.c:40-70, 454). */
+int swem_pspi_tables(const swem_geometry* g, const float* vel, float* vref, int* iref1,
+                     int* iref2, unsigned char* cut);
+/* Edge taper of boundary_condition (wem.c:546-564) for one axis of n points, width l. */
+int swem_taper(int n, int l, float* t);
+
+/* .seis I/O with the reference's layout (seismic.c:270-461, seismic.h:20-86): a text header
+ * file, a data file of native floats (n1 fastest) and 31 x 4-byte trace headers per trace. */
+#define SWEM_SEIS_HEADER_WORDS 31
+typedef struct swem_seis_extent {
+    int n1, n2, n3, n4, n5;
+    float o1, o2, o3, o4, o5;
+    float d1, d2, d3, d4, d5;
+    char label1[100], label2[100], label3[100], label4[100], label5[100];
+    char unit1[100], unit2[100], unit3[100], unit4[100], unit5[100];
+    char title[100];
+    char data_format[100];
+    int esize;
+    char dname[100];
+    char hname[100];
+} swem_seis_extent;
+void swem_seis_extent_init(swem_seis_extent* fh);
+int swem_seis_read_header(const char* name, swem_seis_extent* fh);
+/* data [ntrace][n1] floats, headers [ntrace][31] 4-byte words (either may be NULL). */
+int swem_seis_read(const char* name, float* data, void* headers, swem_seis_extent* fh);
+int swem_seis_write(const char* name, const float* data, const void* headers, swem_seis_extent* fh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

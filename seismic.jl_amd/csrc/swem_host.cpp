@@ -1,0 +1,700 @@
+// swem_host.cpp -- plans (device-resident tables and work buffers), the in-memory C ABI
+// (swem_wem, swem_shots, swem_plan_*) and the status helpers.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "swem_internal.hpp"
+
+namespace swem {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_error(hipError_t e, const char* what, const char* file, int line)
+{
+    return set_error(e == hipErrorOutOfMemory ? SWEM_ENOMEM : SWEM_EDEVICE, "%s failed: %s (%s:%d)", what,
+                     hipGetErrorString(e), file, line);
+}
+
+static const double kPi = 3.141592653589793;
+
+// A device allocation owned by a plan.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count)
+    {
+        if (count <= n && p) return SWEM_OK;
+        release();
+        if (count == 0) return SWEM_OK;
+        SWEM_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+        return SWEM_OK;
+    }
+    int upload(const T* h, size_t count)
+    {
+        SWEM_TRY(alloc(count));
+        if (count) SWEM_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+        return SWEM_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+static int check_device(int dev)
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count < 1)
+        return set_error(SWEM_EDEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    if (dev < 0 || dev >= count) return set_error(SWEM_EINVAL, "device %d out of range (%d devices)", dev, count);
+    SWEM_HIP(hipSetDevice(dev));
+    hipDeviceProp_t prop;
+    SWEM_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_error(SWEM_EDEVICE, "device %d is %s; this library is built for gfx950 (MI355X)", dev, prop.gcnArchName);
+    return SWEM_OK;
+}
+
+}  // namespace swem
+
+using namespace swem;
+
+struct swem_plan {
+    swem_geometry g{};
+    swem_options opt{};
+    Sampling s{};
+    int nm = 0, nf = 0, nkx = 0, T = 0, F = 1, ngroups = 0, mode = 0;
+    std::vector<float> vel;  // host copy (source of every table)
+    // tables
+    DevBuf<float2> tw, spec_s;
+    DevBuf<float> pdpi, taper, p2f, w2, basis_adj, basis_fwd;
+    DevBuf<double> p2;
+    DevBuf<uint8_t> sel, sel2;
+    DevBuf<uint32_t> used;
+    // work
+    DevBuf<float2> spec_r, smig, dgw;
+    DevBuf<float> partial, mT;
+    DevBuf<int> igx;
+    int* igx_pinned = nullptr;
+    int igx_cap = 0;
+    size_t budget = 0;
+    // timing of the last call
+    std::vector<hipEvent_t> events;
+    int nev = 0;
+    std::vector<int> ev_kind;  // stage of the interval that ends at event i (0 spectra, 1 march, 2 reduce)
+    int last_launches = 0;
+    double last_flops = 0;
+    hipStream_t last_stream = nullptr;
+};
+
+namespace swem {
+
+// Algorithmic FP32 flops of one (frequency, depth) step: 2 wavefields + imaging.
+static double step_flops(const swem_plan* p)
+{
+    const double nk = p->s.nkx * (double)p->s.nky, nm = p->nm, lg = std::log2(nk);
+    double fw;
+    if (p->g.pspi)
+        fw = 5.0 * nk * lg * (1.0 + p->g.nref) + 6.0 * p->g.nref * nk + 8.0 * nm;
+    else
+        fw = 10.0 * nk * lg + 6.0 * nk + 8.0 * nm;
+    return 2.0 * fw + 4.0 * nm;
+}
+
+static int choose_F(const swem_plan* p)
+{
+    if (p->opt.freqs_per_block > 0) return p->opt.freqs_per_block;
+    // Enough workgroups to fill 256 CUs for a single shot, otherwise the largest F that
+    // fits 1024 threads and the LDS (fewer partial images to write and reduce).
+    int F = std::max(1, march2d_max_block() / p->T);
+    while (F > 1 && march2d_lds_bytes(p->nkx, F) > 160 * 1024) F /= 2;
+    while (F > 1 && (p->nf + F - 1) / F < 256) F /= 2;
+    return F;
+}
+
+static int build_tables(swem_plan* p, const float* wav)
+{
+    const swem_geometry& g = p->g;
+    const int nmx = g.nmx, nz = g.nz, nm = p->nm, nf = p->nf, nt = g.nt, nkx = p->nkx;
+    // twiddles exp(-2 pi i m / nkx) in FP64
+    std::vector<float2> tw(nkx);
+    for (int m = 0; m < nkx; m++) {
+        double a = -2.0 * kPi * (double)m / (double)nkx;
+        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    SWEM_TRY(p->tw.upload(tw.data(), tw.size()));
+    // taper (wem.c:546-564)
+    std::vector<float> tap(nmx);
+    taper_f32(nmx, p->s.lmx, tap.data());
+    SWEM_TRY(p->taper.upload(tap.data(), tap.size()));
+    // slowness split (wem.c:27-36): FP64 for the propagation, float copy for naive FP32
+    std::vector<double> po(nz);
+    std::vector<float> pof(nz), pdf((size_t)nm * nz), pdpi((size_t)nz * nmx);
+    slowness_f32(g, p->vel.data(), pof.data(), pdf.data());
+    for (int iz = 0; iz < nz; iz++) {
+        double sum = 0;
+        for (int ix = 0; ix < nm; ix++) sum += p->vel[(size_t)ix * nz + iz];
+        po[iz] = 1.0 / (sum / (double)nm);
+        for (int ix = 0; ix < nmx; ix++) {
+            if (p->opt.precision == SWEM_PREC_FP32)
+                pdpi[(size_t)iz * nmx + ix] = (float)((double)pdf[(size_t)ix * nz + iz] * g.dz / kPi);
+            else
+                pdpi[(size_t)iz * nmx + ix] =
+                    (float)((1.0 / (double)p->vel[(size_t)ix * nz + iz] - po[iz]) * (double)g.dz / kPi);
+        }
+    }
+    SWEM_TRY(p->pdpi.upload(pdpi.data(), pdpi.size()));
+    if (!g.pspi) {
+        std::vector<double> p2(nz);
+        std::vector<float> p2f(nz);
+        for (int iz = 0; iz < nz; iz++) {
+            p2[iz] = po[iz] * po[iz];
+            p2f[iz] = pof[iz] * pof[iz];
+        }
+        SWEM_TRY(p->p2.upload(p2.data(), p2.size()));
+        SWEM_TRY(p->p2f.upload(p2f.data(), p2f.size()));
+        std::vector<uint32_t> used(nz, 1u);
+        SWEM_TRY(p->used.upload(used.data(), used.size()));
+    } else {
+        const int nref = g.nref;
+        std::vector<float> vref((size_t)nref * nz);
+        std::vector<int> i1((size_t)nm * nz), i2((size_t)nm * nz);
+        std::vector<uint8_t> cut((size_t)nm * nz);
+        SWEM_TRY(pspi_tables(g, p->vel.data(), vref.data(), i1.data(), i2.data(), cut.data()));
+        std::vector<double> p2((size_t)nz * nref);
+        std::vector<float> p2f((size_t)nz * nref);
+        std::vector<double> vd((size_t)nz * nref);
+        for (int iz = 0; iz < nz; iz++) {
+            const double vmin = vref[iz], vmax = vref[(size_t)(nref - 1) * nz + iz];
+            for (int ir = 0; ir < nref; ir++) {
+                const double v = vmin + (double)ir * (vmax - vmin) / ((double)nref - 1.0);
+                vd[(size_t)iz * nref + ir] = v;
+                p2[(size_t)iz * nref + ir] = 1.0 / (v * v);
+                const float vf = vref[(size_t)ir * nz + iz];
+                p2f[(size_t)iz * nref + ir] = 1 / (vf * vf);
+            }
+        }
+        std::vector<uint8_t> sel((size_t)nz * nmx), sel2((size_t)nz * nmx);
+        std::vector<float> w2((size_t)nz * nmx, 0.f);
+        std::vector<uint32_t> used(nz, 0u);
+        for (int ix = 0; ix < nm; ix++)
+            for (int iz = 0; iz < nz; iz++) {
+                const size_t i = (size_t)ix * nz + iz, o = (size_t)iz * nmx + ix;
+                sel[o] = (uint8_t)(i1[i] | (cut[i] ? 0x80 : 0));
+                sel2[o] = (uint8_t)i2[i];
+                used[iz] |= 1u << i1[i];
+                if (cut[i]) {
+                    const double v1 = vd[(size_t)iz * nref + i1[i]], v2 = vd[(size_t)iz * nref + i2[i]];
+                    w2[o] = (float)(((double)p->vel[i] - v1) / (v2 - v1));
+                    if (p->opt.pspi_mode == SWEM_PSPI_LINEAR) used[iz] |= 1u << i2[i];
+                }
+            }
+        SWEM_TRY(p->p2.upload(p2.data(), p2.size()));
+        SWEM_TRY(p->p2f.upload(p2f.data(), p2f.size()));
+        SWEM_TRY(p->sel.upload(sel.data(), sel.size()));
+        SWEM_TRY(p->sel2.upload(sel2.data(), sel2.size()));
+        SWEM_TRY(p->w2.upload(w2.data(), w2.size()));
+        SWEM_TRY(p->used.upload(used.data(), used.size()));
+    }
+    // Fourier bases of the band (f_op, wem.c:480-514, with the w^2 of wem.c:129,182 and the
+    // 1/sqrt(ntfft) of wem.c:231-232,278).  Angle index (iw*t) mod n exact in integers.
+    const int n = p->s.fop_n;
+    const double rnt = 1.0 / std::sqrt((double)p->s.ntfft);
+    const double dw = 2.0 * kPi / ((double)p->s.ntfft * (double)g.dt);
+    std::vector<float> badj((size_t)nt * 2 * nf), bsrc((size_t)nt * 2 * nf), bfwd((size_t)2 * nf * nt);
+    for (int f = 0; f < nf; f++) {
+        const int iw = p->s.ifmin + f;
+        const double w = iw * dw, w2s = w * w * rnt;
+        const double cf = (iw == 0 || 2 * iw == n) ? 1.0 : 2.0;
+        for (int t = 0; t < nt; t++) {
+            const long long e = ((long long)iw * t) % n;
+            const double ang = 2.0 * kPi * (double)e / (double)n;
+            const double c = std::cos(ang), sn = std::sin(ang);
+            badj[(size_t)t * 2 * nf + 2 * f] = (float)(w2s * c);
+            badj[(size_t)t * 2 * nf + 2 * f + 1] = (float)(-w2s * sn);
+            bsrc[(size_t)t * 2 * nf + 2 * f] = (float)(rnt * c);
+            bsrc[(size_t)t * 2 * nf + 2 * f + 1] = (float)(-rnt * sn);
+            bfwd[(size_t)(2 * f) * nt + t] = (float)(cf * w2s * c);
+            bfwd[(size_t)(2 * f + 1) * nt + t] = (float)(-cf * w2s * sn);
+        }
+    }
+    SWEM_TRY(p->basis_adj.upload(badj.data(), badj.size()));
+    SWEM_TRY(p->basis_fwd.upload(bfwd.data(), bfwd.size()));
+    // source spectrum (wem.c:115-119) through the same transform on the device
+    DevBuf<float> dsrc, dwav;
+    SWEM_TRY(dsrc.upload(bsrc.data(), bsrc.size()));
+    SWEM_TRY(dwav.upload(wav, nt));
+    SWEM_TRY(p->spec_s.alloc(nf));
+    SWEM_TRY(launch_sgemm(1, 2 * nf, nt, dwav.p, nt, dsrc.p, 2 * nf, reinterpret_cast<float*>(p->spec_s.p), 2 * nf,
+                          nullptr));
+    SWEM_HIP(hipDeviceSynchronize());
+    dsrc.release();
+    dwav.release();
+    return SWEM_OK;
+}
+
+static size_t default_budget()
+{
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)8 << 30;
+    return std::min((size_t)24 << 30, fr / 2);
+}
+
+static int plan_events(swem_plan* p, int n)
+{
+    while ((int)p->events.size() < n) {
+        hipEvent_t e;
+        SWEM_HIP(hipEventCreate(&e));
+        p->events.push_back(e);
+        p->ev_kind.push_back(0);
+    }
+    return SWEM_OK;
+}
+
+static int record(swem_plan* p, int kind, hipStream_t st)
+{
+    SWEM_TRY(plan_events(p, p->nev + 1));
+    p->ev_kind[p->nev] = kind;
+    SWEM_HIP(hipEventRecord(p->events[p->nev], st));
+    p->nev++;
+    return SWEM_OK;
+}
+
+static int set_igx(swem_plan* p, const float* sx, const float* sy, int nshot, hipStream_t st)
+{
+    if (p->igx_cap < nshot) {
+        if (p->igx_pinned) (void)hipHostFree(p->igx_pinned);
+        p->igx_pinned = nullptr;
+        SWEM_HIP(hipHostMalloc((void**)&p->igx_pinned, sizeof(int) * nshot, hipHostMallocDefault));
+        p->igx_cap = nshot;
+    }
+    // the previous call's copy out of the pinned buffer must have finished
+    if (p->last_stream || p->nev) SWEM_HIP(hipStreamSynchronize(p->last_stream));
+    for (int i = 0; i < nshot; i++) {
+        int ix, iy;
+        SWEM_TRY(source_index(p->g, sx[i], sy ? sy[i] : 0.f, ix, iy));
+        p->igx_pinned[i] = ix * p->g.nmy + iy;
+    }
+    SWEM_TRY(p->igx.alloc(nshot));
+    SWEM_HIP(hipMemcpyAsync(p->igx.p, p->igx_pinned, sizeof(int) * nshot, hipMemcpyHostToDevice, st));
+    return SWEM_OK;
+}
+
+static March2DArgs base_args(const swem_plan* p)
+{
+    March2DArgs a{};
+    a.nmx = p->g.nmx;
+    a.nz = p->g.nz;
+    a.nf = p->nf;
+    a.nref = p->g.pspi ? p->g.nref : 1;
+    a.ngroups = p->ngroups;
+    a.ifmin = p->s.ifmin;
+    a.oz = p->g.oz;
+    a.dz = p->g.dz;
+    a.sz = p->g.sz;
+    a.gz = p->g.gz;
+    a.dw = 2.0 * kPi / ((double)p->s.ntfft * (double)p->g.dt);
+    a.dwf = p->s.dw;
+    a.dkx = 2.0 * kPi / ((double)p->s.nkx * (double)p->g.dmx);
+    a.dkxf = p->s.dkx;
+    a.dz_pi = (double)p->g.dz / kPi;
+    a.evan = -0.2f * std::fabs(p->g.dz);
+    a.inv_nk = 1.0f / (float)p->s.nkx;
+    a.tw = p->tw.p;
+    a.spec_s = p->spec_s.p;
+    a.igx = p->igx.p;
+    a.pdpi = p->pdpi.p;
+    a.taper = p->taper.p;
+    a.p2 = p->p2.p;
+    a.p2f = p->p2f.p;
+    a.sel = p->sel.p;
+    a.sel2 = p->sel2.p;
+    a.w2 = p->w2.p;
+    a.used = p->used.p;
+    return a;
+}
+
+static int mode_of(const swem_plan* p)
+{
+    if (!p->g.pspi) return 0;
+    return p->opt.pspi_mode == SWEM_PSPI_LINEAR ? 2 : 1;
+}
+
+static int migrate_impl(swem_plan* p, const float* d_dev, const float* sx, const float* sy, int nshot, float* m_dev,
+                        int accumulate, int per_shot, hipStream_t st)
+{
+    if (!p || !d_dev || !sx || !m_dev || nshot < 1) return set_error(SWEM_EINVAL, "migrate: null argument or nshot < 1");
+    SWEM_HIP(hipSetDevice(p->opt.device));
+    const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
+    const size_t per = nm * nf * sizeof(float2) + (size_t)p->ngroups * nz * p->g.nmx * sizeof(float);
+    int chunk = (int)std::max<size_t>(1, std::min<size_t>(nshot, p->budget / std::max<size_t>(per, 1)));
+    SWEM_TRY(p->spec_r.alloc((size_t)chunk * nm * nf));
+    SWEM_TRY(p->partial.alloc((size_t)chunk * p->ngroups * nz * p->g.nmx));
+    SWEM_TRY(set_igx(p, sx, sy, nshot, st));
+    p->nev = 0;
+    p->last_launches = 0;
+    p->last_flops = 0;
+    p->last_stream = st;
+    SWEM_TRY(record(p, -1, st));
+    for (int s0 = 0; s0 < nshot; s0 += chunk) {
+        const int ns = std::min(chunk, nshot - s0);
+        SWEM_TRY(launch_sgemm(ns * (int)nm, 2 * (int)nf, (int)nt, d_dev + (size_t)s0 * nm * nt, (int)nt, p->basis_adj.p,
+                              2 * (int)nf, reinterpret_cast<float*>(p->spec_r.p), 2 * (int)nf, st));
+        SWEM_TRY(record(p, 0, st));
+        March2DArgs a = base_args(p);
+        a.nshot = ns;
+        a.igx = p->igx.p + s0;
+        a.spec_r = p->spec_r.p;
+        a.partial = p->partial.p;
+        SWEM_TRY(launch_march2d(a, p->nkx, p->F, mode_of(p), p->opt.precision, 1, st));
+        SWEM_TRY(record(p, 1, st));
+        p->last_launches++;
+        p->last_flops += step_flops(p) * (double)nf * nz * ns;
+        float* out = per_shot ? m_dev + (size_t)s0 * nm * nz : m_dev;
+        SWEM_TRY(launch_reduce_image(p->partial.p, ns, p->ngroups, (int)nz, p->g.nmx, out,
+                                     per_shot ? 0 : (accumulate || s0 > 0), per_shot, st));
+        SWEM_TRY(record(p, 2, st));
+    }
+    return SWEM_OK;
+}
+
+static int model_impl(swem_plan* p, const float* m_dev, const float* sx, const float* sy, int nshot, float* d_dev,
+                      hipStream_t st)
+{
+    if (!p || !d_dev || !sx || !m_dev || nshot < 1) return set_error(SWEM_EINVAL, "model: null argument or nshot < 1");
+    SWEM_HIP(hipSetDevice(p->opt.device));
+    const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
+    const size_t per = nm * nf * sizeof(float2) + nf * nz * p->g.nmx * sizeof(float2);
+    int chunk = (int)std::max<size_t>(1, std::min<size_t>(nshot, p->budget / std::max<size_t>(per, 1)));
+    SWEM_TRY(p->dgw.alloc((size_t)chunk * nm * nf));
+    SWEM_TRY(p->smig.alloc((size_t)chunk * nf * nz * p->g.nmx));
+    SWEM_TRY(p->mT.alloc(nm * nz));
+    SWEM_TRY(set_igx(p, sx, sy, nshot, st));
+    p->nev = 0;
+    p->last_launches = 0;
+    p->last_flops = 0;
+    p->last_stream = st;
+    SWEM_TRY(record(p, -1, st));
+    SWEM_TRY(launch_transpose(m_dev, (int)nm, (int)nz, p->mT.p, st));
+    SWEM_TRY(record(p, 0, st));
+    for (int s0 = 0; s0 < nshot; s0 += chunk) {
+        const int ns = std::min(chunk, nshot - s0);
+        March2DArgs a = base_args(p);
+        a.nshot = ns;
+        a.igx = p->igx.p + s0;
+        a.mT = p->mT.p;
+        a.smig = p->smig.p;
+        a.dgw = p->dgw.p;
+        SWEM_TRY(launch_march2d(a, p->nkx, p->F, mode_of(p), p->opt.precision, 0, st));
+        SWEM_TRY(record(p, 1, st));
+        p->last_launches++;
+        p->last_flops += step_flops(p) * (double)nf * nz * ns;
+        SWEM_TRY(launch_sgemm(ns * (int)nm, (int)nt, 2 * (int)nf, reinterpret_cast<const float*>(p->dgw.p), 2 * (int)nf,
+                              p->basis_fwd.p, (int)nt, d_dev + (size_t)s0 * nm * nt, (int)nt, st));
+        SWEM_TRY(record(p, 2, st));
+    }
+    return SWEM_OK;
+}
+
+}  // namespace swem
+
+// ============================================================================ C ABI
+
+extern "C" {
+
+int swem_version(void) { return SWEM_VERSION; }
+
+const char* swem_last_error(void) { return g_err.c_str(); }
+
+void swem_options_init(swem_options* opt)
+{
+    if (!opt) return;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->pspi_mode = SWEM_PSPI_COMPAT;
+    opt->precision = SWEM_PREC_KZ64;
+}
+
+int swem_device_count(int* count)
+{
+    if (!count) return set_error(SWEM_EINVAL, "null count");
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        return set_error(SWEM_EDEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    return SWEM_OK;
+}
+
+int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav, const swem_options* opt,
+                     swem_plan** out)
+{
+    if (!g || !vel || !wav || !out) return set_error(SWEM_EINVAL, "plan_create: null argument");
+    *out = nullptr;
+    swem_options o;
+    swem_options_init(&o);
+    if (opt) o = *opt;
+    if (o.pspi_mode != SWEM_PSPI_COMPAT && o.pspi_mode != SWEM_PSPI_LINEAR)
+        return set_error(SWEM_EINVAL, "pspi_mode %d", o.pspi_mode);
+    if (o.precision != SWEM_PREC_KZ64 && o.precision != SWEM_PREC_FP32)
+        return set_error(SWEM_EINVAL, "precision %d", o.precision);
+    swem_plan* p = new swem_plan();
+    p->g = *g;
+    p->opt = o;
+    int rc = compute_sampling(p->g, p->s);
+    if (rc) { delete p; return rc; }
+    p->nm = g->nmx * g->nmy;
+    p->nf = p->s.ifmax - p->s.ifmin;
+    p->nkx = p->s.nkx;
+    if (g->nmy > 1) { delete p; return set_error(SWEM_EUNSUPPORTED, "3-D (nmy=%d > 1) is not available in this build", g->nmy); }
+    if (!march2d_supported(p->nkx)) {
+        delete p;
+        return set_error(SWEM_EUNSUPPORTED, "nkx = padx*nmx = %d: this build handles powers of two 128..4096", p->nkx);
+    }
+    if (p->nf < 1) { delete p; return set_error(SWEM_EINVAL, "empty frequency band [ifmin=%d, ifmax=%d)", p->s.ifmin, p->s.ifmax); }
+    if (g->pspi && (g->nref < 2 || g->nref > 31)) { delete p; return set_error(SWEM_EINVAL, "PSPI needs 2 <= nref <= 31 (nref=%d)", g->nref); }
+    if (!(g->dz != 0.f)) { delete p; return set_error(SWEM_EINVAL, "dz must be non-zero"); }
+    rc = check_device(o.device);
+    if (rc) { delete p; return rc; }
+    p->T = march2d_threads(p->nkx);
+    p->F = choose_F(p);
+    if (p->F < 1 || p->F * p->T > march2d_max_block() || march2d_lds_bytes(p->nkx, p->F) > 160 * 1024) {
+        delete p;
+        return set_error(SWEM_EINVAL, "freqs_per_block=%d does not fit a workgroup at nkx=%d", o.freqs_per_block, p->s.nkx);
+    }
+    p->ngroups = (p->nf + p->F - 1) / p->F;
+    p->mode = mode_of(p);
+    p->vel.assign(vel, vel + (size_t)p->nm * g->nz);
+    p->budget = o.scratch_bytes ? o.scratch_bytes : default_budget();
+    rc = build_tables(p, wav);
+    if (rc) { swem_plan_destroy(p); return rc; }
+    *out = p;
+    return SWEM_OK;
+}
+
+int swem_plan_destroy(swem_plan* p)
+{
+    if (!p) return SWEM_OK;
+    (void)hipSetDevice(p->opt.device);
+    if (p->last_stream || p->nev) (void)hipStreamSynchronize(p->last_stream);
+    for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
+    if (p->igx_pinned) (void)hipHostFree(p->igx_pinned);
+    p->tw.release(); p->spec_s.release(); p->pdpi.release(); p->taper.release(); p->p2f.release(); p->w2.release();
+    p->basis_adj.release(); p->basis_fwd.release(); p->p2.release(); p->sel.release(); p->sel2.release();
+    p->used.release(); p->spec_r.release(); p->smig.release(); p->dgw.release(); p->partial.release();
+    p->mT.release(); p->igx.release();
+    delete p;
+    return SWEM_OK;
+}
+
+int swem_plan_migrate(swem_plan* p, const float* d_dev, const float* sx, const float* sy, int nshot, float* m_dev,
+                      int accumulate, void* stream)
+{
+    return migrate_impl(p, d_dev, sx, sy, nshot, m_dev, accumulate, 0, (hipStream_t)stream);
+}
+
+int swem_plan_migrate_each(swem_plan* p, const float* d_dev, const float* sx, const float* sy, int nshot, float* m_dev,
+                           void* stream)
+{
+    return migrate_impl(p, d_dev, sx, sy, nshot, m_dev, 0, 1, (hipStream_t)stream);
+}
+
+int swem_plan_model(swem_plan* p, const float* m_dev, const float* sx, const float* sy, int nshot, float* d_dev,
+                    void* stream)
+{
+    return model_impl(p, m_dev, sx, sy, nshot, d_dev, (hipStream_t)stream);
+}
+
+int swem_plan_timing(swem_plan* p, swem_timing* out)
+{
+    if (!p || !out) return set_error(SWEM_EINVAL, "plan_timing: null argument");
+    std::memset(out, 0, sizeof(*out));
+    if (p->nev < 2) return SWEM_OK;
+    SWEM_HIP(hipSetDevice(p->opt.device));
+    SWEM_HIP(hipEventSynchronize(p->events[p->nev - 1]));
+    float ms = 0.f;
+    SWEM_HIP(hipEventElapsedTime(&ms, p->events[0], p->events[p->nev - 1]));
+    out->total_ms = ms;
+    for (int i = 1; i < p->nev; i++) {
+        SWEM_HIP(hipEventElapsedTime(&ms, p->events[i - 1], p->events[i]));
+        if (p->ev_kind[i] == 0) out->spectra_ms += ms;
+        else if (p->ev_kind[i] == 1) out->march_ms += ms;
+        else out->reduce_ms += ms;
+    }
+    out->march_launches = p->last_launches;
+    out->march_flops = p->last_flops;
+    return SWEM_OK;
+}
+
+int swem_plan_bytes(const swem_plan* p, size_t* table_bytes, size_t* work_bytes)
+{
+    if (!p) return set_error(SWEM_EINVAL, "plan_bytes: null plan");
+    if (table_bytes)
+        *table_bytes = p->tw.bytes() + p->spec_s.bytes() + p->pdpi.bytes() + p->taper.bytes() + p->p2f.bytes() +
+                       p->w2.bytes() + p->basis_adj.bytes() + p->basis_fwd.bytes() + p->p2.bytes() + p->sel.bytes() +
+                       p->sel2.bytes() + p->used.bytes();
+    if (work_bytes)
+        *work_bytes = p->spec_r.bytes() + p->smig.bytes() + p->dgw.bytes() + p->partial.bytes() + p->mT.bytes() +
+                      p->igx.bytes();
+    return SWEM_OK;
+}
+
+static void geom_from_args(swem_geometry* g, int nt, float ot, float dt, int nmx, float omx, float dmx, int nmy,
+                           float omy, float dmy, int nz, float oz, float dz, float gz, float sz, int nref, float fmin,
+                           float fmax, int padt, int padx, int pspi)
+{
+    g->nt = nt; g->ot = ot; g->dt = dt;
+    g->nmx = nmx; g->omx = omx; g->dmx = dmx;
+    g->nmy = nmy; g->omy = omy; g->dmy = dmy;
+    g->nz = nz; g->oz = oz; g->dz = dz;
+    g->gz = gz; g->sz = sz;
+    g->nref = nref; g->fmin = fmin; g->fmax = fmax;
+    g->padt = padt; g->padx = padx; g->pspi = pspi;
+}
+
+int swem_shots(const swem_geometry* g, const float* vel, const float* wav, const float* sx, const float* sy, int nshot,
+               float* d, float* m, int adj, const swem_options* opt)
+{
+    if (!g || !vel || !wav || !sx || !d || !m || nshot < 1) return set_error(SWEM_EINVAL, "shots: null argument");
+    swem_plan* p = nullptr;
+    SWEM_TRY(swem_plan_create(g, vel, wav, opt, &p));
+    const size_t nm = (size_t)g->nmx * g->nmy, dbytes = (size_t)nshot * nm * g->nt * sizeof(float),
+                 mbytes = nm * g->nz * sizeof(float);
+    float *dd = nullptr, *dm = nullptr;
+    int rc = SWEM_OK;
+    hipError_t e = hipMalloc((void**)&dd, dbytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&dm, mbytes);
+    if (e != hipSuccess) rc = hip_error(e, "hipMalloc", __FILE__, __LINE__);
+    if (!rc) {
+        if (adj) {
+            e = hipMemcpy(dd, d, dbytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = hip_error(e, "hipMemcpy", __FILE__, __LINE__);
+            if (!rc) rc = swem_plan_migrate(p, dd, sx, sy, nshot, dm, 0, nullptr);
+            if (!rc) {
+                e = hipMemcpy(m, dm, mbytes, hipMemcpyDeviceToHost);
+                if (e != hipSuccess) rc = hip_error(e, "hipMemcpy", __FILE__, __LINE__);
+            }
+        } else {
+            e = hipMemcpy(dm, m, mbytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = hip_error(e, "hipMemcpy", __FILE__, __LINE__);
+            if (!rc) rc = swem_plan_model(p, dm, sx, sy, nshot, dd, nullptr);
+            if (!rc) {
+                e = hipMemcpy(d, dd, dbytes, hipMemcpyDeviceToHost);
+                if (e != hipSuccess) rc = hip_error(e, "hipMemcpy", __FILE__, __LINE__);
+            }
+        }
+    }
+    if (dd) (void)hipFree(dd);
+    if (dm) (void)hipFree(dm);
+    swem_plan_destroy(p);
+    return rc;
+}
+
+int swem_wem_ex(float* d, float* m, const float* wav, const swem_geometry* g, float sx, float sy, const float* vel,
+                int adj, const swem_options* opt)
+{
+    return swem_shots(g, vel, wav, &sx, &sy, 1, d, m, adj, opt);
+}
+
+int swem_wem(float* d, float* m, const float* wav, int nt, float ot, float dt, int nmx, float omx, float dmx, int nmy,
+             float omy, float dmy, float sx, float sy, int nz, float oz, float dz, float gz, float sz, const float* vel,
+             int nref, float fmin, float fmax, int padt, int padx, int adj, int pspi, int verbose)
+{
+    swem_geometry g;
+    geom_from_args(&g, nt, ot, dt, nmx, omx, dmx, nmy, omy, dmy, nz, oz, dz, gz, sz, nref, fmin, fmax, padt, padx, pspi);
+    swem_options o;
+    swem_options_init(&o);
+    o.verbose = verbose;
+    return swem_wem_ex(d, m, wav, &g, sx, sy, vel, adj, &o);
+}
+
+int swem_malloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return set_error(SWEM_EINVAL, "null pointer");
+    SWEM_HIP(hipMalloc(ptr, bytes));
+    return SWEM_OK;
+}
+
+int swem_free(void* ptr)
+{
+    if (ptr) SWEM_HIP(hipFree(ptr));
+    return SWEM_OK;
+}
+
+int swem_memcpy_htod(void* dst, const void* src, size_t bytes)
+{
+    SWEM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SWEM_OK;
+}
+
+int swem_memcpy_dtoh(void* dst, const void* src, size_t bytes)
+{
+    SWEM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SWEM_OK;
+}
+
+int swem_memset(void* ptr, int value, size_t bytes)
+{
+    SWEM_HIP(hipMemset(ptr, value, bytes));
+    return SWEM_OK;
+}
+
+int swem_synchronize(void)
+{
+    SWEM_HIP(hipDeviceSynchronize());
+    return SWEM_OK;
+}
+
+int swem_sampling(const swem_geometry* g, int* out)
+{
+    if (!g || !out) return set_error(SWEM_EINVAL, "sampling: null argument");
+    Sampling s;
+    SWEM_TRY(compute_sampling(*g, s));
+    out[0] = s.ntfft; out[1] = s.nw; out[2] = s.nkx; out[3] = s.nky;
+    out[4] = s.ifmin; out[5] = s.ifmax; out[6] = s.lmx; out[7] = s.lmy;
+    return SWEM_OK;
+}
+
+int swem_source_index(const swem_geometry* g, float sx, float sy, int* igx, int* igy)
+{
+    if (!g || !igx || !igy) return set_error(SWEM_EINVAL, "source_index: null argument");
+    return source_index(*g, sx, sy, *igx, *igy);
+}
+
+int swem_pspi_tables(const swem_geometry* g, const float* vel, float* vref, int* iref1, int* iref2, unsigned char* cut)
+{
+    if (!g || !vel || !vref) return set_error(SWEM_EINVAL, "pspi_tables: null argument");
+    if (g->nmx < 1 || g->nmy < 1 || g->nz < 1) return set_error(SWEM_EINVAL, "pspi_tables: bad sizes");
+    return pspi_tables(*g, vel, vref, iref1, iref2, cut);
+}
+
+int swem_taper(int n, int l, float* t)
+{
+    if (!t || n < 0 || l < 0) return set_error(SWEM_EINVAL, "taper: bad argument");
+    taper_f32(n, l, t);
+    return SWEM_OK;
+}
+
+}  // extern "C"

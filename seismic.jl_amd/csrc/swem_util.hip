@@ -1,0 +1,136 @@
+// swem_util.hip -- band-limited time<->frequency transforms (as a tiled SGEMM against an FP64-built
+// Fourier basis), the compensated image reduction and a transpose.
+//
+// The reference transforms every trace with an r2c / c2r FFT of length (nw-1)*2 and keeps the
+// band [ifmin, ifmax) (wem.c:121-140, 177-188, f_op wem.c:480-514).  Only nf of nw bins are used
+// and the transform length need not be a power of two (2000, 3000, 4000, 8000 in the
+// workloads), so the band is computed directly: spec[x][f] = sum_t d[x][t] B[t][f] with
+// B[t][2f] = c_f cos(2 pi iw t / n), B[t][2f+1] = -c_f sin(2 pi iw t / n), the angle reduced
+// exactly in integers and evaluated in FP64 on the host.
+#include "swem_internal.hpp"
+
+namespace swem {
+
+// 64x64 output tile per 256-thread workgroup, 4x4 per thread, K tile 16.
+__global__ __launch_bounds__(256) void sgemm_nn(int M, int Nc, int K, const float* __restrict__ A, int lda,
+                                                const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc)
+{
+    __shared__ float As[16][64 + 4];
+    __shared__ float Bs[16][64 + 4];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = tid + i * 256;
+            const int mm = idx >> 4, kk = idx & 15;
+            As[kk][mm] = (m0 + mm < M && k0 + kk < K) ? A[(size_t)(m0 + mm) * lda + k0 + kk] : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = tid + i * 256;
+            const int kk = idx >> 6, nn = idx & 63;
+            Bs[kk][nn] = (k0 + kk < K && n0 + nn < Nc) ? B[(size_t)(k0 + kk) * ldb + n0 + nn] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            float av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(av[i], bv[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = m0 + ty + 16 * i;
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + tx + 16 * j;
+            if (n < Nc) C[(size_t)m * ldc + n] = acc[i][j];
+        }
+    }
+}
+
+int launch_sgemm(int M, int Nc, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc,
+                 hipStream_t st)
+{
+    if (M <= 0 || Nc <= 0 || K <= 0) return SWEM_OK;
+    if ((Nc + 63) / 64 > 65535 * 16 || (M + 63) / 64 > 65535) return set_error(SWEM_EINVAL, "sgemm: %d x %d too large", M, Nc);
+    dim3 grid((Nc + 63) / 64, (M + 63) / 64), block(256);
+    hipLaunchKernelGGL(sgemm_nn, grid, block, 0, st, M, Nc, K, A, lda, B, ldb, C, ldc);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
+// out (+)= sum over (shot, group) of partial[shot][g][iz][x], Kahan-compensated, fixed order.
+__global__ __launch_bounds__(256) void reduce_image(const float* __restrict__ partial, int nshot, int ngroups, int nz,
+                                                    int nmx, float* __restrict__ out, int accumulate, int per_shot)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iz = blockIdx.y;
+    const int s0 = per_shot ? (int)blockIdx.z : 0;
+    const int s1 = per_shot ? s0 + 1 : nshot;
+    if (x >= nmx) return;
+    float sum = 0.f, comp = 0.f;
+    for (int s = s0; s < s1; ++s) {
+        const float* p = partial + ((size_t)s * ngroups * nz + iz) * nmx + x;
+        for (int g = 0; g < ngroups; ++g) {
+            const float y = p[(size_t)g * nz * nmx] - comp;
+            const float t = sum + y;
+            comp = (t - sum) - y;
+            sum = t;
+        }
+    }
+    const size_t o = per_shot ? ((size_t)s0 * nmx + x) * nz + iz : (size_t)x * nz + iz;
+    out[o] = accumulate ? out[o] + sum : sum;
+}
+
+int launch_reduce_image(const float* partial, int nshot, int ngroups, int nz, int nmx, float* out, int accumulate,
+                        int per_shot, hipStream_t st)
+{
+    dim3 grid((nmx + 255) / 256, nz, per_shot ? nshot : 1), block(256);
+    hipLaunchKernelGGL(reduce_image, grid, block, 0, st, partial, nshot, ngroups, nz, nmx, out, accumulate, per_shot);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
+__global__ __launch_bounds__(256) void transpose_k(const float* __restrict__ in, int rows, int cols,
+                                                   float* __restrict__ out)
+{
+    __shared__ float tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int i = ty; i < 32; i += 8) {
+        const int r = r0 + i, c = c0 + tx;
+        tile[i][tx] = (r < rows && c < cols) ? in[(size_t)r * cols + c] : 0.f;
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int c = c0 + i, r = r0 + tx;
+        if (c < cols && r < rows) out[(size_t)c * rows + r] = tile[tx][i];
+    }
+}
+
+int launch_transpose(const float* in, int rows, int cols, float* out, hipStream_t st)
+{
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32), block(256);
+    hipLaunchKernelGGL(transpose_k, grid, block, 0, st, in, rows, cols, out);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
+}  // namespace swem

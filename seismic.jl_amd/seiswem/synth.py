@@ -1,0 +1,83 @@
+"""Synthetic inputs of the migration workloads (velocity, reflectivity, wavelet, geometry).
+
+The record of every workload is produced by the library's own demigration of the reflectivity,
+as the workloads prescribe; these generators only build the models.  Seeded with
+numpy.random.default_rng.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .wem import WemGeometry
+
+
+def ricker(dt: float = 0.002, f0: float = 20.0) -> np.ndarray:
+    """Ricker wavelet (src/Wavelets/Ricker.jl:21-29): nc = floor(1/(f0 dt)), t = dt*(-nc:nc),
+    w = (1 - 2b) exp(-b), b = (pi f0 t)^2."""
+    nw = 2.0 / (f0 * dt)
+    nc = int(np.floor(nw / 2))
+    t = dt * np.arange(-nc, nc + 1)
+    b = (np.pi * f0 * t) ** 2
+    return (1.0 - 2.0 * b) * np.exp(-b)
+
+
+def wavelet_trace(nt: int, dt: float, f0: float) -> np.ndarray:
+    """Ricker wavelet zero-padded to nt samples (the wavelet file the driver reads nt from)."""
+    w = ricker(dt, f0)
+    out = np.zeros(nt, np.float32)
+    n = min(nt, w.size)
+    out[:n] = w[:n]
+    return out
+
+
+def config1():
+    """2-D SSF, reference CPU path: nx=256, dx=10; nz=200, dz=10; nt=1000, dt=4 ms; 2-60 Hz;
+    v(z)=1500+0.5 z; reflectors at 500/1000/1500 m with amplitudes 0.1/-0.08/0.12; one shot at
+    x=1280 m; 20 Hz Ricker."""
+    g = WemGeometry(nt=1000, dt=0.004, nmx=256, dmx=10.0, nz=200, dz=10.0, fmin=2.0, fmax=60.0, pspi=False)
+    z = np.arange(g.nz) * g.dz
+    vel = np.repeat((1500.0 + 0.5 * z)[None, :], g.nmx, axis=0).astype(np.float32)
+    refl = np.zeros((g.nm, g.nz), np.float32)
+    for zr, a in ((500.0, 0.1), (1000.0, -0.08), (1500.0, 0.12)):
+        refl[:, int(round(zr / g.dz))] = a
+    wav = wavelet_trace(g.nt, g.dt, 20.0)
+    return g, vel, refl, wav, np.array([1280.0], np.float32)
+
+
+def blob_velocity(nx: int, nz: int, dx: float, dz: float, rng, nblobs: int = 12, v0: float = 1800.0) -> np.ndarray:
+    """v0 plus Gaussian blobs A exp(-r^2/(2 sigma^2)), A~U(-800,800), sigma~U(100,300) m,
+    centres uniform in the model, clipped to [1500, 4000] m/s.  Layout [nx, nz]."""
+    x = np.arange(nx) * dx
+    z = np.arange(nz) * dz
+    X, Z = np.meshgrid(x, z, indexing="ij")
+    v = np.full((nx, nz), v0)
+    for _ in range(nblobs):
+        a = rng.uniform(-800.0, 800.0)
+        s = rng.uniform(100.0, 300.0)
+        x0 = rng.uniform(0.0, x[-1])
+        z0 = rng.uniform(0.0, z[-1])
+        v += a * np.exp(-((X - x0) ** 2 + (Z - z0) ** 2) / (2.0 * s * s))
+    return np.clip(v, 1500.0, 4000.0).astype(np.float32)
+
+
+def random_reflectivity(nx: int, nz: int, rng, density: float = 0.05, amp: float = 0.1) -> np.ndarray:
+    mask = rng.uniform(size=(nx, nz)) < density
+    return np.where(mask, rng.uniform(-amp, amp, size=(nx, nz)), 0.0).astype(np.float32)
+
+
+def config2(nshot: int = 64, seed: int = 2, nx: int = 1024, nz: int = 500, nt: int = 2000, fmax: float = 80.0):
+    """2-D PSPI: nx=1024, dx=10; nz=500, dz=5; nt=2000, dt=2 ms; 4 reference velocities;
+    1800 m/s + 12 Gaussian blobs; 5 % random reflectivity; 64 shots 160 m apart; 25 Hz Ricker."""
+    g = WemGeometry(nt=nt, dt=0.002, nmx=nx, dmx=10.0, nz=nz, dz=5.0, fmin=0.0, fmax=fmax, nref=4, pspi=True)
+    rng = np.random.default_rng(seed)
+    vel = blob_velocity(g.nmx, g.nz, g.dmx, g.dz, rng)
+    refl = random_reflectivity(g.nmx, g.nz, rng)
+    wav = wavelet_trace(g.nt, g.dt, 25.0)
+    sx = (80.0 + 160.0 * np.arange(nshot)).astype(np.float32)
+    sx = sx[sx < g.nmx * g.dmx]
+    return g, vel, refl, wav, sx
+
+
+def config5_shots() -> np.ndarray:
+    """Config-1 geometry with 32 shots at sx = 40 + 80 k."""
+    return (40.0 + 80.0 * np.arange(32)).astype(np.float32)
