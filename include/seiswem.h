@@ -41,6 +41,12 @@ extern "C" {
                                 compensated image sums (default) */
 #define SWEM_PREC_FP32 1     /* naive FP32: s, sqrt and phase in float like wem.c:341-343 */
 
+/* variant of the 2-D depth-march kernels */
+#define SWEM_VARIANT_AUTO 0
+#define SWEM_VARIANT_SUBGROUP 1  /* nkx/8 threads per wavefield, ping-pong LDS, barrier per pass */
+#define SWEM_VARIANT_WAVE 2      /* one 64-lane wave per wavefield, no barrier inside a transform */
+#define SWEM_VARIANT_SUBGROUP16 3 /* nkx/16 threads per wavefield, radix-16 passes (nkx >= 1024) */
+
 /* Geometry of one migration problem: the scalar arguments of wem() (wem.c:3-11). */
 typedef struct swem_geometry {
     int nt;   float ot, dt;
@@ -60,6 +66,10 @@ typedef struct swem_options {
     int freqs_per_block;    /* frequencies marched together by one workgroup; 0 = autotuned */
     int verbose;            /* progress to stderr like progress_msg (wem.c:516-520) */
     size_t scratch_bytes;   /* device scratch budget for one launch chunk; 0 = default */
+    int variant;            /* SWEM_VARIANT_* (default AUTO) */
+    size_t table_bytes;     /* device memory for the phase tables; 0 = min(32 GiB, free/4).
+                               A band whose tables exceed it is marched in frequency chunks,
+                               the tables of each chunk rebuilt per call. */
 } swem_options;
 
 /* Per-call timing of the last plan call (HIP events on the plan's stream). */

@@ -6,10 +6,14 @@
 // b + r*P/R of the strided mapping -- twiddles them, runs an R-point DFT in registers and
 // writes the self-sorting Stockham position (j/Ns)*Ns*R + j%Ns + r*Ns to LDS.  The last pass
 // (Ns*R == N) writes back to the same register slots, so both the input and the output of a
-// transform are in the strided mapping and the output is in natural order: forward FFT
-// leaves X[k = t + q*T] in v[q].  A transform therefore costs (passes - 1) LDS round trips.
-// LDS indices are padded by one complex every 8 to spread the stride-R stores over banks.
-// NW wavefields are transformed together (shared twiddles, shared barriers).
+// transform are in the strided mapping and the output is in natural order: the forward FFT
+// leaves X[k = t + q*T] in v[q].  A transform costs (passes - 1) LDS round trips.
+//
+// LDS: each subgroup owns two buffers of LdsLen<N> complex and alternates between them
+// (parity `pp`, carried across calls), so one barrier per round trip suffices: a buffer written
+// in round trip k is rewritten in round trip k+2, after every thread has passed barrier k+1 and
+// hence finished reading it.  Indices are padded by one complex every 8 to spread the
+// stride-R stores over the banks.
 // Sign convention as FFTW: SIGN = -1 forward exp(-i...), SIGN = +1 backward, unnormalised.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -38,11 +42,6 @@ __device__ __forceinline__ float2 mul_si(float2 a)
 
 template <int R, int SIGN>
 struct Dft;
-
-template <int SIGN>
-struct Dft<1, SIGN> {
-    static __device__ __forceinline__ void run(float2*) {}
-};
 
 template <int SIGN>
 struct Dft<2, SIGN> {
@@ -103,26 +102,22 @@ struct Dft<16, SIGN> {
         }
         Dft<8, SIGN>::run(e);
         Dft<8, SIGN>::run(o);
-        // W16^k = cos(pi k/8) + SIGN i sin(pi k/8)
+        // X_k = E_k + W16^k O_k, X_{k+8} = E_k - W16^k O_k, W16^k = cos(pi k/8) + SIGN i sin(pi k/8)
         const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
-        const float c2 = 0.70710678118654752440f;
+        const float r = 0.70710678118654752440f;
+        float2 t[8];
+        t[0] = o[0];
+        t[1] = cmul(o[1], make_float2(c1, SIGN * s1));
+        t[2] = cscale(cadd(o[2], mul_si<SIGN>(o[2])), r);
+        t[3] = cmul(o[3], make_float2(s1, SIGN * c1));
+        t[4] = mul_si<SIGN>(o[4]);
+        t[5] = cmul(o[5], make_float2(-s1, SIGN * c1));
+        t[6] = cscale(csub(mul_si<SIGN>(o[6]), o[6]), r);
+        t[7] = cmul(o[7], make_float2(-c1, SIGN * s1));
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            float cr, ci;
-            switch (k) {
-            case 0: cr = 1.f; ci = 0.f; break;
-            case 1: cr = c1; ci = s1; break;
-            case 2: cr = c2; ci = c2; break;
-            case 3: cr = s1; ci = c1; break;
-            case 4: cr = 0.f; ci = 1.f; break;
-            case 5: cr = -s1; ci = c1; break;
-            case 6: cr = -c2; ci = c2; break;
-            default: cr = -c1; ci = s1; break;
-            }
-            float2 w = make_float2(cr, SIGN < 0 ? -ci : ci);
-            float2 t = k == 0 ? o[0] : cmul(o[k], w);
-            v[k] = cadd(e[k], t);
-            v[k + 8] = csub(e[k], t);
+            v[k] = cadd(e[k], t[k]);
+            v[k + 8] = csub(e[k], t[k]);
         }
     }
 };
@@ -132,60 +127,6 @@ template <int N>
 struct LdsLen {
     static constexpr int value = N + (N >> 3);
 };
-
-// One Stockham pass and the passes after it.  v: NW wavefields in the strided mapping.
-// lds: NW buffers of LdsLen<N> complex each, private to the subgroup.  tw[m] = exp(-2 pi i m/N).
-template <int N, int P, int NW, int SIGN, int NS = 1>
-__device__ __forceinline__ void fft(float2 (&v)[NW][P], float2* lds, const float2* __restrict__ tw, int t)
-{
-    constexpr int T = N / P;
-    constexpr int REM = N / NS;
-    constexpr int R = REM >= P ? P : REM;
-    constexpr int B = P / R;
-    constexpr bool LAST = (NS * R == N);
-    constexpr int L = LdsLen<N>::value;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const int j = t + b * T;
-        const int k = j & (NS - 1);
-        float2 w[R];
-        if constexpr (NS > 1) {
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                float2 x = tw[r * k * (N / (NS * R))];
-                w[r] = SIGN > 0 ? make_float2(x.x, -x.y) : x;
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < NW; ++a) {
-            float2 x[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) x[r] = v[a][b + r * B];
-            if constexpr (NS > 1) {
-#pragma unroll
-                for (int r = 1; r < R; ++r) x[r] = cmul(x[r], w[r]);
-            }
-            Dft<R, SIGN>::run(x);
-            if constexpr (LAST) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) v[a][b + r * B] = x[r];
-            } else {
-                const int base = (j - k) * R + k;
-#pragma unroll
-                for (int r = 0; r < R; ++r) lds[a * L + lds_pad(base + r * NS)] = x[r];
-            }
-        }
-    }
-    if constexpr (!LAST) {
-        __syncthreads();
-#pragma unroll
-        for (int a = 0; a < NW; ++a)
-#pragma unroll
-            for (int q = 0; q < P; ++q) v[a][q] = lds[a * L + lds_pad(t + q * T)];
-        __syncthreads();
-        fft<N, P, NW, SIGN, NS * R>(v, lds, tw, t);
-    }
-}
 
 // Number of radix passes of fft<N,P> (for flop accounting and tests).
 template <int N, int P>
@@ -198,6 +139,113 @@ constexpr int fft_passes()
         ++p;
     }
     return p;
+}
+
+// One Stockham pass and the passes after it on one wavefield v in the strided mapping.
+// lds: the subgroup's two buffers (2 * LdsLen<N> complex); pp: buffer parity, toggled per
+// round trip.  tw[m] = exp(-2 pi i m/N).
+template <int N, int P, int SIGN, int NS = 1>
+__device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* __restrict__ tw, int t, int& pp)
+{
+    constexpr int T = N / P;
+    constexpr int REM = N / NS;
+    constexpr int R = REM >= P ? P : REM;
+    constexpr int B = P / R;
+    constexpr bool LAST = (NS * R == N);
+    constexpr int L = LdsLen<N>::value;
+    float2* buf = lds + pp * L;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = t + b * T;
+        const int k = j & (NS - 1);
+        float2 x[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[r] = v[b + r * B];
+        if constexpr (NS > 1) {
+            // The twiddle index is made opaque to the optimiser so the loads stay in this pass
+            // instead of being hoisted out of the depth loop, where 21 complex twiddles per
+            // transform would pin 42 registers for the whole kernel.
+            int ti = k * (N / (NS * R));
+            asm volatile("" : "+v"(ti));
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const float2 w = tw[r * ti];
+                x[r] = cmul(x[r], SIGN > 0 ? make_float2(w.x, -w.y) : w);
+            }
+        }
+        Dft<R, SIGN>::run(x);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b + r * B] = x[r];
+        } else {
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[lds_pad(base + r * NS)] = x[r];
+        }
+    }
+    if constexpr (!LAST) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = buf[lds_pad(t + q * T)];
+        pp ^= 1;
+        fft<N, P, SIGN, NS * R>(v, lds, tw, t, pp);
+    }
+}
+
+
+// Wave-level variant: one 64-lane wave transforms N = 64 P points held as v[q] = x[t + 64 q].
+// All exchanges stay inside the wave, whose LDS instructions execute in program order, so a
+// read issued after a write observes it and one buffer (LdsLen<N> complex, private to the wave)
+// suffices with no s_barrier; wave_barrier only stops the compiler from reordering the LDS
+// accesses across the exchange.
+template <int N, int SIGN, int NS = 1>
+__device__ __forceinline__ void wfft(float2 (&v)[N / 64], float2* buf, const float2* __restrict__ tw, int t)
+{
+    constexpr int P = N / 64;
+    constexpr int T = 64;
+    constexpr int REM = N / NS;
+    constexpr int RMAX = P < 8 ? P : 8;
+    constexpr int R = REM >= RMAX ? RMAX : REM;
+    constexpr int B = P / R;
+    constexpr bool LAST = (NS * R == N);
+    static_assert(P % R == 0, "radix must divide the points per lane");
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = t + b * T;
+        const int k = j & (NS - 1);
+        float2 x[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[r] = v[b + r * B];
+        if constexpr (NS > 1) {
+            int ti = k * (N / (NS * R));
+            asm volatile("" : "+v"(ti));
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const float2 w = tw[r * ti];
+                x[r] = cmul(x[r], SIGN > 0 ? make_float2(w.x, -w.y) : w);
+            }
+        }
+        Dft<R, SIGN>::run(x);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b + r * B] = x[r];
+        } else {
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[lds_pad(base + r * NS)] = x[r];
+        }
+    }
+    if constexpr (!LAST) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = buf[lds_pad(t + q * T)];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wfft<N, SIGN, NS * R>(v, buf, tw, t);
+    }
 }
 
 }  // namespace swem

@@ -85,12 +85,14 @@ struct swem_plan {
     swem_geometry g{};
     swem_options opt{};
     Sampling s{};
-    int nm = 0, nf = 0, nkx = 0, T = 0, F = 1, ngroups = 0, mode = 0;
+    int nm = 0, nf = 0, nkx = 0, T = 0, F = 1, Ffwd = 1, mode = 0;
+    // frequency chunks: [c * nfc, min(nf, (c + 1) * nfc)); the phase tables hold one chunk
+    int nfc = 0, nchunks = 1, resident = -1;
     std::vector<float> vel;  // host copy (source of every table)
     // tables
-    DevBuf<float2> tw, spec_s;
-    DevBuf<float> pdpi, taper, p2f, w2, basis_adj, basis_fwd;
-    DevBuf<double> p2;
+    DevBuf<float2> tw, spec_s, ktab, stab;
+    DevBuf<float> taper, p2f, pdf, w2, basis_adj, basis_fwd;
+    DevBuf<double> p2, pd;
     DevBuf<uint8_t> sel, sel2;
     DevBuf<uint32_t> used;
     // work
@@ -123,14 +125,15 @@ static double step_flops(const swem_plan* p)
     return 2.0 * fw + 4.0 * nm;
 }
 
-static int choose_F(const swem_plan* p)
+// Frequencies per migration workgroup (2F subgroups of T threads) or subgroups per modelling
+// workgroup (one frequency each).  Preferred: at most 512 threads and 80 KiB of LDS so two
+// workgroups share a CU; fewer when a single shot would leave CUs idle.
+static int choose_group(const swem_plan* p, int per_freq, int forced)
 {
-    if (p->opt.freqs_per_block > 0) return p->opt.freqs_per_block;
-    // Enough workgroups to fill 256 CUs for a single shot, otherwise the largest F that
-    // fits 1024 threads and the LDS (fewer partial images to write and reduce).
-    int F = std::max(1, march2d_max_block() / p->T);
-    while (F > 1 && march2d_lds_bytes(p->nkx, F) > 160 * 1024) F /= 2;
-    while (F > 1 && (p->nf + F - 1) / F < 256) F /= 2;
+    if (forced > 0) return forced;
+    int F = std::max(1, std::min(512, march2d_max_block(p->nkx, p->opt.variant)) / (p->T * per_freq));
+    while (F > 1 && march2d_lds_bytes(p->nkx, F * per_freq, p->opt.variant) > 80 * 1024) F--;
+    while (F > 1 && (p->nf + F - 1) / F < 512) F--;
     return F;
 }
 
@@ -149,24 +152,26 @@ static int build_tables(swem_plan* p, const float* wav)
     std::vector<float> tap(nmx);
     taper_f32(nmx, p->s.lmx, tap.data());
     SWEM_TRY(p->taper.upload(tap.data(), tap.size()));
-    // slowness split (wem.c:27-36): FP64 for the propagation, float copy for naive FP32
+    // slowness split (wem.c:27-36): FP64 for the propagation, float copy for naive FP32;
+    // depth-major [nz][nmx] for the split-step table
     std::vector<double> po(nz);
-    std::vector<float> pof(nz), pdf((size_t)nm * nz), pdpi((size_t)nz * nmx);
-    slowness_f32(g, p->vel.data(), pof.data(), pdf.data());
-    for (int iz = 0; iz < nz; iz++) {
-        double sum = 0;
-        for (int ix = 0; ix < nm; ix++) sum += p->vel[(size_t)ix * nz + iz];
-        po[iz] = 1.0 / (sum / (double)nm);
-        for (int ix = 0; ix < nmx; ix++) {
-            if (p->opt.precision == SWEM_PREC_FP32)
-                pdpi[(size_t)iz * nmx + ix] = (float)((double)pdf[(size_t)ix * nz + iz] * g.dz / kPi);
-            else
-                pdpi[(size_t)iz * nmx + ix] =
-                    (float)((1.0 / (double)p->vel[(size_t)ix * nz + iz] - po[iz]) * (double)g.dz / kPi);
-        }
-    }
-    SWEM_TRY(p->pdpi.upload(pdpi.data(), pdpi.size()));
+    std::vector<float> pof(nz), pdf_xz((size_t)nm * nz);
+    slowness_f32(g, p->vel.data(), pof.data(), pdf_xz.data());
     if (!g.pspi) {
+        std::vector<double> pd((size_t)nz * nmx);
+        std::vector<float> pdf((size_t)nz * nmx);
+        for (int iz = 0; iz < nz; iz++) {
+            double sum = 0;
+            for (int ix = 0; ix < nm; ix++) sum += p->vel[(size_t)ix * nz + iz];
+            po[iz] = 1.0 / (sum / (double)nm);
+            for (int ix = 0; ix < nmx; ix++) {
+                pd[(size_t)iz * nmx + ix] = 1.0 / (double)p->vel[(size_t)ix * nz + iz] - po[iz];
+                pdf[(size_t)iz * nmx + ix] = pdf_xz[(size_t)ix * nz + iz];
+            }
+        }
+        SWEM_TRY(p->pd.upload(pd.data(), pd.size()));
+        SWEM_TRY(p->pdf.upload(pdf.data(), pdf.size()));
+
         std::vector<double> p2(nz);
         std::vector<float> p2f(nz);
         for (int iz = 0; iz < nz; iz++) {
@@ -255,6 +260,65 @@ static int build_tables(swem_plan* p, const float* wav)
     return SWEM_OK;
 }
 
+// Bytes of the phase tables of one frequency.
+static size_t table_bytes_per_freq(const swem_plan* p)
+{
+    const size_t nref = p->g.pspi ? p->g.nref : 1, nkh = p->nkx / 2 + 1;
+    return sizeof(float2) * (size_t)p->g.nz * (nref * nkh + (p->g.pspi ? 0 : (size_t)p->g.nmx));
+}
+
+// Builds the phase tables of frequency chunk c (no-op when they are resident).
+static int ensure_chunk(swem_plan* p, int c, hipStream_t st)
+{
+    if (p->resident == c) return SWEM_OK;
+    PhaseTableArgs t{};
+    t.f0 = c * p->nfc;
+    t.nfc = std::min(p->nfc, p->nf - t.f0);
+    t.ifmin = p->s.ifmin;
+    t.nz = p->g.nz;
+    t.nref = p->g.pspi ? p->g.nref : 1;
+    t.nkh = p->nkx / 2 + 1;
+    t.nmx = p->g.nmx;
+    t.prec = p->opt.precision;
+    t.dw = 2.0 * kPi / ((double)p->s.ntfft * (double)p->g.dt);
+    t.dkx = 2.0 * kPi / ((double)p->s.nkx * (double)p->g.dmx);
+    t.dz = p->g.dz;
+    t.inv_nk = 1.0 / (double)p->s.nkx;
+    t.dwf = p->s.dw;
+    t.dkxf = p->s.dkx;
+    t.p2 = p->p2.p;
+    t.p2f = p->p2f.p;
+    t.pd = p->pd.p;
+    t.pdf = p->pdf.p;
+    t.ktab = p->ktab.p;
+    t.stab = p->stab.p;
+    p->resident = -1;
+    SWEM_TRY(launch_phase_tables(t, !p->g.pspi, st));
+    p->resident = c;
+    return SWEM_OK;
+}
+
+static int alloc_tables(swem_plan* p)
+{
+    const size_t per = table_bytes_per_freq(p);
+    size_t cap = p->opt.table_bytes;
+    if (!cap) {
+        size_t fr = 0, tot = 0;
+        cap = hipMemGetInfo(&fr, &tot) == hipSuccess ? std::min((size_t)32 << 30, fr / 4) : (size_t)8 << 30;
+    }
+    int nfc = (int)std::min<size_t>(p->nf, cap / per);
+    if (nfc < p->nf) nfc = std::max(p->F, std::max(p->Ffwd, (nfc / (p->F * p->Ffwd)) * p->F * p->Ffwd));
+    if (nfc < 1) nfc = 1;
+    p->nfc = std::min(nfc, p->nf);
+    p->nchunks = (p->nf + p->nfc - 1) / p->nfc;
+    const size_t nref = p->g.pspi ? p->g.nref : 1, nkh = p->nkx / 2 + 1;
+    SWEM_TRY(p->ktab.alloc((size_t)p->nfc * p->g.nz * nref * nkh));
+    if (!p->g.pspi) SWEM_TRY(p->stab.alloc((size_t)p->nfc * p->g.nz * p->g.nmx));
+    SWEM_TRY(ensure_chunk(p, 0, nullptr));
+    SWEM_HIP(hipDeviceSynchronize());
+    return SWEM_OK;
+}
+
 static size_t default_budget()
 {
     size_t fr = 0, tot = 0;
@@ -302,33 +366,30 @@ static int set_igx(swem_plan* p, const float* sx, const float* sy, int nshot, hi
     return SWEM_OK;
 }
 
-static March2DArgs base_args(const swem_plan* p)
+// Arguments of frequency chunk c; F = frequencies (or subgroups) per workgroup.
+static March2DArgs base_args(const swem_plan* p, int c, int F)
 {
     March2DArgs a{};
     a.nmx = p->g.nmx;
     a.nz = p->g.nz;
     a.nf = p->nf;
     a.nref = p->g.pspi ? p->g.nref : 1;
-    a.ngroups = p->ngroups;
+    a.f0 = c * p->nfc;
+    a.nfc = std::min(p->nfc, p->nf - a.f0);
+    a.ngroups = (a.nfc + F - 1) / F;
+    a.nkh = p->nkx / 2 + 1;
     a.ifmin = p->s.ifmin;
     a.oz = p->g.oz;
     a.dz = p->g.dz;
     a.sz = p->g.sz;
     a.gz = p->g.gz;
-    a.dw = 2.0 * kPi / ((double)p->s.ntfft * (double)p->g.dt);
-    a.dwf = p->s.dw;
-    a.dkx = 2.0 * kPi / ((double)p->s.nkx * (double)p->g.dmx);
-    a.dkxf = p->s.dkx;
-    a.dz_pi = (double)p->g.dz / kPi;
-    a.evan = -0.2f * std::fabs(p->g.dz);
     a.inv_nk = 1.0f / (float)p->s.nkx;
     a.tw = p->tw.p;
+    a.ktab = p->ktab.p;
+    a.stab = p->stab.p;
     a.spec_s = p->spec_s.p;
     a.igx = p->igx.p;
-    a.pdpi = p->pdpi.p;
     a.taper = p->taper.p;
-    a.p2 = p->p2.p;
-    a.p2f = p->p2f.p;
     a.sel = p->sel.p;
     a.sel2 = p->sel2.p;
     a.w2 = p->w2.p;
@@ -348,10 +409,11 @@ static int migrate_impl(swem_plan* p, const float* d_dev, const float* sx, const
     if (!p || !d_dev || !sx || !m_dev || nshot < 1) return set_error(SWEM_EINVAL, "migrate: null argument or nshot < 1");
     SWEM_HIP(hipSetDevice(p->opt.device));
     const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
-    const size_t per = nm * nf * sizeof(float2) + (size_t)p->ngroups * nz * p->g.nmx * sizeof(float);
+    const size_t gmax = (size_t)(p->nfc + p->F - 1) / p->F;
+    const size_t per = nm * nf * sizeof(float2) + gmax * nz * p->g.nmx * sizeof(float);
     int chunk = (int)std::max<size_t>(1, std::min<size_t>(nshot, p->budget / std::max<size_t>(per, 1)));
     SWEM_TRY(p->spec_r.alloc((size_t)chunk * nm * nf));
-    SWEM_TRY(p->partial.alloc((size_t)chunk * p->ngroups * nz * p->g.nmx));
+    SWEM_TRY(p->partial.alloc((size_t)chunk * gmax * nz * p->g.nmx));
     SWEM_TRY(set_igx(p, sx, sy, nshot, st));
     p->nev = 0;
     p->last_launches = 0;
@@ -363,19 +425,25 @@ static int migrate_impl(swem_plan* p, const float* d_dev, const float* sx, const
         SWEM_TRY(launch_sgemm(ns * (int)nm, 2 * (int)nf, (int)nt, d_dev + (size_t)s0 * nm * nt, (int)nt, p->basis_adj.p,
                               2 * (int)nf, reinterpret_cast<float*>(p->spec_r.p), 2 * (int)nf, st));
         SWEM_TRY(record(p, 0, st));
-        March2DArgs a = base_args(p);
-        a.nshot = ns;
-        a.igx = p->igx.p + s0;
-        a.spec_r = p->spec_r.p;
-        a.partial = p->partial.p;
-        SWEM_TRY(launch_march2d(a, p->nkx, p->F, mode_of(p), p->opt.precision, 1, st));
-        SWEM_TRY(record(p, 1, st));
-        p->last_launches++;
-        p->last_flops += step_flops(p) * (double)nf * nz * ns;
-        float* out = per_shot ? m_dev + (size_t)s0 * nm * nz : m_dev;
-        SWEM_TRY(launch_reduce_image(p->partial.p, ns, p->ngroups, (int)nz, p->g.nmx, out,
-                                     per_shot ? 0 : (accumulate || s0 > 0), per_shot, st));
-        SWEM_TRY(record(p, 2, st));
+        for (int c = 0; c < p->nchunks; c++) {
+            if (p->resident != c) {
+                SWEM_TRY(ensure_chunk(p, c, st));
+                SWEM_TRY(record(p, 0, st));
+            }
+            March2DArgs a = base_args(p, c, p->F);
+            a.nshot = ns;
+            a.igx = p->igx.p + s0;
+            a.spec_r = p->spec_r.p;
+            a.partial = p->partial.p;
+            SWEM_TRY(launch_march2d(a, p->nkx, p->F, mode_of(p), 1, p->opt.variant, st));
+            SWEM_TRY(record(p, 1, st));
+            p->last_launches++;
+            p->last_flops += step_flops(p) * (double)a.nfc * nz * ns;
+            float* out = per_shot ? m_dev + (size_t)s0 * nm * nz : m_dev;
+            const int acc = c > 0 || (!per_shot && (accumulate || s0 > 0));
+            SWEM_TRY(launch_reduce_image(p->partial.p, ns, a.ngroups, (int)nz, p->g.nmx, out, acc, per_shot, st));
+            SWEM_TRY(record(p, 2, st));
+        }
     }
     return SWEM_OK;
 }
@@ -386,10 +454,10 @@ static int model_impl(swem_plan* p, const float* m_dev, const float* sx, const f
     if (!p || !d_dev || !sx || !m_dev || nshot < 1) return set_error(SWEM_EINVAL, "model: null argument or nshot < 1");
     SWEM_HIP(hipSetDevice(p->opt.device));
     const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
-    const size_t per = nm * nf * sizeof(float2) + nf * nz * p->g.nmx * sizeof(float2);
+    const size_t per = nm * nf * sizeof(float2) + (size_t)p->nfc * nz * p->g.nmx * sizeof(float2);
     int chunk = (int)std::max<size_t>(1, std::min<size_t>(nshot, p->budget / std::max<size_t>(per, 1)));
     SWEM_TRY(p->dgw.alloc((size_t)chunk * nm * nf));
-    SWEM_TRY(p->smig.alloc((size_t)chunk * nf * nz * p->g.nmx));
+    SWEM_TRY(p->smig.alloc((size_t)chunk * p->nfc * nz * p->g.nmx));
     SWEM_TRY(p->mT.alloc(nm * nz));
     SWEM_TRY(set_igx(p, sx, sy, nshot, st));
     p->nev = 0;
@@ -401,16 +469,22 @@ static int model_impl(swem_plan* p, const float* m_dev, const float* sx, const f
     SWEM_TRY(record(p, 0, st));
     for (int s0 = 0; s0 < nshot; s0 += chunk) {
         const int ns = std::min(chunk, nshot - s0);
-        March2DArgs a = base_args(p);
-        a.nshot = ns;
-        a.igx = p->igx.p + s0;
-        a.mT = p->mT.p;
-        a.smig = p->smig.p;
-        a.dgw = p->dgw.p;
-        SWEM_TRY(launch_march2d(a, p->nkx, p->F, mode_of(p), p->opt.precision, 0, st));
-        SWEM_TRY(record(p, 1, st));
-        p->last_launches++;
-        p->last_flops += step_flops(p) * (double)nf * nz * ns;
+        for (int c = 0; c < p->nchunks; c++) {
+            if (p->resident != c) {
+                SWEM_TRY(ensure_chunk(p, c, st));
+                SWEM_TRY(record(p, 0, st));
+            }
+            March2DArgs a = base_args(p, c, p->Ffwd);
+            a.nshot = ns;
+            a.igx = p->igx.p + s0;
+            a.mT = p->mT.p;
+            a.smig = p->smig.p;
+            a.dgw = p->dgw.p;
+            SWEM_TRY(launch_march2d(a, p->nkx, p->Ffwd, mode_of(p), 0, p->opt.variant, st));
+            SWEM_TRY(record(p, 1, st));
+            p->last_launches++;
+            p->last_flops += step_flops(p) * (double)a.nfc * nz * ns;
+        }
         SWEM_TRY(launch_sgemm(ns * (int)nm, (int)nt, 2 * (int)nf, reinterpret_cast<const float*>(p->dgw.p), 2 * (int)nf,
                               p->basis_fwd.p, (int)nt, d_dev + (size_t)s0 * nm * nt, (int)nt, st));
         SWEM_TRY(record(p, 2, st));
@@ -460,6 +534,7 @@ int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav,
         return set_error(SWEM_EINVAL, "pspi_mode %d", o.pspi_mode);
     if (o.precision != SWEM_PREC_KZ64 && o.precision != SWEM_PREC_FP32)
         return set_error(SWEM_EINVAL, "precision %d", o.precision);
+    if (o.variant < SWEM_VARIANT_AUTO || o.variant > SWEM_VARIANT_SUBGROUP16) return set_error(SWEM_EINVAL, "variant %d", o.variant);
     swem_plan* p = new swem_plan();
     p->g = *g;
     p->opt = o;
@@ -478,17 +553,21 @@ int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav,
     if (!(g->dz != 0.f)) { delete p; return set_error(SWEM_EINVAL, "dz must be non-zero"); }
     rc = check_device(o.device);
     if (rc) { delete p; return rc; }
-    p->T = march2d_threads(p->nkx);
-    p->F = choose_F(p);
-    if (p->F < 1 || p->F * p->T > march2d_max_block() || march2d_lds_bytes(p->nkx, p->F) > 160 * 1024) {
+    p->T = march2d_threads(p->nkx, o.variant);
+    p->F = choose_group(p, 2, o.freqs_per_block);
+    p->Ffwd = choose_group(p, 1, o.freqs_per_block);
+    if (2 * p->F * p->T > march2d_max_block(p->nkx, o.variant) ||
+        march2d_lds_bytes(p->nkx, 2 * p->F, o.variant) > 160 * 1024 ||
+        p->Ffwd * p->T > march2d_max_block(p->nkx, o.variant) ||
+        march2d_lds_bytes(p->nkx, p->Ffwd, o.variant) > 160 * 1024) {
         delete p;
         return set_error(SWEM_EINVAL, "freqs_per_block=%d does not fit a workgroup at nkx=%d", o.freqs_per_block, p->s.nkx);
     }
-    p->ngroups = (p->nf + p->F - 1) / p->F;
     p->mode = mode_of(p);
     p->vel.assign(vel, vel + (size_t)p->nm * g->nz);
     p->budget = o.scratch_bytes ? o.scratch_bytes : default_budget();
     rc = build_tables(p, wav);
+    if (!rc) rc = alloc_tables(p);
     if (rc) { swem_plan_destroy(p); return rc; }
     *out = p;
     return SWEM_OK;
@@ -501,7 +580,8 @@ int swem_plan_destroy(swem_plan* p)
     if (p->last_stream || p->nev) (void)hipStreamSynchronize(p->last_stream);
     for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
     if (p->igx_pinned) (void)hipHostFree(p->igx_pinned);
-    p->tw.release(); p->spec_s.release(); p->pdpi.release(); p->taper.release(); p->p2f.release(); p->w2.release();
+    p->tw.release(); p->spec_s.release(); p->ktab.release(); p->stab.release(); p->pd.release(); p->pdf.release();
+    p->taper.release(); p->p2f.release(); p->w2.release();
     p->basis_adj.release(); p->basis_fwd.release(); p->p2.release(); p->sel.release(); p->sel2.release();
     p->used.release(); p->spec_r.release(); p->smig.release(); p->dgw.release(); p->partial.release();
     p->mT.release(); p->igx.release();
@@ -552,7 +632,8 @@ int swem_plan_bytes(const swem_plan* p, size_t* table_bytes, size_t* work_bytes)
 {
     if (!p) return set_error(SWEM_EINVAL, "plan_bytes: null plan");
     if (table_bytes)
-        *table_bytes = p->tw.bytes() + p->spec_s.bytes() + p->pdpi.bytes() + p->taper.bytes() + p->p2f.bytes() +
+        *table_bytes = p->tw.bytes() + p->spec_s.bytes() + p->ktab.bytes() + p->stab.bytes() + p->pd.bytes() +
+                       p->pdf.bytes() + p->taper.bytes() + p->p2f.bytes() +
                        p->w2.bytes() + p->basis_adj.bytes() + p->basis_fwd.bytes() + p->p2.bytes() + p->sel.bytes() +
                        p->sel2.bytes() + p->used.bytes();
     if (work_bytes)

@@ -36,43 +36,56 @@ int pspi_tables(const swem_geometry& g, const float* vel, float* vref, int* iref
 void taper_f32(int n, int l, float* t);
 void slowness_f32(const swem_geometry& g, const float* vel, float* po, float* pd);
 
-// Arguments of the 2-D depth-march kernels (one subgroup of N/P threads per frequency,
-// F subgroups per workgroup, grid = (ngroups, nshot)).
+// Arguments of the FP64 phase-table builders (swem_phase.hip).
+struct PhaseTableArgs {
+    int nfc, f0, ifmin, nz, nref, nkh, nmx, prec;
+    double dw, dkx, dz, inv_nk;
+    float dwf, dkxf;
+    const double* p2;   // [nz][nref] po^2 (SSF, nref = 1) or 1/vref^2 (PSPI)
+    const float* p2f;   // float counterparts (naive FP32)
+    const double* pd;   // [nz][nmx] slowness perturbation (SSF)
+    const float* pdf;   // float counterpart (naive FP32)
+    float2* ktab;       // [nfc][nz][nref][nkh]
+    float2* stab;       // [nfc][nz][nmx]
+};
+int launch_phase_tables(const PhaseTableArgs& a, bool ss, hipStream_t st);
+
+// Arguments of the 2-D depth-march kernels.  A launch covers the frequency chunk
+// [f0, f0 + nfc) of the band (indices relative to ifmin) for nshot shots; the block index is
+// decoded XCD-aware into (frequency group g, shot) so the shots that share a frequency run on
+// the same XCD and share its L2 copy of the phase tables.
 struct March2DArgs {
     int nmx, nz, nf, nref, ngroups, ifmin, nshot;
+    int f0, nfc, nkh;
     float oz, dz, sz, gz;
-    double dw;        // 2 pi / (ntfft dt)          (FP64 frequency step)
-    float dwf;        // reference float dw        (wem.c:85)
-    double dkx;       // 2 pi / (nkx dmx)
-    float dkxf;       // reference float dkx       (wem.c:83)
-    double dz_pi;     // dz / pi
-    float evan;       // -0.2 |dz|                  (wem.c:343)
-    float inv_nk;     // 1 / nkx                    (the two 1/sqrt(nk) of wem.c:344,354)
+    float inv_nk;           // 1 / nkx (identity steps)
     const float2* tw;       // [nkx] exp(-2 pi i m / nkx)
+    const float2* ktab;     // [nfc][nz][nref][nkh] kz phase / nk, +dz
+    const float2* stab;     // [nfc][nz][nmx] split-step phase, +dz (SSF)
     const float2* spec_s;   // [nf] source spectrum / sqrt(ntfft)
     const int* igx;         // [nshot]
     const float2* spec_r;   // [nshot][nmx][nf]  receiver spectra w^2 D / sqrt(ntfft)
-    const float* pdpi;      // [nz][nmx] pd * dz / pi (SSF)
     const float* taper;     // [nmx]
-    const double* p2;       // SSF: po^2 [nz];  PSPI: 1/vref^2 [nz][nref]
-    const float* p2f;       // float counterparts (naive FP32 precision)
     const uint8_t* sel;     // [nz][nmx] iref1 | cut << 7 (PSPI)
     const uint8_t* sel2;    // [nz][nmx] iref2 (PSPI linear)
     const float* w2;        // [nz][nmx] weight of iref2 (PSPI linear)
     const uint32_t* used;   // [nz] bit ir set when some x selects vref ir at iz
     float* partial;         // [nshot][ngroups][nz][nmx] migration partial images
     const float* mT;        // [nz][nmx] image (modelling input, depth-major)
-    float2* smig;           // [nshot][nf][nz][nmx] source wavefield scratch (modelling)
+    float2* smig;           // [nshot][nfc][nz][nmx] source wavefield scratch (modelling)
     float2* dgw;            // [nshot][nmx][nf] modelled spectra (modelling output)
 };
 
-// mode: 0 = SSF, 1 = PSPI compat, 2 = PSPI linear.  prec: SWEM_PREC_*.
-int launch_march2d(const March2DArgs& a, int nkx, int F, int mode, int prec, int adj, hipStream_t st);
-// Threads per frequency subgroup for a given nkx (0 if unsupported).
-int march2d_threads(int nkx);
+// mode: 0 = SSF, 1 = PSPI compat, 2 = PSPI linear.  variant: SWEM_VARIANT_*.
+int launch_march2d(const March2DArgs& a, int nkx, int F, int mode, int adj, int variant, hipStream_t st);
+// Number of blocks of the XCD-aware 1-D grid of a launch.
+int march2d_grid_blocks(int ngroups, int nshot);
+int march2d_resolve_variant(int nkx, int variant);
+// Threads per wavefield subgroup for a given nkx (0 if unsupported).
+int march2d_threads(int nkx, int variant);
 bool march2d_supported(int nkx);
-int march2d_max_block();
-size_t march2d_lds_bytes(int nkx, int F);
+int march2d_max_block(int nkx, int variant);
+size_t march2d_lds_bytes(int nkx, int nsub, int variant);
 
 // C[M x Nc] = A[M x K] * B[K x Nc], row-major with leading dimensions lda, ldb, ldc.
 int launch_sgemm(int M, int Nc, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc,

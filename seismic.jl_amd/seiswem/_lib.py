@@ -16,6 +16,7 @@ SHOTWEM_PATH = os.path.join(LIB_DIR, "shotwem.so")
 SWEM_OK, SWEM_EINVAL, SWEM_ENOMEM, SWEM_EDEVICE, SWEM_EIO, SWEM_EUNSUPPORTED, SWEM_EBUSY = range(7)
 PSPI_COMPAT, PSPI_LINEAR = 0, 1
 PREC_KZ64, PREC_FP32 = 0, 1
+VARIANT_AUTO, VARIANT_SUBGROUP, VARIANT_WAVE, VARIANT_SUBGROUP16 = 0, 1, 2, 3
 
 # Every symbol declared in include/seiswem.h.
 EXPORTS = (
@@ -54,6 +55,7 @@ class Options(ctypes.Structure):
     _fields_ = [
         ("pspi_mode", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int),
         ("freqs_per_block", ctypes.c_int), ("verbose", ctypes.c_int), ("scratch_bytes", ctypes.c_size_t),
+        ("variant", ctypes.c_int), ("table_bytes", ctypes.c_size_t),
     ]
 
 
@@ -142,11 +144,13 @@ def check(rc: int):
 
 
 def options(pspi_mode: int = PSPI_COMPAT, precision: int = PREC_KZ64, device: int = 0, freqs_per_block: int = 0,
-            verbose: bool = False, scratch_bytes: int = 0) -> Options:
+            verbose: bool = False, scratch_bytes: int = 0, variant: int = VARIANT_AUTO,
+            table_bytes: int = 0) -> Options:
     o = Options()
     load().swem_options_init(ctypes.byref(o))
     o.pspi_mode, o.precision, o.device = pspi_mode, precision, device
     o.freqs_per_block, o.verbose, o.scratch_bytes = freqs_per_block, int(verbose), scratch_bytes
+    o.variant, o.table_bytes = variant, table_bytes
     return o
 
 

@@ -106,7 +106,8 @@ class Plan:
     """Velocity-derived tables and the source spectrum resident on the device (swem_plan_*)."""
 
     def __init__(self, geom: WemGeometry, vel: np.ndarray, wav: np.ndarray, *, pspi_mode: int = _lib.PSPI_COMPAT,
-                 precision: int = _lib.PREC_KZ64, freqs_per_block: int = 0, scratch_bytes: int = 0, device: int = 0):
+                 precision: int = _lib.PREC_KZ64, freqs_per_block: int = 0, scratch_bytes: int = 0, device: int = 0,
+                 variant: int = _lib.VARIANT_AUTO, table_bytes: int = 0):
         self.geom = geom
         vel = np.ascontiguousarray(vel, dtype=np.float32).reshape(geom.nm, geom.nz)
         wav = np.ascontiguousarray(wav, dtype=np.float32).reshape(-1)
@@ -114,7 +115,7 @@ class Plan:
             wav = np.concatenate([wav, np.zeros(geom.nt - wav.size, np.float32)])
         wav = np.ascontiguousarray(wav[: geom.nt])
         g = geom.c()
-        o = _lib.options(pspi_mode, precision, device, freqs_per_block, False, scratch_bytes)
+        o = _lib.options(pspi_mode, precision, device, freqs_per_block, False, scratch_bytes, variant, table_bytes)
         p = ctypes.c_void_p()
         check(load().swem_plan_create(ctypes.byref(g), _fp(vel), _fp(wav), ctypes.byref(o), ctypes.byref(p)))
         self._p = p.value
@@ -188,7 +189,7 @@ def wem(d, m, wav, nt, ot, dt, nmx, omx, dmx, nmy, omy, dmy, sx, sy, nz, oz, dz,
 
 def ShotProfileWEM(x: np.ndarray, adj: bool = True, *, vel: np.ndarray, wav: np.ndarray, geom: WemGeometry,
                    sx=(0.0,), sy=None, pspi_mode: int = _lib.PSPI_COMPAT, precision: int = _lib.PREC_KZ64,
-                   freqs_per_block: int = 0) -> np.ndarray:
+                   freqs_per_block: int = 0, variant: int = _lib.VARIANT_AUTO) -> np.ndarray:
     """In-memory ShotProfileWEM (ShotProfileWEM.jl:39-310) with nangx = nangy = 1.
 
     adj=True:  x = d [nshot, nm, nt]  ->  stacked image m [nm, nz] (sum over shots, :140-201)
@@ -198,7 +199,7 @@ def ShotProfileWEM(x: np.ndarray, adj: bool = True, *, vel: np.ndarray, wav: np.
     sy = np.zeros_like(sx) if sy is None else np.ascontiguousarray(np.atleast_1d(sy), dtype=np.float32)
     ns = sx.size
     g = geom.c()
-    o = _lib.options(pspi_mode, precision, 0, freqs_per_block)
+    o = _lib.options(pspi_mode, precision, 0, freqs_per_block, variant=variant)
     vel = np.ascontiguousarray(vel, dtype=np.float32)
     wav = np.ascontiguousarray(wav, dtype=np.float32)
     if adj:

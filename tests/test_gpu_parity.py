@@ -33,25 +33,34 @@ def oparams(g: seiswem.WemGeometry, pspi_mode: int = 0) -> O.WemParams:
                        padx=g.padx, pspi=g.pspi, pspi_mode=pspi_mode)
 
 
-@pytest.mark.parametrize("pspi,mode", [(False, 0), (True, 0), (True, 1)])
-def test_migrate_matches_oracle(gpu, pspi, mode):
-    g, vel, wav, rng = small_case(pspi)
+VARIANTS = [seiswem.VARIANT_SUBGROUP, seiswem.VARIANT_WAVE, seiswem.VARIANT_SUBGROUP16]
+CASES = [(False, 0), (True, 0), (True, 1)]
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("nx", [64, 256, 512])
+@pytest.mark.parametrize("pspi,mode", CASES)
+def test_migrate_matches_oracle(gpu, pspi, mode, nx, variant):
+    """nx=64 pads to nkx=128 with no taper (nmx <= 100); nx=256 / 512 pad to nkx=512 / 1024 with
+    the taper (the radix-16 variant needs nkx >= 1024 and falls back below it)."""
+    g, vel, wav, rng = small_case(pspi, nx=nx, nz=30 if nx > 64 else 40, nt=256 if nx < 512 else 128)
     sx = 320.0
     d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
     m_ref = O.wem(oparams(g, mode), vel, wav, sx, d=d, adj=True)
-    m_gpu = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=[sx], pspi_mode=mode)
-    print(f"migrate pspi={pspi} mode={mode}: rel-L2 {rel_l2(m_gpu, m_ref):.3e}")
+    m_gpu = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=[sx], pspi_mode=mode, variant=variant)
+    print(f"migrate pspi={pspi} mode={mode} nx={nx} variant={variant}: rel-L2 {rel_l2(m_gpu, m_ref):.3e}")
     assert rel_l2(m_gpu, m_ref) <= TOL
 
 
-@pytest.mark.parametrize("pspi,mode", [(False, 0), (True, 0), (True, 1)])
-def test_model_matches_oracle(gpu, pspi, mode):
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("pspi,mode", CASES)
+def test_model_matches_oracle(gpu, pspi, mode, variant):
     g, vel, wav, rng = small_case(pspi, seed=3)
     sx = 200.0
     m = rng.standard_normal((g.nm, g.nz)).astype(np.float32)
     d_ref = O.wem(oparams(g, mode), vel, wav, sx, m=m, adj=False)
-    d_gpu = seiswem.ShotProfileWEM(m, False, vel=vel, wav=wav, geom=g, sx=[sx], pspi_mode=mode)[0]
-    print(f"model pspi={pspi} mode={mode}: rel-L2 {rel_l2(d_gpu, d_ref):.3e}")
+    d_gpu = seiswem.ShotProfileWEM(m, False, vel=vel, wav=wav, geom=g, sx=[sx], pspi_mode=mode, variant=variant)[0]
+    print(f"model pspi={pspi} mode={mode} variant={variant}: rel-L2 {rel_l2(d_gpu, d_ref):.3e}")
     assert rel_l2(d_gpu, d_ref) <= TOL
 
 
@@ -136,3 +145,32 @@ def test_linearity_full_config2_shape(gpu):
     err = rel_l2(m12, a * m1.astype(np.float64) + m2)
     print(f"config2 linearity rel-L2 {err:.3e}")
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("pspi,mode", CASES)
+def test_frequency_chunks_match_single_chunk(gpu, pspi, mode):
+    """A phase-table cap smaller than the band splits it into frequency chunks whose tables are
+    rebuilt per call; migration and modelling agree with the single-chunk plan and the oracle."""
+    g, vel, wav, rng = small_case(pspi, seed=9)
+    sx = [150.0, 410.0]
+    s = g.sampling()
+    nf, nkh = s["ifmax"] - s["ifmin"], s["nkx"] // 2 + 1
+    per_freq = 8 * g.nz * ((g.nref if pspi else 1) * nkh + (0 if pspi else g.nmx))
+    whole = seiswem.Plan(g, vel, wav, pspi_mode=mode)
+    chunked = seiswem.Plan(g, vel, wav, pspi_mode=mode, table_bytes=per_freq * (nf // 3 + 1))
+    d = rng.standard_normal((2, g.nm, g.nt)).astype(np.float32)
+    dd = seiswem.DeviceArray(d.shape, d)
+    m1 = whole.migrate(dd, sx).download()
+    m2 = chunked.migrate(dd, sx).download()
+    e2 = chunked.migrate_each(dd, sx).download()
+    assert rel_l2(m2, m1) <= 1e-6
+    assert rel_l2(m2, e2.astype(np.float64).sum(0)) <= 1e-6
+    alone = chunked.migrate(seiswem.DeviceArray((1, g.nm, g.nt), d[1:2]), sx[1:]).download()
+    assert np.array_equal(alone, e2[1])
+    m_ref = O.wem(oparams(g, mode), vel, wav, sx[0], d=d[0], adj=True)
+    assert rel_l2(e2[0], m_ref) <= TOL
+    m = rng.standard_normal((g.nm, g.nz)).astype(np.float32)
+    md = seiswem.DeviceArray(m.shape, m)
+    r1 = whole.model(md, sx).download()
+    r2 = chunked.model(md, sx).download()
+    assert rel_l2(r2, r1) <= 1e-6
