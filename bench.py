@@ -34,7 +34,7 @@ def cpu_baseline(g, vel, wav, sx, m_gpu_fn, cores):
     from seiswem import WemGeometry
 
     bins_per_hz = g.dt * (g.padt * g.nt)  # ifmax - ifmin per Hz
-    nsamp = 2 * cores
+    nsamp = 10 * cores  # about 10 s of CPU work
     fmin = 40.0
     fmax = fmin + nsamp / bins_per_hz
     gs = WemGeometry(**{**g.__dict__, "fmin": fmin, "fmax": fmax})
@@ -57,6 +57,26 @@ def cpu_baseline(g, vel, wav, sx, m_gpu_fn, cores):
     return {"value": shots_per_s, "unit": "shots/s", "cores": cores, "kind": "port", "sample": sample}, err, nf_samp
 
 
+COPY_BYTES = 1 << 30
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "march2d_adj_pspi_nkx2048_64shots_traffic.json")
+
+
+def traffic_per_launch(nshot):
+    """HBM bytes per march launch from the committed FETCH_SIZE / WRITE_SIZE passes (KiB per
+    dispatch), each scaled by the same counter's reading for a 1 GiB device copy; None when no
+    pass of this workload is on file."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("nshot") != nshot:
+        return None
+    fetch = t["march_fetch_kib"] * (COPY_BYTES / 1024) / t["copy_fetch_kib"]
+    write = t["march_write_kib"] * (COPY_BYTES / 1024) / t["copy_write_kib"]
+    return (fetch + write) * 1024
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +85,9 @@ def main():
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--freqs-per-block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed steps, copy a 1 GiB device buffer once (the reference kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not run in this build (shots shard across GPUs "
@@ -102,7 +125,7 @@ def main():
     flops_per_launch = flops / launches
     achieved = flops_per_launch / avg_launch_s / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_per_launch(nshot),
                 "kernel": "march2d_adj (PSPI, nkx=2048)", "avg_launch_ms": march_ms / launches,
                 "flops_per_launch": flops_per_launch}
     out = {
@@ -127,6 +150,13 @@ def main():
         out["cpu_baseline"] = cb
         out["rel_l2_vs_fp64"] = err
         out["rel_l2_sample"] = f"shot 0, {nf_samp} frequency bins, full nx/nz, PSPI compat"
+    if args.copy_calibration:
+        src = seiswem.DeviceArray((COPY_BYTES // 4,))
+        dst = seiswem.DeviceArray((COPY_BYTES // 4,))
+        src.zero()
+        seiswem.synchronize()
+        seiswem.copy_device(dst, src)  # the runtime's copy kernel, timed by the profiler
+        seiswem.synchronize()
     print(json.dumps(out))
 
 

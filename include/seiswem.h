@@ -65,7 +65,9 @@ typedef struct swem_options {
     int device;             /* HIP device ordinal (default 0) */
     int freqs_per_block;    /* frequencies marched together by one workgroup; 0 = autotuned */
     int verbose;            /* progress to stderr like progress_msg (wem.c:516-520) */
-    size_t scratch_bytes;   /* device scratch budget for one launch chunk; 0 = default */
+    size_t scratch_bytes;   /* device work memory of one call (spectra, partial images, source
+                               wavefields); 0 = min(96 GiB, free/2).  Shots beyond it run in
+                               several launches. */
     int variant;            /* SWEM_VARIANT_* (default AUTO) */
     size_t table_bytes;     /* device memory for the phase tables; 0 = min(32 GiB, free/4).
                                A band whose tables exceed it is marched in frequency chunks,
@@ -153,6 +155,7 @@ int swem_malloc(void** ptr, size_t bytes);
 int swem_free(void* ptr);
 int swem_memcpy_htod(void* dst, const void* src, size_t bytes);
 int swem_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int swem_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int swem_memset(void* ptr, int value, size_t bytes);
 int swem_synchronize(void);
 

@@ -323,7 +323,7 @@ static size_t default_budget()
 {
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)8 << 30;
-    return std::min((size_t)24 << 30, fr / 2);
+    return std::min((size_t)96 << 30, fr / 2);
 }
 
 static int plan_events(swem_plan* p, int n)
@@ -733,6 +733,12 @@ int swem_memcpy_htod(void* dst, const void* src, size_t bytes)
 int swem_memcpy_dtoh(void* dst, const void* src, size_t bytes)
 {
     SWEM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SWEM_OK;
+}
+
+int swem_memcpy_dtod(void* dst, const void* src, size_t bytes)
+{
+    SWEM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return SWEM_OK;
 }
 

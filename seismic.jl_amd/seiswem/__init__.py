@@ -2,5 +2,5 @@
 ShotProfileWEM path over the C ABI of libseiswem.so)."""
 from ._lib import (PREC_FP32, PREC_KZ64, PSPI_COMPAT, PSPI_LINEAR, VARIANT_AUTO, VARIANT_SUBGROUP,  # noqa: F401
                    VARIANT_WAVE, VARIANT_SUBGROUP16, SwemError, device_count, load, LIB_PATH, SHOTWEM_PATH)
-from .wem import DeviceArray, Plan, ShotProfileWEM, WemGeometry, synchronize, wem  # noqa: F401
+from .wem import DeviceArray, Plan, ShotProfileWEM, WemGeometry, copy_device, synchronize, wem  # noqa: F401
 from . import synth  # noqa: F401

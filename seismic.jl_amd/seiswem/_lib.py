@@ -24,7 +24,7 @@ EXPORTS = (
     "swem_shot_main", "swem_wem", "swem_wem_ex", "swem_shots",
     "swem_plan_create", "swem_plan_destroy", "swem_plan_migrate", "swem_plan_migrate_each",
     "swem_plan_model", "swem_plan_timing", "swem_plan_bytes",
-    "swem_malloc", "swem_free", "swem_memcpy_htod", "swem_memcpy_dtoh", "swem_memset", "swem_synchronize",
+    "swem_malloc", "swem_free", "swem_memcpy_htod", "swem_memcpy_dtoh", "swem_memcpy_dtod", "swem_memset", "swem_synchronize",
     "swem_sampling", "swem_source_index", "swem_pspi_tables", "swem_taper",
     "swem_seis_extent_init", "swem_seis_read_header", "swem_seis_read", "swem_seis_write",
 )
@@ -119,6 +119,7 @@ def load():
         "swem_free": ([vp], ctypes.c_int),
         "swem_memcpy_htod": ([vp, vp, ctypes.c_size_t], ctypes.c_int),
         "swem_memcpy_dtoh": ([vp, vp, ctypes.c_size_t], ctypes.c_int),
+        "swem_memcpy_dtod": ([vp, vp, ctypes.c_size_t], ctypes.c_int),
         "swem_memset": ([vp, ctypes.c_int, ctypes.c_size_t], ctypes.c_int),
         "swem_synchronize": ([], ctypes.c_int),
         "swem_sampling": ([gp, i32p], ctypes.c_int),

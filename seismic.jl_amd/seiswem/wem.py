@@ -174,6 +174,13 @@ def synchronize():
     check(load().swem_synchronize())
 
 
+def copy_device(dst: DeviceArray, src: DeviceArray):
+    """Device-to-device copy of a whole buffer."""
+    if dst.nbytes != src.nbytes:
+        raise ValueError("copy between buffers of different sizes")
+    check(load().swem_memcpy_dtod(dst.ptr, src.ptr, src.nbytes))
+
+
 def wem(d, m, wav, nt, ot, dt, nmx, omx, dmx, nmy, omy, dmy, sx, sy, nz, oz, dz, gz, sz, vel, nref, fmin, fmax,
         padt, padx, adj, pspi, verbose=False):
     """wem() (wem.c:3-11) on host arrays: adj fills m [nm, nz] from d [nm, nt], else d from m."""
