@@ -88,15 +88,17 @@ struct swem_plan {
     int nm = 0, nf = 0, nkx = 0, T = 0, F = 1, Ffwd = 1, mode = 0;
     // frequency chunks: [c * nfc, min(nf, (c + 1) * nfc)); the phase tables hold one chunk
     int nfc = 0, nchunks = 1, resident = -1;
+    bool three_d = false;
+    int F3 = 4, F3f = 8;  // frequencies per 3-D y-pass workgroup (migration, modelling)
     std::vector<float> vel;  // host copy (source of every table)
     // tables
-    DevBuf<float2> tw, spec_s, ktab, stab;
-    DevBuf<float> taper, p2f, pdf, w2, basis_adj, basis_fwd;
+    DevBuf<float2> tw, twy, spec_s, ktab, stab;
+    DevBuf<float> taper, taper_y, p2f, pdf, w2, basis_adj, basis_fwd;
     DevBuf<double> p2, pd;
     DevBuf<uint8_t> sel, sel2;
     DevBuf<uint32_t> used;
     // work
-    DevBuf<float2> spec_r, smig, dgw;
+    DevBuf<float2> spec_r, smig, dgw, W;
     DevBuf<float> partial, mT;
     DevBuf<int> igx;
     int* igx_pinned = nullptr;
@@ -152,21 +154,33 @@ static int build_tables(swem_plan* p, const float* wav)
     std::vector<float> tap(nmx);
     taper_f32(nmx, p->s.lmx, tap.data());
     SWEM_TRY(p->taper.upload(tap.data(), tap.size()));
+    if (p->three_d) {
+        const int nky = p->s.nky;
+        std::vector<float2> twy(nky);
+        for (int m = 0; m < nky; m++) {
+            double a = -2.0 * kPi * (double)m / (double)nky;
+            twy[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        SWEM_TRY(p->twy.upload(twy.data(), twy.size()));
+        std::vector<float> tapy(g.nmy);
+        taper_f32(g.nmy, p->s.lmy, tapy.data());
+        SWEM_TRY(p->taper_y.upload(tapy.data(), tapy.size()));
+    }
     // slowness split (wem.c:27-36): FP64 for the propagation, float copy for naive FP32;
     // depth-major [nz][nmx] for the split-step table
     std::vector<double> po(nz);
     std::vector<float> pof(nz), pdf_xz((size_t)nm * nz);
     slowness_f32(g, p->vel.data(), pof.data(), pdf_xz.data());
     if (!g.pspi) {
-        std::vector<double> pd((size_t)nz * nmx);
-        std::vector<float> pdf((size_t)nz * nmx);
+        std::vector<double> pd((size_t)nz * nm);
+        std::vector<float> pdf((size_t)nz * nm);
         for (int iz = 0; iz < nz; iz++) {
             double sum = 0;
             for (int ix = 0; ix < nm; ix++) sum += p->vel[(size_t)ix * nz + iz];
             po[iz] = 1.0 / (sum / (double)nm);
-            for (int ix = 0; ix < nmx; ix++) {
-                pd[(size_t)iz * nmx + ix] = 1.0 / (double)p->vel[(size_t)ix * nz + iz] - po[iz];
-                pdf[(size_t)iz * nmx + ix] = pdf_xz[(size_t)ix * nz + iz];
+            for (int ix = 0; ix < nm; ix++) {
+                pd[(size_t)iz * nm + ix] = 1.0 / (double)p->vel[(size_t)ix * nz + iz] - po[iz];
+                pdf[(size_t)iz * nm + ix] = pdf_xz[(size_t)ix * nz + iz];
             }
         }
         SWEM_TRY(p->pd.upload(pd.data(), pd.size()));
@@ -403,11 +417,17 @@ static int mode_of(const swem_plan* p)
     return p->opt.pspi_mode == SWEM_PSPI_LINEAR ? 2 : 1;
 }
 
+static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, const float* sy, int nshot, float* m_dev,
+                          int accumulate, int per_shot, hipStream_t st);
+static int model3d_impl(swem_plan* p, const float* m_dev, const float* sx, const float* sy, int nshot, float* d_dev,
+                        hipStream_t st);
+
 static int migrate_impl(swem_plan* p, const float* d_dev, const float* sx, const float* sy, int nshot, float* m_dev,
                         int accumulate, int per_shot, hipStream_t st)
 {
     if (!p || !d_dev || !sx || !m_dev || nshot < 1) return set_error(SWEM_EINVAL, "migrate: null argument or nshot < 1");
     SWEM_HIP(hipSetDevice(p->opt.device));
+    if (p->three_d) return migrate3d_impl(p, d_dev, sx, sy, nshot, m_dev, accumulate, per_shot, st);
     const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
     const size_t gmax = (size_t)(p->nfc + p->F - 1) / p->F;
     const size_t per = nm * nf * sizeof(float2) + gmax * nz * p->g.nmx * sizeof(float);
@@ -448,11 +468,177 @@ static int migrate_impl(swem_plan* p, const float* d_dev, const float* sx, const
     return SWEM_OK;
 }
 
+static March3DArgs base3(const swem_plan* p)
+{
+    March3DArgs a{};
+    a.nmx = p->g.nmx;
+    a.nmy = p->g.nmy;
+    a.nm = p->nm;
+    a.nkx = p->s.nkx;
+    a.nky = p->s.nky;
+    a.nz = p->g.nz;
+    a.nf = p->nf;
+    a.ifmin = p->s.ifmin;
+    a.prec = p->opt.precision;
+    a.dw = 2.0 * kPi / ((double)p->s.ntfft * (double)p->g.dt);
+    a.dkx = 2.0 * kPi / ((double)p->s.nkx * (double)p->g.dmx);
+    a.dky = 2.0 * kPi / ((double)p->s.nky * (double)p->g.dmy);
+    a.dzd = p->g.dz;
+    a.dkxf = p->s.dkx;
+    a.dkyf = p->s.dky;
+    a.inv_nk = (float)(1.0 / ((double)p->s.nkx * (double)p->s.nky));
+    a.p2 = p->p2.p;
+    a.p2f = p->p2f.p;
+    a.pd = p->pd.p;
+    a.pdf = p->pdf.p;
+    a.taper_x = p->taper.p;
+    a.taper_y = p->taper_y.p;
+    a.twx = p->tw.p;
+    a.twy = p->twy.p;
+    a.spec_s = p->spec_s.p;
+    return a;
+}
+
+static bool stepped_at(const swem_plan* p, int iz, float level)
+{
+    const float z = p->g.oz + p->g.dz * (float)iz;  // wem.c:235 in float
+    return iz >= 0 && iz < p->g.nz && z >= level;
+}
+
+// Frequencies per 3-D chunk: a multiple of the y-pass group that fits the work budget.
+static int chunk3(const swem_plan* p, size_t per_item, int F)
+{
+    size_t n = std::max<size_t>(1, p->budget / std::max<size_t>(per_item, 1));
+    if (n >= (size_t)p->nf) return p->nf;
+    n = std::max<size_t>(F, (n / F) * F);
+    return (int)std::min<size_t>(n, p->nf);
+}
+
+static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, const float* sy, int nshot, float* m_dev,
+                          int accumulate, int per_shot, hipStream_t st)
+{
+    const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
+    const int F = p->F3;
+    const size_t per_item = 2 * (size_t)p->g.nmx * p->s.nky * sizeof(float2) + nz * nm * sizeof(float) / F + 1;
+    const int nfc = chunk3(p, per_item, F);
+    const size_t gmax = (nfc + F - 1) / F;
+    SWEM_TRY(p->W.alloc((size_t)nfc * 2 * p->g.nmx * p->s.nky));
+    SWEM_TRY(p->partial.alloc(gmax * nz * nm));
+    SWEM_TRY(p->spec_r.alloc(nm * nf));
+    SWEM_TRY(set_igx(p, sx, sy, nshot, st));
+    p->nev = 0;
+    p->last_launches = 0;
+    p->last_flops = 0;
+    p->last_stream = st;
+    SWEM_TRY(record(p, -1, st));
+    for (int s = 0; s < nshot; s++) {
+        SWEM_TRY(launch_sgemm((int)nm, 2 * (int)nf, (int)nt, d_dev + (size_t)s * nm * nt, (int)nt, p->basis_adj.p,
+                              2 * (int)nf, reinterpret_cast<float*>(p->spec_r.p), 2 * (int)nf, st));
+        SWEM_TRY(record(p, 0, st));
+        for (int f0 = 0; f0 < (int)nf; f0 += nfc) {
+            March3DArgs a = base3(p);
+            a.f0 = f0;
+            a.nitem = std::min(nfc, (int)nf - f0);
+            a.igx = p->igx_pinned[s];
+            a.adj = 1;
+            a.spec_r = p->spec_r.p;
+            a.W = p->W.p;
+            a.partial = p->partial.p;
+            a.conj_src = 1;
+            a.conj_rec = 0;
+            SWEM_TRY(launch_march3d(0, a, F, 0, st));
+            for (int iz = 0; iz < (int)nz; iz++) {
+                a.iz = iz;
+                a.ds = stepped_at(p, iz, p->g.sz);
+                a.dr = stepped_at(p, iz, p->g.gz);
+                a.next_s = stepped_at(p, iz + 1, p->g.sz);
+                a.next_r = stepped_at(p, iz + 1, p->g.gz);
+                a.wmask = (a.ds ? 1 : 0) | (a.dr ? 2 : 0);
+                if (a.wmask) SWEM_TRY(launch_march3d(1, a, F, 0, st));
+                SWEM_TRY(launch_march3d(2, a, F, 0, st));
+            }
+            SWEM_TRY(record(p, 1, st));
+            p->last_launches++;
+            p->last_flops += step_flops(p) * (double)a.nitem * nz;
+            float* out = per_shot ? m_dev + (size_t)s * nm * nz : m_dev;
+            const int acc = f0 > 0 || (!per_shot && (accumulate || s > 0));
+            SWEM_TRY(launch_reduce_image(p->partial.p, 1, (a.nitem + F - 1) / F, (int)nz, (int)nm, out, acc, 0, st));
+            SWEM_TRY(record(p, 2, st));
+        }
+    }
+    return SWEM_OK;
+}
+
+static int model3d_impl(swem_plan* p, const float* m_dev, const float* sx, const float* sy, int nshot, float* d_dev,
+                        hipStream_t st)
+{
+    const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
+    const int F = p->F3f;
+    const size_t per_item = 2 * (size_t)p->g.nmx * p->s.nky * sizeof(float2) + nz * nm * sizeof(float2);
+    const int nfc = chunk3(p, per_item, F);
+    SWEM_TRY(p->W.alloc((size_t)nfc * 2 * p->g.nmx * p->s.nky));
+    SWEM_TRY(p->smig.alloc((size_t)nfc * nz * nm));
+    SWEM_TRY(p->dgw.alloc(nm * nf));
+    SWEM_TRY(p->mT.alloc(nm * nz));
+    SWEM_TRY(set_igx(p, sx, sy, nshot, st));
+    p->nev = 0;
+    p->last_launches = 0;
+    p->last_flops = 0;
+    p->last_stream = st;
+    SWEM_TRY(record(p, -1, st));
+    SWEM_TRY(launch_transpose(m_dev, (int)nm, (int)nz, p->mT.p, st));
+    SWEM_TRY(record(p, 0, st));
+    for (int s = 0; s < nshot; s++) {
+        for (int f0 = 0; f0 < (int)nf; f0 += nfc) {
+            March3DArgs a = base3(p);
+            a.f0 = f0;
+            a.nitem = std::min(nfc, (int)nf - f0);
+            a.igx = p->igx_pinned[s];
+            a.adj = 0;
+            a.W = p->W.p;
+            a.smig = p->smig.p;
+            a.mT = p->mT.p;
+            a.dgw = p->dgw.p;
+            a.conj_src = 1;
+            a.conj_rec = 1;
+            SWEM_TRY(launch_march3d(0, a, F, 0, st));
+            // source march, stored per depth (wem.c:254-265)
+            for (int iz = 0; iz < (int)nz; iz++) {
+                a.iz = iz;
+                a.ds = stepped_at(p, iz, p->g.sz);
+                a.next_s = stepped_at(p, iz + 1, p->g.sz);
+                a.wmask = 1;
+                if (a.ds) SWEM_TRY(launch_march3d(1, a, F, 0, st));
+                SWEM_TRY(launch_march3d(3, a, F, 0, st));
+            }
+            // receiver march upward with injection (wem.c:266-276)
+            a.first = 1;
+            for (int iz = (int)nz - 1; iz >= 0; iz--) {
+                if (!stepped_at(p, iz, p->g.gz)) continue;
+                a.iz = iz;
+                a.wmask = 2;
+                SWEM_TRY(launch_march3d(3, a, F, 1, st));
+                SWEM_TRY(launch_march3d(1, a, F, 0, st));
+                a.first = 0;
+            }
+            SWEM_TRY(launch_march3d(3, a, F, 2, st));
+            SWEM_TRY(record(p, 1, st));
+            p->last_launches++;
+            p->last_flops += step_flops(p) * (double)a.nitem * nz;
+        }
+        SWEM_TRY(launch_sgemm((int)nm, (int)nt, 2 * (int)nf, reinterpret_cast<const float*>(p->dgw.p), 2 * (int)nf,
+                              p->basis_fwd.p, (int)nt, d_dev + (size_t)s * nm * nt, (int)nt, st));
+        SWEM_TRY(record(p, 2, st));
+    }
+    return SWEM_OK;
+}
+
 static int model_impl(swem_plan* p, const float* m_dev, const float* sx, const float* sy, int nshot, float* d_dev,
                       hipStream_t st)
 {
     if (!p || !d_dev || !sx || !m_dev || nshot < 1) return set_error(SWEM_EINVAL, "model: null argument or nshot < 1");
     SWEM_HIP(hipSetDevice(p->opt.device));
+    if (p->three_d) return model3d_impl(p, m_dev, sx, sy, nshot, d_dev, st);
     const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
     const size_t per = nm * nf * sizeof(float2) + (size_t)p->nfc * nz * p->g.nmx * sizeof(float2);
     int chunk = (int)std::max<size_t>(1, std::min<size_t>(nshot, p->budget / std::max<size_t>(per, 1)));
@@ -543,8 +729,15 @@ int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav,
     p->nm = g->nmx * g->nmy;
     p->nf = p->s.ifmax - p->s.ifmin;
     p->nkx = p->s.nkx;
-    if (g->nmy > 1) { delete p; return set_error(SWEM_EUNSUPPORTED, "3-D (nmy=%d > 1) is not available in this build", g->nmy); }
-    if (!march2d_supported(p->nkx)) {
+    p->three_d = g->nmy > 1;
+    if (p->three_d) {
+        if (g->pspi) { delete p; return set_error(SWEM_EUNSUPPORTED, "3-D PSPI is not available in this build (3-D SSF is)"); }
+        if (g->nmx < 2 || !march3d_supported(p->s.nkx, p->s.nky)) {
+            delete p;
+            return set_error(SWEM_EUNSUPPORTED, "3-D: nkx=%d, nky=%d: this build handles powers of two 128..1024", p->s.nkx,
+                             p->s.nky);
+        }
+    } else if (!march2d_supported(p->nkx)) {
         delete p;
         return set_error(SWEM_EUNSUPPORTED, "nkx = padx*nmx = %d: this build handles powers of two 128..4096", p->nkx);
     }
@@ -553,6 +746,14 @@ int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav,
     if (!(g->dz != 0.f)) { delete p; return set_error(SWEM_EINVAL, "dz must be non-zero"); }
     rc = check_device(o.device);
     if (rc) { delete p; return rc; }
+    if (p->three_d) {
+        // y-pass workgroups: F frequencies x 2 wavefields (migration), F x 1 (modelling)
+        if (o.freqs_per_block > 0) p->F3 = p->F3f = o.freqs_per_block;
+        if (p->F3 < 1 || 128 * p->F3 > 1024 || 64 * p->F3f > 1024) {
+            delete p;
+            return set_error(SWEM_EINVAL, "3-D: freqs_per_block=%d (1..8)", o.freqs_per_block);
+        }
+    } else {
     p->T = march2d_threads(p->nkx, o.variant);
     p->F = choose_group(p, 2, o.freqs_per_block);
     p->Ffwd = choose_group(p, 1, o.freqs_per_block);
@@ -563,11 +764,12 @@ int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav,
         delete p;
         return set_error(SWEM_EINVAL, "freqs_per_block=%d does not fit a workgroup at nkx=%d", o.freqs_per_block, p->s.nkx);
     }
+    }
     p->mode = mode_of(p);
     p->vel.assign(vel, vel + (size_t)p->nm * g->nz);
     p->budget = o.scratch_bytes ? o.scratch_bytes : default_budget();
     rc = build_tables(p, wav);
-    if (!rc) rc = alloc_tables(p);
+    if (!rc && !p->three_d) rc = alloc_tables(p);
     if (rc) { swem_plan_destroy(p); return rc; }
     *out = p;
     return SWEM_OK;
@@ -580,6 +782,7 @@ int swem_plan_destroy(swem_plan* p)
     if (p->last_stream || p->nev) (void)hipStreamSynchronize(p->last_stream);
     for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
     if (p->igx_pinned) (void)hipHostFree(p->igx_pinned);
+    p->twy.release(); p->taper_y.release(); p->W.release();
     p->tw.release(); p->spec_s.release(); p->ktab.release(); p->stab.release(); p->pd.release(); p->pdf.release();
     p->taper.release(); p->p2f.release(); p->w2.release();
     p->basis_adj.release(); p->basis_fwd.release(); p->p2.release(); p->sel.release(); p->sel2.release();

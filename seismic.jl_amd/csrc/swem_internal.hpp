@@ -87,6 +87,39 @@ bool march2d_supported(int nkx);
 int march2d_max_block(int nkx, int variant);
 size_t march2d_lds_bytes(int nkx, int nsub, int variant);
 
+// Arguments of the 3-D split-step kernels (swem_march3d.hip).  A launch covers the items
+// (frequencies f0 .. f0 + nitem of one shot) of a frequency chunk at depth iz.
+struct March3DArgs {
+    int nmx, nmy, nm, nkx, nky, nz, nf, ifmin, f0, nitem, iz, prec;
+    int igx;                    // source trace ix * nmy + iy
+    int adj;                    // init: load receiver spectra (migration) or zeros (modelling)
+    int wmask;                  // x-pass: bit 0 source, bit 1 receiver
+    int conj_src, conj_rec;     // x-pass: conj(L) for a -dz step
+    int ds, dr, next_s, next_r; // stepped at this depth / at the next depth
+    int first;                  // modelling receiver: first step (zero state)
+    double dw, dkx, dky, dzd;
+    float dkxf, dkyf, inv_nk;
+    const double* p2;           // [nz] po^2
+    const float* p2f;           // [nz] float po^2 (naive FP32)
+    const double* pd;           // [nz][nm] slowness perturbation
+    const float* pdf;           // [nz][nm] float counterpart
+    const float* taper_x;       // [nmx]
+    const float* taper_y;       // [nmy]
+    const float2* twx;          // [nkx]
+    const float2* twy;          // [nky]
+    const float2* spec_s;       // [nf]
+    const float2* spec_r;       // [nm][nf] this shot's receiver spectra
+    float2* W;                  // [nitem][2][nmx][nky] step state (y transformed)
+    float* partial;             // [ngroups][nz][nm] partial images of this shot
+    const float* mT;            // [nz][nm] image (modelling input)
+    float2* smig;               // [nitem][nz][nm] source wavefields (modelling)
+    float2* dgw;                // [nm][nf] modelled spectra of this shot
+};
+bool march3d_supported(int nkx, int nky);
+// which: 0 init, 1 x-pass, 2 migration y-pass, 3 modelling y-pass (phase 0 source,
+// 1 receiver, 2 output).  F: frequencies per y-pass workgroup.
+int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_t st);
+
 // C[M x Nc] = A[M x K] * B[K x Nc], row-major with leading dimensions lda, ldb, ldc.
 int launch_sgemm(int M, int Nc, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc,
                  hipStream_t st);

@@ -1,0 +1,408 @@
+// swem_march3d.hip -- 3-D split-step Fourier depth march (nmy > 1).
+//
+// A 3-D slice (nkx x nky complex, 2 MiB at 512 x 512) does not fit in LDS, so one depth step
+// is two passes over HBM per wavefield, each a read and a write of the live part of the slice:
+//
+//   state between steps: W[item][wf][x][ky], x < nmx only (rows x >= nmx of the padded slice are
+//   zero on input and discarded on output, so they are never stored), y transformed to ky.
+//
+//   march3d_cols  (x-pass)  for each ky column: FFT over x (input rows x < nmx, zeros above),
+//                           times the kz phase exp(+-i kz dz)/nk built from FP64
+//                           s = w^2 po^2 - kx^2 - ky^2, inverse FFT, keep x < nmx.
+//   march3d_rows  (y-pass)  for each row x: inverse FFT over y, keep y < nmy, then the
+//                           split-step phase and taper (adjoint, wem.c:350-359), the imaging
+//                           condition of the F frequencies of the workgroup summed in a fixed
+//                           order, then the forward y-FFT that starts the next step.
+//
+// so a step of the adjoint (wem.c:313-320, 334-359) FFT2 -> L -> IFFT2 -> SS -> taper is carried
+// out as  [FFTy] FFTx L IFFTx | IFFTy SS taper image FFTy ...  and the forward step (wem.c:321-
+// 332, 362-369) taper -> SS -> FFT2 -> L -> IFFT2 as  IFFTy inject taper SS FFTy | FFTx L IFFTx.
+// An item is one frequency of one shot; a launch covers every item of the frequency chunk.
+// Transforms are wave-level (one 64-lane wave per row / column, N/64 points per lane, no
+// workgroup barrier inside a transform).
+#include "swem_fft.hpp"
+#include "swem_internal.hpp"
+
+namespace swem {
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586;
+
+// exp(i 2 pi r) for r already reduced to [-1/2, 1/2] (hardware sine / cosine take revolutions).
+__device__ __forceinline__ float2 cis_rev(double r)
+{
+    const float rf = (float)r;
+    return make_float2(__builtin_amdgcn_cosf(rf), __builtin_amdgcn_sinf(rf));
+}
+
+__device__ __forceinline__ double reduce_rev(double phase)
+{
+    double r = phase * (1.0 / kTwoPi);
+    return r - rint(r);
+}
+
+// kz phase of one (kx, ky) point, +dz orientation, with 1/nk folded in (wem.c:336-344).
+__device__ __forceinline__ float2 kz_phase(const March3DArgs& a, double w, double p2, int ikx, int iky)
+{
+    const int kxi = ikx <= a.nkx / 2 ? ikx : a.nkx - ikx;
+    const int kyi = iky <= a.nky / 2 ? iky : a.nky - iky;
+    if (a.prec == SWEM_PREC_KZ64) {
+        const double kx = kxi * a.dkx, ky = kyi * a.dky;
+        const double s = w * w * p2 - kx * kx - ky * ky;
+        if (s >= 0.0) {
+            const float2 c = cis_rev(reduce_rev(sqrt(s) * a.dzd));
+            return make_float2(c.x * a.inv_nk, c.y * a.inv_nk);
+        }
+        return make_float2(expf((float)(-0.2 * sqrt(-s) * fabs(a.dzd))) * a.inv_nk, 0.f);
+    }
+    // the reference's float expressions (wem.c:336-343)
+    const float wf = (float)w;
+    const float kx = a.dkxf * (float)kxi, ky = a.dkyf * (float)kyi;
+    const float s = (wf * wf) * a.p2f[a.iz] - kx * kx - ky * ky;
+    if (s >= 0.f) {
+        float sn, cs;
+        sincosf(sqrtf(s) * (float)a.dzd, &sn, &cs);
+        return make_float2(cs * a.inv_nk, sn * a.inv_nk);
+    }
+    return make_float2(expf(-0.2f * sqrtf(fabsf(s)) * fabsf((float)a.dzd)) * a.inv_nk, 0.f);
+}
+
+// split-step phase exp(i w pd dz) (+dz orientation, wem.c:326,353)
+__device__ __forceinline__ float2 ss_phase(const March3DArgs& a, double w, size_t ixy)
+{
+    if (a.prec == SWEM_PREC_KZ64) return cis_rev(reduce_rev(w * a.pd[(size_t)a.iz * a.nm + ixy] * a.dzd));
+    float sn, cs;
+    sincosf((float)w * a.pdf[(size_t)a.iz * a.nm + ixy] * (float)a.dzd, &sn, &cs);
+    return make_float2(cs, sn);
+}
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float2* state(const March3DArgs& a, int item, int wf)
+{
+    return a.W + ((size_t)item * 2 + wf) * (size_t)a.nmx * a.nky;
+}
+
+}  // namespace
+
+// Initial wavefields of the chunk (adjoint: source delta at the shot position and receiver
+// spectra; forward: source delta, receiver zero), forward y-FFT, stored as the step state.
+// Workgroup: WAVES rows (one wave each); grid: nitem * 2 * nmx rows.
+template <int NY>
+__global__ __launch_bounds__(256) void march3d_init(March3DArgs a)
+{
+    constexpr int P = NY / 64;
+    constexpr int L = LdsLen<NY>::value;
+    __shared__ float2 lds[4 * L];
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const size_t row = (size_t)blockIdx.x * 4 + w;
+    const size_t nrow = (size_t)a.nitem * 2 * a.nmx;
+    if (row >= nrow) return;  // whole waves only: the transform below has no workgroup barrier
+    const int x = (int)(row % a.nmx);
+    const int wf = (int)((row / a.nmx) & 1);
+    const int item = (int)(row / ((size_t)2 * a.nmx));
+    const int fl = a.f0 + item;  // frequency index in the band (one shot per launch)
+    float2 v[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int y = t + 64 * q;
+        float2 u = make_float2(0.f, 0.f);
+        if (y < a.nmy) {
+            const size_t ixy = (size_t)x * a.nmy + y;
+            if (wf == 0)
+                u = (int)ixy == a.igx ? a.spec_s[fl] : u;
+            else if (a.adj)
+                u = a.spec_r[ixy * a.nf + fl];
+        }
+        v[q] = u;
+    }
+    wfft<NY, -1>(v, lds + w * L, a.twy, t);
+    float2* dst = state(a, item, wf) + (size_t)x * a.nky;
+#pragma unroll
+    for (int q = 0; q < P; ++q) dst[t + 64 * q] = v[q];
+}
+
+// x-pass.  Workgroup: C columns x nwf wavefields, one wave per (wavefield, column); the tile of
+// C consecutive ky columns is staged through LDS so global reads and writes are row segments of
+// C complex.  wmask: bit 0 source, bit 1 receiver.  conj_src / conj_rec: use conj(L) (-dz).
+template <int NX>
+__global__ __launch_bounds__(NX >= 1024 ? 512 : 1024) void march3d_cols(March3DArgs a, int C)
+{
+    constexpr int P = NX / 64;
+    constexpr int L = LdsLen<NX>::value;
+    extern __shared__ float2 smem[];
+    const int nwf = (a.wmask & 1) + ((a.wmask >> 1) & 1);
+    const int tiles = a.nky / C;
+    const int item = blockIdx.x / tiles;
+    const int ky0 = (blockIdx.x - item * tiles) * C;
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const int nthr = blockDim.x;
+    // stage: LDS buffer of wave (k, c) holds column ky0 + c of wavefield k (k-th active one)
+    for (int i = threadIdx.x; i < a.nmx * C * nwf; i += nthr) {
+        const int c = i % C, r = i / C;
+        const int k = r % nwf, x = r / nwf;
+        const int wf = nwf == 2 ? k : ((a.wmask & 1) ? 0 : 1);
+        smem[(k * C + c) * L + lds_pad(x)] = state(a, item, wf)[(size_t)x * a.nky + ky0 + c];
+    }
+    __syncthreads();
+    const int k = w / C, c = w - k * C;
+    const int wf = nwf == 2 ? k : ((a.wmask & 1) ? 0 : 1);
+    float2* buf = smem + w * L;
+    float2 v[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int x = t + 64 * q;
+        v[q] = x < a.nmx ? buf[lds_pad(x)] : make_float2(0.f, 0.f);
+    }
+    wave_sync_lds();
+    wfft<NX, -1>(v, buf, a.twx, t);
+    const int iw = a.ifmin + a.f0 + item;
+    const double wd = (double)iw * a.dw;
+    const double p2 = a.p2[a.iz];
+    const bool cj = wf == 0 ? a.conj_src : a.conj_rec;
+    const int iky = ky0 + c;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const float2 ph = kz_phase(a, wd, p2, t + 64 * q, iky);
+        v[q] = cj ? cmulc(v[q], ph) : cmul(v[q], ph);
+    }
+    wave_sync_lds();
+    wfft<NX, +1>(v, buf, a.twx, t);
+    wave_sync_lds();
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int x = t + 64 * q;
+        if (x < a.nmx) buf[lds_pad(x)] = v[q];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.nmx * C * nwf; i += nthr) {
+        const int cc = i % C, r = i / C;
+        const int kk = r % nwf, x = r / nwf;
+        const int wff = nwf == 2 ? kk : ((a.wmask & 1) ? 0 : 1);
+        state(a, item, wff)[(size_t)x * a.nky + ky0 + cc] = smem[(kk * C + cc) * L + lds_pad(x)];
+    }
+}
+
+// y-pass of the migration (adjoint): F frequencies x {source, receiver} of one row x.
+// Wave s = 2f + wf.  ds / dr: the source / receiver was stepped at this depth; next_s / next_r:
+// it is stepped at the next depth (its forward y-FFT is needed); the imaging runs when dr.
+template <int NY>
+__global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
+{
+    constexpr int P = NY / 64;
+    constexpr int L = LdsLen<NY>::value;
+    extern __shared__ float2 smem[];
+    const int ngroups = (a.nitem + F - 1) / F;
+    const int g = blockIdx.x % ngroups, x = blockIdx.x / ngroups;
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const int f = w >> 1, wf = w & 1;
+    const int item = g * F + f;
+    const bool active = item < a.nitem;
+    const int iw = a.ifmin + a.f0 + (active ? item : 0);
+    const double wd = (double)iw * a.dw;
+    float2* buf = smem + w * L;
+    const bool stepped = wf == 0 ? a.ds : a.dr;
+    const bool need = active && (stepped || (wf == 0 && a.dr));  // spatial values needed
+    float2 v[P];
+    if (need) {
+        const float2* src = state(a, item, wf) + (size_t)x * a.nky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = src[t + 64 * q];
+        wfft<NY, +1>(v, buf, a.twy, t);
+        const float tx = a.taper_x[x];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int y = t + 64 * q;
+            if (y < a.nmy) {
+                if (stepped) {
+                    const float2 s = ss_phase(a, wd, (size_t)x * a.nmy + y);
+                    v[q] = cscale(wf == 0 ? cmulc(v[q], s) : cmul(v[q], s), tx * a.taper_y[y]);
+                }
+            } else
+                v[q] = make_float2(0.f, 0.f);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);
+    }
+    // imaging (wem.c:243-247): the source wave hands its row to the receiver wave
+    wave_sync_lds();
+    if (wf == 0) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) buf[lds_pad(t + 64 * q)] = v[q];
+    }
+    __syncthreads();
+    float cc[P];
+    const float factor = iw == 0 ? 1.f : 2.f;
+    if (wf == 1) {
+        const float2* us = smem + (w - 1) * L;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const float2 s = us[lds_pad(t + 64 * q)];
+            cc[q] = (a.dr && active) ? factor * (v[q].x * s.x + v[q].y * s.y) : 0.f;
+        }
+    }
+    __syncthreads();
+    if (wf == 1) {
+        float* red = reinterpret_cast<float*>(buf);
+#pragma unroll
+        for (int q = 0; q < P; ++q) red[t + 64 * q] = cc[q];
+    }
+    __syncthreads();
+    float* part = a.partial + ((size_t)g * a.nz + a.iz) * a.nm + (size_t)x * a.nmy;
+    for (int y = threadIdx.x; y < a.nmy; y += blockDim.x) {
+        float sum = 0.f;
+        for (int ff = 0; ff < F; ++ff) sum += reinterpret_cast<const float*>(smem + (2 * ff + 1) * L)[y];
+        part[y] = sum;
+    }
+    __syncthreads();
+    // forward y-FFT that starts the next step
+    const bool next = wf == 0 ? a.next_s : a.next_r;
+    if (active && stepped && next) {
+        wfft<NY, -1>(v, buf, a.twy, t);
+        float2* dst = state(a, item, wf) + (size_t)x * a.nky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) dst[t + 64 * q] = v[q];
+    }
+}
+
+// y-pass of modelling (forward), one wavefield per wave, F frequencies of one row x.
+//   phase 0 (source march): IFFTy, SS (conj) and taper as the adjoint source, store the
+//           spatial source of this depth in smig (zero where z < sz), FFTy for the next step.
+//   phase 1 (receiver march, upward): IFFTy of the previous step's result (skipped at the first
+//           step, whose state is zero), inject m * smig, taper, conj(SS), FFTy.
+//   phase 2 (output): IFFTy of the last step's result, write the spectra of the record.
+template <int NY>
+__global__ __launch_bounds__(1024) void march3d_rows_fwd(March3DArgs a, int F, int phase)
+{
+    constexpr int P = NY / 64;
+    constexpr int L = LdsLen<NY>::value;
+    extern __shared__ float2 smem[];
+    const int ngroups = (a.nitem + F - 1) / F;
+    const int g = blockIdx.x % ngroups, x = blockIdx.x / ngroups;
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const int item = g * F + w;
+    if (item >= a.nitem) return;  // no workgroup barriers in this kernel
+    const int fl = a.f0 + item;
+    const double wd = (double)(a.ifmin + fl) * a.dw;
+    float2* buf = smem + w * L;
+    const int wf = phase == 0 ? 0 : 1;
+    float2* row = state(a, item, wf) + (size_t)x * a.nky;
+    float2 v[P];
+    if (phase == 1 && a.first) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);
+    } else {
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = row[t + 64 * q];
+        wfft<NY, +1>(v, buf, a.twy, t);
+    }
+    const float tx = a.taper_x[x];
+    float2* sm = a.smig + ((size_t)item * a.nz + a.iz) * a.nm + (size_t)x * a.nmy;
+    const float* mt = a.mT + (size_t)a.iz * a.nm + (size_t)x * a.nmy;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int y = t + 64 * q;
+        if (y >= a.nmy) {
+            v[q] = make_float2(0.f, 0.f);
+            continue;
+        }
+        if (phase == 0) {
+            if (a.ds) v[q] = cscale(cmulc(v[q], ss_phase(a, wd, (size_t)x * a.nmy + y)), tx * a.taper_y[y]);
+            sm[y] = a.ds ? v[q] : make_float2(0.f, 0.f);
+        } else if (phase == 1) {
+            const float2 u = cadd(v[q], cscale(sm[y], mt[y]));
+            v[q] = cmulc(cscale(u, tx * a.taper_y[y]), ss_phase(a, wd, (size_t)x * a.nmy + y));
+        }
+    }
+    if (phase == 2) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int y = t + 64 * q;
+            if (y < a.nmy) a.dgw[((size_t)x * a.nmy + y) * a.nf + fl] = v[q];
+        }
+        return;
+    }
+    if (phase == 0 && !(a.ds && a.next_s)) return;  // state unchanged / no further step
+    wave_sync_lds();
+    wfft<NY, -1>(v, buf, a.twy, t);
+#pragma unroll
+    for (int q = 0; q < P; ++q) row[t + 64 * q] = v[q];
+}
+
+// ------------------------------------------------------------------------ dispatch
+
+bool march3d_supported(int nkx, int nky)
+{
+    auto ok = [](int n) { return n >= 128 && n <= 1024 && (n & (n - 1)) == 0; };
+    return ok(nkx) && ok(nky);
+}
+
+// Columns per x-pass workgroup: 16 waves (8 per wavefield) below nkx = 1024, 8 waves at 1024
+// (16 points per lane need more than the 128 registers of a 1024-thread workgroup).
+static int march3d_cols_tile(int nwf, int nkx) { return (nkx >= 1024 ? 8 : 16) / nwf; }
+
+template <typename K>
+static int launch3(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, const March3DArgs& a, int extra)
+{
+    if (lds > 48 * 1024)
+        SWEM_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, a, extra);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
+template <int N>
+static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStream_t st)
+{
+    constexpr int L = LdsLen<N>::value;
+    if (which == 0) {  // init
+        const size_t rows = (size_t)a.nitem * 2 * a.nmx;
+        hipLaunchKernelGGL(march3d_init<N>, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st, a);
+        SWEM_HIP(hipGetLastError());
+        return SWEM_OK;
+    }
+    if (which == 1) {  // cols
+        const int nwf = (a.wmask & 1) + ((a.wmask >> 1) & 1);
+        const int C = march3d_cols_tile(nwf, N);
+        const size_t lds = (size_t)C * nwf * L * sizeof(float2);
+        return launch3(march3d_cols<N>, dim3(a.nitem * (a.nky / C)), dim3(64 * C * nwf), lds, st, a, C);
+    }
+    const int ngroups = (a.nitem + F - 1) / F;
+    if (which == 2) {
+        const size_t lds = (size_t)2 * F * L * sizeof(float2);
+        return launch3(march3d_rows_adj<N>, dim3(ngroups * a.nmx), dim3(128 * F), lds, st, a, F);
+    }
+    const size_t lds = (size_t)F * L * sizeof(float2);
+    SWEM_HIP(hipFuncSetAttribute((const void*)march3d_rows_fwd<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
+    hipLaunchKernelGGL(march3d_rows_fwd<N>, dim3(ngroups * a.nmx), dim3(64 * F), lds, st, a, F, phase);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
+// which: 0 init (y-FFT of the initial wavefields), 1 x-pass, 2 adjoint y-pass, 3 forward y-pass
+// (phase 0 source, 1 receiver, 2 output).  F: frequencies per y-pass workgroup.
+int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_t st)
+{
+    const int n = which == 1 ? a.nkx : a.nky;
+    if (!march3d_supported(a.nkx, a.nky) || a.nmx > a.nkx || a.nmy > a.nky || a.nitem < 1)
+        return set_error(SWEM_EINVAL, "3-D march: inconsistent launch (nkx=%d nky=%d)", a.nkx, a.nky);
+    if ((which == 2 && 128 * F > 1024) || (which == 3 && 64 * F > 1024) || F < 1)
+        return set_error(SWEM_EINVAL, "3-D march: %d frequencies per workgroup", F);
+    switch (n) {
+    case 128: return launch3_n<128>(which, a, F, phase, st);
+    case 256: return launch3_n<256>(which, a, F, phase, st);
+    case 512: return launch3_n<512>(which, a, F, phase, st);
+    case 1024: return launch3_n<1024>(which, a, F, phase, st);
+    }
+    return set_error(SWEM_EUNSUPPORTED, "3-D march: n=%d", n);
+}
+
+}  // namespace swem

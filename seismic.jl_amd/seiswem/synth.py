@@ -78,6 +78,43 @@ def config2(nshot: int = 64, seed: int = 2, nx: int = 1024, nz: int = 500, nt: i
     return g, vel, refl, wav, sx
 
 
+def config3(seed: int = 3, nx: int = 256, ny: int = 256, nz: int = 300, nt: int = 1500, fmax: float = 50.0,
+            dx: float = 12.5, dz: float = 10.0):
+    """3-D SSF: nx=ny=256, dx=dy=12.5; nz=300, dz=10; nt=1500, dt=4 ms; 3-50 Hz;
+    v = 1600 + 0.6 z + 600 exp(-|r - c|^2 / (2 400^2)), c = (1600, 1600, 1500) m; 4 planar
+    reflectors with normals at dip U(0, 30 deg), azimuth U(0, 360 deg) through random points,
+    amplitudes +-U(0.05, 0.15); one shot at (1600, 1600); 15 Hz Ricker.  Smaller grids keep the
+    same construction with the anomaly and the shot at the centre of the model.
+    Layouts [nm, nz] with trace ix = imx * ny + imy."""
+    g = WemGeometry(nt=nt, dt=0.004, nmx=nx, dmx=dx, nmy=ny, dmy=dx, nz=nz, dz=dz, fmin=3.0, fmax=fmax, pspi=False)
+    rng = np.random.default_rng(seed)
+    x = np.arange(nx) * dx
+    y = np.arange(ny) * dx
+    z = np.arange(nz) * dz
+    full = nx == 256 and ny == 256 and nz == 300 and dx == 12.5 and dz == 10.0
+    cx, cy, cz = (1600.0, 1600.0, 1500.0) if full else (x[nx // 2], y[ny // 2], z[nz // 2])
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    r2 = (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2
+    vel = (1600.0 + 0.6 * Z + 600.0 * np.exp(-r2 / (2.0 * 400.0 ** 2))).astype(np.float32).reshape(nx * ny, nz)
+    refl = np.zeros((nx, ny, nz), np.float32)
+    XX, YY = np.meshgrid(x, y, indexing="ij")
+    for _ in range(4):
+        th = np.deg2rad(rng.uniform(0.0, 30.0))
+        ph = np.deg2rad(rng.uniform(0.0, 360.0))
+        n = np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+        p0 = np.array([rng.uniform(0, x[-1]), rng.uniform(0, y[-1]), rng.uniform(0.15 * z[-1], 0.85 * z[-1])])
+        amp = rng.uniform(0.05, 0.15) * rng.choice([-1.0, 1.0])
+        zp = p0[2] - (n[0] * (XX - p0[0]) + n[1] * (YY - p0[1])) / n[2]
+        iz = np.rint(zp / dz).astype(int)
+        ok = (iz >= 0) & (iz < nz)
+        ix, iy = np.nonzero(ok)
+        refl[ix, iy, iz[ok]] += amp
+    wav = wavelet_trace(g.nt, g.dt, 15.0)
+    sx = np.array([cx if full else x[nx // 2]], np.float32)
+    sy = np.array([cy if full else y[ny // 2]], np.float32)
+    return g, vel, refl.reshape(nx * ny, nz), wav, sx, sy
+
+
 def config5_shots() -> np.ndarray:
     """Config-1 geometry with 32 shots at sx = 40 + 80 k."""
     return (40.0 + 80.0 * np.arange(32)).astype(np.float32)

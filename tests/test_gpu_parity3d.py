@@ -1,0 +1,109 @@
+"""GPU parity of the 3-D split-step path (nmy > 1) through the C ABI against the FP64 oracle.
+
+Tolerances as the 2-D tests: relative L2 <= 1e-4 against the FP64 oracle on the same seeded
+inputs, dot-product tests <= 1e-5.  The full config-3 shape (256 x 256 x 300, 565 frequencies)
+is checked through the dot test and linearity, whose results do not depend on an oracle run.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import seiswem
+from seiswem import synth
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+DOT_TOL = 1e-5
+
+
+def oparams(g: seiswem.WemGeometry) -> O.WemParams:
+    return O.WemParams(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, nmy=g.nmy, dmy=g.dmy, ot=g.ot,
+                       omx=g.omx, oz=g.oz, gz=g.gz, sz=g.sz, nref=g.nref, fmin=g.fmin, fmax=g.fmax, padt=g.padt,
+                       padx=g.padx, pspi=g.pspi, pspi_mode=0)
+
+
+def small3d(nx=64, ny=64, nz=24, nt=128, fmax=25.0, seed=3):
+    return synth.config3(seed=seed, nx=nx, ny=ny, nz=nz, nt=nt, fmax=fmax)
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(64, 64, 24), (128, 64, 12), (64, 128, 12)])
+def test_migrate3d_matches_oracle(gpu, nx, ny, nz):
+    """nx = 128 > 100 switches the taper on along x (lmx = 30), ny = 128 along y."""
+    g, vel, refl, wav, sx, sy = small3d(nx, ny, nz)
+    rng = np.random.default_rng(nx + ny)
+    d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
+    m_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), d=d, adj=True)
+    m_gpu = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)
+    err = rel_l2(m_gpu, m_ref)
+    print(f"3-D migrate {nx}x{ny}x{nz}: rel-L2 {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(64, 64, 24), (128, 64, 12)])
+def test_model3d_matches_oracle(gpu, nx, ny, nz):
+    g, vel, refl, wav, sx, sy = small3d(nx, ny, nz)
+    d_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), m=refl, adj=False)
+    d_gpu = seiswem.ShotProfileWEM(refl, False, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)[0]
+    err = rel_l2(d_gpu, d_ref)
+    print(f"3-D model {nx}x{ny}x{nz}: rel-L2 {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+def dot_test(g, vel, wav, sx, sy, seed, **kw):
+    plan = seiswem.Plan(g, vel, wav, **kw)
+    rng = np.random.default_rng(seed)
+    m = rng.standard_normal((g.nm, g.nz)).astype(np.float32)
+    d = rng.standard_normal((1, g.nm, g.nt)).astype(np.float32)
+    lm = plan.model(seiswem.DeviceArray(m.shape, m), sx, sy).download()
+    lhd = plan.migrate(seiswem.DeviceArray(d.shape, d), sx, sy).download()
+    a = float(np.sum(d.astype(np.float64) * lm))
+    b = float(np.sum(m.astype(np.float64) * lhd))
+    return a, b, abs(a - b) / abs(b)
+
+
+def test_dot_product_3d_small(gpu):
+    g, vel, refl, wav, sx, sy = small3d(128, 64, 16)
+    a, b, e = dot_test(g, vel, wav, sx, sy, 5)
+    print(f"3-D dot small: {a:.9e} {b:.9e} rel {e:.3e}", flush=True)
+    assert e <= DOT_TOL
+
+
+def test_frequency_chunks_3d(gpu):
+    """A work budget below one chunk of the whole band splits it; images and records agree."""
+    g, vel, refl, wav, sx, sy = small3d(64, 64, 16)
+    rng = np.random.default_rng(1)
+    d = rng.standard_normal((2, g.nm, g.nt)).astype(np.float32)
+    sxs = np.array([sx[0], sx[0] - 200.0], np.float32)
+    sys_ = np.array([sy[0], sy[0] + 100.0], np.float32)
+    whole = seiswem.Plan(g, vel, wav)
+    small = seiswem.Plan(g, vel, wav, scratch_bytes=2 << 20)
+    dd = seiswem.DeviceArray(d.shape, d)
+    m1 = whole.migrate(dd, sxs, sys_).download()
+    m2 = small.migrate(dd, sxs, sys_).download()
+    e2 = small.migrate_each(dd, sxs, sys_).download()
+    assert rel_l2(m2, m1) <= 1e-6
+    assert rel_l2(m2, e2.astype(np.float64).sum(0)) <= 1e-6
+    mm = seiswem.DeviceArray(refl.shape, refl)
+    r1 = whole.model(mm, sxs, sys_).download()
+    r2 = small.model(mm, sxs, sys_).download()
+    assert rel_l2(r2, r1) <= 1e-6
+
+
+def test_config3_full_size_dot_and_linearity(gpu):
+    """Config 3 at full size (nkx = nky = 512, nz = 300, 565 frequencies): the adjoint
+    dot-product test and linearity of the migration."""
+    g, vel, refl, wav, sx, sy = synth.config3()
+    a, b, e = dot_test(g, vel, wav, sx, sy, 7)
+    print(f"config3 dot: {a:.9e} {b:.9e} rel {e:.3e}", flush=True)
+    assert e <= DOT_TOL
+    plan = seiswem.Plan(g, vel, wav)
+    rng = np.random.default_rng(8)
+    d1 = rng.standard_normal((1, g.nm, g.nt)).astype(np.float32)
+    d2 = rng.standard_normal((1, g.nm, g.nt)).astype(np.float32)
+    m1 = plan.migrate(seiswem.DeviceArray(d1.shape, d1), sx, sy).download()
+    m2 = plan.migrate(seiswem.DeviceArray(d2.shape, d2), sx, sy).download()
+    m12 = plan.migrate(seiswem.DeviceArray(d1.shape, 0.5 * d1 - d2), sx, sy).download()
+    err = rel_l2(m12, 0.5 * m1.astype(np.float64) - m2)
+    print(f"config3 linearity rel-L2 {err:.3e}", flush=True)
+    assert err <= 1e-5

@@ -252,3 +252,62 @@ def test_config1_kinematics_fp64():
         assert np.argmax(np.abs(win)) == 3 and np.sign(win[3]) == sgn
     img32 = O.wem(p, vel, wav, 1280.0, d=d.astype(np.float32), adj=True, f64=False)
     assert rel_l2(img32, img) <= 1e-4
+
+
+# ----------------------------------------------------------------------------- 3-D (nmy > 1)
+def plane_wave_3d(nmx, nmy, dmx, dmy, v, f_bin, kx_bin, ky_bin):
+    p = O.WemParams(nt=64, dt=0.004, nmx=nmx, dmx=dmx, nmy=nmy, dmy=dmy, nz=3, dz=10.0, fmin=0, fmax=125, padx=1,
+                    pspi=False)
+    vel = np.full((nmx * nmy, 3), v, np.float32)
+    X, Y = np.meshgrid(np.arange(nmx), np.arange(nmy), indexing="ij")
+    u = np.exp(2j * np.pi * (kx_bin * X / nmx + ky_bin * Y / nmy)).reshape(-1)
+    s = O.sampling(p)
+    w = 2 * np.pi * f_bin / (s.ntfft * float(np.float32(p.dt)))
+    kx = 2 * np.pi * kx_bin / (nmx * float(np.float32(dmx)))
+    ky = 2 * np.pi * ky_bin / (nmy * float(np.float32(dmy)))
+    return p, vel, u, w, kx, ky
+
+
+@pytest.mark.parametrize("kx_bin,ky_bin", [(0, 0), (3, 0), (0, 5), (4, -3)])
+def test_ssf_plane_wave_3d(kx_bin, ky_bin):
+    """An on-grid 3-D plane wave exp(i (kx x + ky y)) is multiplied by exp(i kz dz) with
+    kz = sqrt(w^2/v^2 - kx^2 - ky^2): a swapped dkx/dky or a dropped ky term fails it
+    (dmx != dmy, nmx != nmy)."""
+    p, vel, u, w, kx, ky = plane_wave_3d(32, 16, 10.0, 20.0, 2000.0, 20, kx_bin, ky_bin)
+    kz = np.sqrt((w / 2000.0) ** 2 - kx ** 2 - ky ** 2)
+    for dzs in (10.0, -10.0):
+        out = O.step(p, vel, u, 20, 1, dzs, adj=True)
+        assert np.max(np.abs(out - u * np.exp(1j * kz * dzs))) <= 1e-12
+
+
+def small3d_oracle(nx=24, ny=16, nz=10, nt=64, dmx=10.0, dmy=15.0, seed=0):
+    p = O.WemParams(nt=nt, dt=0.004, nmx=nx, dmx=dmx, nmy=ny, dmy=dmy, nz=nz, dz=10.0, fmin=3, fmax=40, pspi=False)
+    rng = np.random.default_rng(seed)
+    X, Y, Z = np.meshgrid(np.arange(nx) * dmx, np.arange(ny) * dmy, np.arange(nz) * 10.0, indexing="ij")
+    vel = (1700 + 0.5 * Z + 300 * np.exp(-((X - 100) ** 2 + (Y - 120) ** 2 + (Z - 50) ** 2) / (2 * 60.0 ** 2)))
+    wav = np.zeros(nt, np.float32)
+    wav[:15] = np.hanning(15)
+    return p, vel.reshape(nx * ny, nz).astype(np.float32), wav, rng
+
+
+def test_dot_product_fp64_3d():
+    p, vel, wav, rng = small3d_oracle()
+    d = rng.standard_normal((p.nm, p.nt))
+    m = rng.standard_normal((p.nm, p.nz))
+    a = np.sum(d * O.wem(p, vel, wav, 100.0, 120.0, m=m, adj=False))
+    b = np.sum(m * O.wem(p, vel, wav, 100.0, 120.0, d=d, adj=True))
+    assert abs(a - b) / abs(b) <= 1e-12
+
+
+def test_xy_transposition_3d():
+    """Swapping the roles of x and y (sizes, spacings, velocity, data and source position)
+    transposes the image: a transposed operand in the 2-D transform or the trace index fails it."""
+    p, vel, wav, rng = small3d_oracle()
+    nx, ny, nz = p.nmx, p.nmy, p.nz
+    d = rng.standard_normal((p.nm, p.nt))
+    m = O.wem(p, vel, wav, 100.0, 120.0, d=d, adj=True)
+    q = O.WemParams(nt=p.nt, dt=p.dt, nmx=ny, dmx=p.dmy, nmy=nx, dmy=p.dmx, nz=nz, dz=p.dz, fmin=p.fmin, fmax=p.fmax,
+                    pspi=False)
+    tr = lambda a: a.reshape(nx, ny, -1).transpose(1, 0, 2).reshape(nx * ny, -1)  # noqa: E731
+    mt = O.wem(q, tr(vel), wav, 120.0, 100.0, d=tr(d), adj=True)
+    assert np.max(np.abs(mt - tr(m))) <= 1e-10 * np.max(np.abs(m))
