@@ -1,4 +1,7 @@
-"""Benchmark: migrated shots/s of the 2-D PSPI workload on one MI355X.
+"""Benchmark: migrated shots/s on one MI355X.
+
+Default workload (--workload pspi2d) is the headline one; --workload ssf3d and --workload
+serving measure the 3-D and the serving workloads (one JSON line each).
 
 Workload (config 2): nx=1024, dx=10 m; nz=500, dz=5 m; nt=2000, dt=2 ms; 0-80 Hz (641 bins);
 4 reference velocities; 1800 m/s + 12 Gaussian blobs; 64 shots 160 m apart; 25 Hz Ricker.  The
@@ -77,6 +80,128 @@ def traffic_per_launch(nshot):
     return (fetch + write) * 1024
 
 
+def bench_ssf3d(args):
+    """3-D split-step migration of one shot (nx=ny=256, dx=dy=12.5 m; nz=300, dz=10 m; nt=1500,
+    dt=4 ms; 3-50 Hz = 565 bins; nkx=nky=512).  HBM-bound: algorithmic bytes per (w, iz) are
+    64 nk (two wavefields x two passes x read+write of the padded complex slice)."""
+    import seiswem
+    from seiswem import synth
+
+    g, vel, refl, wav, sx, sy = synth.config3()
+    plan = seiswem.Plan(g, vel, wav)
+    mm = seiswem.DeviceArray(refl.shape, refl)
+    d = plan.model(mm, sx, sy)
+    seiswem.synchronize()
+    m = seiswem.DeviceArray((g.nm, g.nz))
+    for _ in range(args.warmup):
+        plan.migrate(d, sx, sy, m=m)
+    seiswem.synchronize()
+    march_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.migrate(d, sx, sy, m=m)
+        march_ms += plan.timing()["march_ms"]
+    seiswem.synchronize()
+    elapsed = time.perf_counter() - t0
+    s = g.sampling()
+    nf, nk = s["ifmax"] - s["ifmin"], s["nkx"] * s["nky"]
+    byts = 64.0 * nk * nf * g.nz
+    achieved = byts / (march_ms / args.steps / 1e3) / 1e9
+    # adjoint dot test at full size
+    rng = np.random.default_rng(7)
+    mr = rng.standard_normal((g.nm, g.nz)).astype(np.float32)
+    dr = rng.standard_normal((1, g.nm, g.nt)).astype(np.float32)
+    lm = plan.model(seiswem.DeviceArray(mr.shape, mr), sx, sy).download()
+    lhd = plan.migrate(seiswem.DeviceArray(dr.shape, dr), sx, sy).download()
+    a, b = float(np.sum(dr.astype(np.float64) * lm)), float(np.sum(mr.astype(np.float64) * lhd))
+    out = {
+        "metric": "migrated shots/sec (3D split-step) at <=1e-4 rel-L2 vs FP64", "value": args.steps / elapsed,
+        "unit": "shots/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (3-D generator; record by the library's own demigration)",
+        "config": {"workload": "3-D SSF migration, 1 shot, nx=ny=256 nz=300 nt=1500 3-50 Hz", "nkx": s["nkx"],
+                   "nky": s["nky"], "nf": nf, "nz": g.nz},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "march3d x-pass + y-pass",
+                     "algorithmic_bytes_per_shot": byts},
+        "dot_test_rel": abs(a - b) / abs(b),
+    }
+    print(json.dumps(out))
+
+
+def bench_serving(args):
+    """Serving workload: Poisson arrivals (default 50 requests/s) of mixed 2-D / 3-D SSF
+    migration requests (80 % 2-D: nx 128..1024, nz 200/400, nt 1000..4000; 20 % 3-D: 128^2 or
+    256^2 x 200).  Reports p50 / p99 latency (submission -> image ready) at that rate, the
+    saturated served throughput (all requests queued at once) and the plain per-request path
+    (plan, upload, migrate, download, free; one request at a time) on a sample of the same
+    requests."""
+    import threading
+
+    import seiswem
+    from seiswem import synth
+
+    records = {}
+    reqs = [synth.config4_request(i, records=records) for i in range(args.requests)]
+    arrivals = synth.poisson_arrivals(args.requests, args.rate)
+    srv = seiswem.Server(streams=4, max_inflight=8)
+    # warm-up: one request of every band, so the band bases are resident as in steady state
+    seen = {}
+    for r in reqs:
+        seen.setdefault((r[0].nt, r[0].nmy > 1), r)
+    for r in seen.values():
+        srv.wait(srv.submit(*r))
+
+    def run(times):
+        ids = [None] * len(reqs)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, r in enumerate(reqs):
+                dt = times[i] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = srv.submit(*r)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, done = [], []
+        for i in range(len(reqs)):
+            while ids[i] is None:
+                time.sleep(1e-4)
+            _, st = srv.wait(ids[i])
+            if st["state"] != seiswem.REQ_DONE:
+                raise RuntimeError(st["message"])
+            lat.append(st["done_ms"] - st["submit_ms"])
+            done.append(st["done_ms"])
+        th.join()
+        return np.array(lat), time.perf_counter() - t0
+
+    lat, wall = run(arrivals)
+    _, wall_sat = run(np.zeros(len(reqs)))
+    served = len(reqs) / wall_sat
+    srv.close()
+    k = min(args.plain_sample, len(reqs))
+    t0 = time.perf_counter()
+    for r in reqs[:k]:
+        seiswem.ShotProfileWEM(r[3], True, vel=r[1], wav=r[2], geom=r[0], sx=r[4], sy=r[5])
+    plain = k / (time.perf_counter() - t0)
+    n3 = sum(1 for r in reqs if r[0].nmy > 1)
+    out = {
+        "metric": "migrated shots/sec (2D & 3D) at <=1e-4 rel-L2 vs FP64; p50/p99 request latency",
+        "value": served, "unit": "shots/s", "n_gpus": 1, "steps": len(reqs), "warmup": len(seen),
+        "ms_per_step": 1e3 / served, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (per-request seeded velocity; random records)",
+        "config": {"workload": f"serving: {len(reqs)} requests ({n3} 3-D), Poisson {args.rate:g}/s",
+                   "streams": 4, "max_inflight": 8},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "mean": float(lat.mean()), "max": float(lat.max())},
+        "poisson_run_s": wall, "saturated_run_s": wall_sat,
+        "plain_shots_per_s": plain, "plain_sample": k, "served_vs_plain": served / plain,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,14 +210,23 @@ def main():
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--freqs-per-block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving"), default="pspi2d")
+    ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--plain-sample", type=int, default=24, help="serving: requests timed on the plain path")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy a 1 GiB device buffer once (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not run in this build (shots shard across GPUs "
-              "with one image all-reduce, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not run in this build (shots would shard across "
+              "GPUs with one image all-reduce)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "ssf3d":
+        return bench_ssf3d(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     import seiswem
     from seiswem import synth

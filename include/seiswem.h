@@ -150,6 +150,68 @@ int swem_plan_timing(swem_plan* plan, swem_timing* out);
 /* Device bytes held by the plan's tables and its largest work chunk. */
 int swem_plan_bytes(const swem_plan* plan, size_t* table_bytes, size_t* work_bytes);
 
+/* ------------------------------------------------ serving loop (continuous batching) */
+/* No counterpart in the reference, whose driver runs one shotwem process per shot
+ * (ShotProfileWEM.jl:104-128, 288): these are the batched / served entry points of the
+ * drop-in.  A server owns one device arena, one pinned staging arena, a few streams and a worker
+ * thread.  Migration requests of any shape are submitted from any thread (inputs are copied);
+ * the worker admits them as arena memory and in-flight slots free up, runs each with its own
+ * plan on one of the streams (requests overlap on the device) and keeps the Fourier bases of
+ * each band resident.  Each result is bit-identical to the same request served alone. */
+#define SWEM_REQ_PENDING 0
+#define SWEM_REQ_RUNNING 1
+#define SWEM_REQ_DONE 2
+#define SWEM_REQ_FAILED 3
+
+typedef struct swem_server swem_server;
+typedef struct swem_server_config {
+    int device;
+    size_t device_bytes;    /* device arena; 0 = min(64 GiB, free/2) */
+    size_t pinned_bytes;    /* pinned staging arena; 0 = 4 GiB */
+    int streams;            /* HIP streams requests are spread over (default 4) */
+    int max_inflight;       /* requests on the device at once (default 8) */
+    int max_pending;        /* queued requests before submit returns SWEM_EBUSY (default 4096) */
+    double skip_window_ms;  /* a head-of-queue request that does not fit lets later ones pass
+                               for this long after its submission (default 500) */
+} swem_server_config;
+
+typedef struct swem_request {   /* one migration request: the arguments of swem_shots, adj=1 */
+    swem_geometry g;
+    int pspi_mode, precision;
+    const float* vel;       /* [nm][nz] */
+    const float* wav;       /* [nt] */
+    const float* d;         /* [nshot][nm][nt] */
+    const float* sx;        /* [nshot] */
+    const float* sy;        /* [nshot] or NULL */
+    int nshot;
+} swem_request;
+
+typedef struct swem_request_status {
+    int state;              /* SWEM_REQ_* */
+    int error;              /* status code of a failed request */
+    double submit_ms, admit_ms, done_ms;   /* server clock (ms since creation) */
+    char message[256];
+} swem_request_status;
+
+typedef struct swem_server_stats {
+    long long submitted, completed, failed;
+    int pending, inflight;
+    size_t device_bytes, device_bytes_used, device_bytes_peak, basis_bytes;
+    double busy_ms;         /* time with at least one request on the device */
+} swem_server_stats;
+
+void swem_server_config_init(swem_server_config* cfg);
+int swem_server_create(const swem_server_config* cfg, swem_server** out);
+/* Finishes the admitted and queued requests, then frees everything. */
+int swem_server_destroy(swem_server* server);
+/* Queues a request; *id names it in poll / wait.  SWEM_EBUSY when the queue is full. */
+int swem_batch_submit(swem_server* server, const swem_request* req, long long* id);
+int swem_batch_poll(swem_server* server, long long id, swem_request_status* status);
+/* Blocks until the request is done, copies its stacked image [nm][nz] into m (if not NULL),
+ * fills status and forgets the request.  Returns the request's error code if it failed. */
+int swem_batch_wait(swem_server* server, long long id, float* m, swem_request_status* status);
+int swem_server_get_stats(swem_server* server, swem_server_stats* out);
+
 /* ---------------------------------------------------------- device memory helpers */
 int swem_malloc(void** ptr, size_t bytes);
 int swem_free(void* ptr);

@@ -1,6 +1,7 @@
 // swem_host.cpp -- plans (device-resident tables and work buffers), the in-memory C ABI
 // (swem_wem, swem_shots, swem_plan_*) and the status helpers.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -8,11 +9,21 @@
 #include <string>
 #include <vector>
 
-#include "swem_internal.hpp"
+#include "swem_plan.hpp"
 
 namespace swem {
 
 static thread_local std::string g_err;
+thread_local DevAllocator* tl_dev_alloc = nullptr;
+thread_local HostStager* tl_stager = nullptr;
+
+int upload_bytes(void* dev, const void* host, size_t bytes)
+{
+    if (!bytes) return SWEM_OK;
+    if (tl_stager) return tl_stager->put(dev, host, bytes);
+    SWEM_HIP(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+    return SWEM_OK;
+}
 
 int set_error(int code, const char* fmt, ...)
 {
@@ -33,36 +44,22 @@ int hip_error(hipError_t e, const char* what, const char* file, int line)
 
 static const double kPi = 3.141592653589793;
 
-// A device allocation owned by a plan.
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    int alloc(size_t count)
-    {
-        if (count <= n && p) return SWEM_OK;
-        release();
-        if (count == 0) return SWEM_OK;
-        SWEM_HIP(hipMalloc((void**)&p, count * sizeof(T)));
-        n = count;
-        return SWEM_OK;
-    }
-    int upload(const T* h, size_t count)
-    {
-        SWEM_TRY(alloc(count));
-        if (count) SWEM_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
-        return SWEM_OK;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    size_t bytes() const { return n * sizeof(T); }
-};
+static int check_device_uncached(int dev);
 
+// A device that passed the check once is current and usable; only hipSetDevice is repeated.
 static int check_device(int dev)
+{
+    static std::atomic<unsigned long long> ok_mask{0};
+    if (dev >= 0 && dev < 64 && (ok_mask.load() >> dev) & 1ull) {
+        SWEM_HIP(hipSetDevice(dev));
+        return SWEM_OK;
+    }
+    SWEM_TRY(check_device_uncached(dev));
+    if (dev >= 0 && dev < 64) ok_mask.fetch_or(1ull << dev);
+    return SWEM_OK;
+}
+
+static int check_device_uncached(int dev)
 {
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
@@ -80,38 +77,6 @@ static int check_device(int dev)
 }  // namespace swem
 
 using namespace swem;
-
-struct swem_plan {
-    swem_geometry g{};
-    swem_options opt{};
-    Sampling s{};
-    int nm = 0, nf = 0, nkx = 0, T = 0, F = 1, Ffwd = 1, mode = 0;
-    // frequency chunks: [c * nfc, min(nf, (c + 1) * nfc)); the phase tables hold one chunk
-    int nfc = 0, nchunks = 1, resident = -1;
-    bool three_d = false;
-    int F3 = 4, F3f = 8;  // frequencies per 3-D y-pass workgroup (migration, modelling)
-    std::vector<float> vel;  // host copy (source of every table)
-    // tables
-    DevBuf<float2> tw, twy, spec_s, ktab, stab;
-    DevBuf<float> taper, taper_y, p2f, pdf, w2, basis_adj, basis_fwd;
-    DevBuf<double> p2, pd;
-    DevBuf<uint8_t> sel, sel2;
-    DevBuf<uint32_t> used;
-    // work
-    DevBuf<float2> spec_r, smig, dgw, W;
-    DevBuf<float> partial, mT;
-    DevBuf<int> igx;
-    int* igx_pinned = nullptr;
-    int igx_cap = 0;
-    size_t budget = 0;
-    // timing of the last call
-    std::vector<hipEvent_t> events;
-    int nev = 0;
-    std::vector<int> ev_kind;  // stage of the interval that ends at event i (0 spectra, 1 march, 2 reduce)
-    int last_launches = 0;
-    double last_flops = 0;
-    hipStream_t last_stream = nullptr;
-};
 
 namespace swem {
 
@@ -139,9 +104,40 @@ static int choose_group(const swem_plan* p, int per_freq, int forced)
     return F;
 }
 
+// Fourier bases of the band (f_op, wem.c:480-514, with the w^2 of wem.c:129,182 and the
+// 1/sqrt(ntfft) of wem.c:231-232,278), built in FP64.  Angle index (iw*t) mod n exact in integers.
+int build_bases(const swem_geometry& g, const Sampling& sm, int nf, Bases& b)
+{
+    const int nt = g.nt, n = sm.fop_n;
+    const double rnt = 1.0 / std::sqrt((double)sm.ntfft);
+    const double dw = 2.0 * kPi / ((double)sm.ntfft * (double)g.dt);
+    std::vector<float> badj((size_t)nt * 2 * nf), bsrc((size_t)nt * 2 * nf), bfwd((size_t)2 * nf * nt);
+    for (int f = 0; f < nf; f++) {
+        const int iw = sm.ifmin + f;
+        const double w = iw * dw, w2s = w * w * rnt;
+        const double cf = (iw == 0 || 2 * iw == n) ? 1.0 : 2.0;
+        for (int t = 0; t < nt; t++) {
+            const long long e = ((long long)iw * t) % n;
+            const double ang = 2.0 * kPi * (double)e / (double)n;
+            const double c = std::cos(ang), sn = std::sin(ang);
+            badj[(size_t)t * 2 * nf + 2 * f] = (float)(w2s * c);
+            badj[(size_t)t * 2 * nf + 2 * f + 1] = (float)(-w2s * sn);
+            bsrc[(size_t)t * 2 * nf + 2 * f] = (float)(rnt * c);
+            bsrc[(size_t)t * 2 * nf + 2 * f + 1] = (float)(-rnt * sn);
+            bfwd[(size_t)(2 * f) * nt + t] = (float)(cf * w2s * c);
+            bfwd[(size_t)(2 * f + 1) * nt + t] = (float)(-cf * w2s * sn);
+        }
+    }
+    SWEM_TRY(b.adj.upload(badj.data(), badj.size()));
+    SWEM_TRY(b.fwd.upload(bfwd.data(), bfwd.size()));
+    SWEM_TRY(b.src.upload(bsrc.data(), bsrc.size()));
+    return SWEM_OK;
+}
+
 static int build_tables(swem_plan* p, const float* wav)
 {
     const swem_geometry& g = p->g;
+    const float* bsrc = nullptr;
     const int nmx = g.nmx, nz = g.nz, nm = p->nm, nf = p->nf, nt = g.nt, nkx = p->nkx;
     // twiddles exp(-2 pi i m / nkx) in FP64
     std::vector<float2> tw(nkx);
@@ -237,40 +233,23 @@ static int build_tables(swem_plan* p, const float* wav)
         SWEM_TRY(p->w2.upload(w2.data(), w2.size()));
         SWEM_TRY(p->used.upload(used.data(), used.size()));
     }
-    // Fourier bases of the band (f_op, wem.c:480-514, with the w^2 of wem.c:129,182 and the
-    // 1/sqrt(ntfft) of wem.c:231-232,278).  Angle index (iw*t) mod n exact in integers.
-    const int n = p->s.fop_n;
-    const double rnt = 1.0 / std::sqrt((double)p->s.ntfft);
-    const double dw = 2.0 * kPi / ((double)p->s.ntfft * (double)g.dt);
-    std::vector<float> badj((size_t)nt * 2 * nf), bsrc((size_t)nt * 2 * nf), bfwd((size_t)2 * nf * nt);
-    for (int f = 0; f < nf; f++) {
-        const int iw = p->s.ifmin + f;
-        const double w = iw * dw, w2s = w * w * rnt;
-        const double cf = (iw == 0 || 2 * iw == n) ? 1.0 : 2.0;
-        for (int t = 0; t < nt; t++) {
-            const long long e = ((long long)iw * t) % n;
-            const double ang = 2.0 * kPi * (double)e / (double)n;
-            const double c = std::cos(ang), sn = std::sin(ang);
-            badj[(size_t)t * 2 * nf + 2 * f] = (float)(w2s * c);
-            badj[(size_t)t * 2 * nf + 2 * f + 1] = (float)(-w2s * sn);
-            bsrc[(size_t)t * 2 * nf + 2 * f] = (float)(rnt * c);
-            bsrc[(size_t)t * 2 * nf + 2 * f + 1] = (float)(-rnt * sn);
-            bfwd[(size_t)(2 * f) * nt + t] = (float)(cf * w2s * c);
-            bfwd[(size_t)(2 * f + 1) * nt + t] = (float)(-cf * w2s * sn);
-        }
+    if (p->basis_cache) {
+        const Bases* b = p->basis_cache->find_or_build(g, p->s, nf);
+        if (!b) return SWEM_ENOMEM;
+        p->basis_adj = b->adj.p;
+        p->basis_fwd = b->fwd.p;
+        bsrc = b->src.p;
+    } else {
+        SWEM_TRY(build_bases(g, p->s, nf, p->own_bases));
+        p->basis_adj = p->own_bases.adj.p;
+        p->basis_fwd = p->own_bases.fwd.p;
+        bsrc = p->own_bases.src.p;
     }
-    SWEM_TRY(p->basis_adj.upload(badj.data(), badj.size()));
-    SWEM_TRY(p->basis_fwd.upload(bfwd.data(), bfwd.size()));
     // source spectrum (wem.c:115-119) through the same transform on the device
-    DevBuf<float> dsrc, dwav;
-    SWEM_TRY(dsrc.upload(bsrc.data(), bsrc.size()));
-    SWEM_TRY(dwav.upload(wav, nt));
+    SWEM_TRY(p->wavd.upload(wav, nt));
     SWEM_TRY(p->spec_s.alloc(nf));
-    SWEM_TRY(launch_sgemm(1, 2 * nf, nt, dwav.p, nt, dsrc.p, 2 * nf, reinterpret_cast<float*>(p->spec_s.p), 2 * nf,
-                          nullptr));
-    SWEM_HIP(hipDeviceSynchronize());
-    dsrc.release();
-    dwav.release();
+    SWEM_TRY(launch_sgemm(1, 2 * nf, nt, p->wavd.p, nt, bsrc, 2 * nf, reinterpret_cast<float*>(p->spec_s.p), 2 * nf,
+                          p->build_stream));
     return SWEM_OK;
 }
 
@@ -312,7 +291,8 @@ static int ensure_chunk(swem_plan* p, int c, hipStream_t st)
     return SWEM_OK;
 }
 
-static int alloc_tables(swem_plan* p)
+// Frequencies per phase-table chunk of a 2-D plan under its table cap.
+static void choose_chunks(swem_plan* p)
 {
     const size_t per = table_bytes_per_freq(p);
     size_t cap = p->opt.table_bytes;
@@ -325,12 +305,14 @@ static int alloc_tables(swem_plan* p)
     if (nfc < 1) nfc = 1;
     p->nfc = std::min(nfc, p->nf);
     p->nchunks = (p->nf + p->nfc - 1) / p->nfc;
+}
+
+static int alloc_tables(swem_plan* p)
+{
     const size_t nref = p->g.pspi ? p->g.nref : 1, nkh = p->nkx / 2 + 1;
     SWEM_TRY(p->ktab.alloc((size_t)p->nfc * p->g.nz * nref * nkh));
     if (!p->g.pspi) SWEM_TRY(p->stab.alloc((size_t)p->nfc * p->g.nz * p->g.nmx));
-    SWEM_TRY(ensure_chunk(p, 0, nullptr));
-    SWEM_HIP(hipDeviceSynchronize());
-    return SWEM_OK;
+    return ensure_chunk(p, 0, p->build_stream);
 }
 
 static size_t default_budget()
@@ -353,6 +335,7 @@ static int plan_events(swem_plan* p, int n)
 
 static int record(swem_plan* p, int kind, hipStream_t st)
 {
+    if (p->served) return SWEM_OK;
     SWEM_TRY(plan_events(p, p->nev + 1));
     p->ev_kind[p->nev] = kind;
     SWEM_HIP(hipEventRecord(p->events[p->nev], st));
@@ -362,6 +345,14 @@ static int record(swem_plan* p, int kind, hipStream_t st)
 
 static int set_igx(swem_plan* p, const float* sx, const float* sy, int nshot, hipStream_t st)
 {
+    p->igx_host.resize(nshot);
+    for (int i = 0; i < nshot; i++) {
+        int ix, iy;
+        SWEM_TRY(source_index(p->g, sx[i], sy ? sy[i] : 0.f, ix, iy));
+        p->igx_host[i] = ix * p->g.nmy + iy;
+    }
+    SWEM_TRY(p->igx.alloc(nshot));
+    if (p->served) return upload_bytes(p->igx.p, p->igx_host.data(), sizeof(int) * nshot);
     if (p->igx_cap < nshot) {
         if (p->igx_pinned) (void)hipHostFree(p->igx_pinned);
         p->igx_pinned = nullptr;
@@ -370,12 +361,7 @@ static int set_igx(swem_plan* p, const float* sx, const float* sy, int nshot, hi
     }
     // the previous call's copy out of the pinned buffer must have finished
     if (p->last_stream || p->nev) SWEM_HIP(hipStreamSynchronize(p->last_stream));
-    for (int i = 0; i < nshot; i++) {
-        int ix, iy;
-        SWEM_TRY(source_index(p->g, sx[i], sy ? sy[i] : 0.f, ix, iy));
-        p->igx_pinned[i] = ix * p->g.nmy + iy;
-    }
-    SWEM_TRY(p->igx.alloc(nshot));
+    std::copy(p->igx_host.begin(), p->igx_host.end(), p->igx_pinned);
     SWEM_HIP(hipMemcpyAsync(p->igx.p, p->igx_pinned, sizeof(int) * nshot, hipMemcpyHostToDevice, st));
     return SWEM_OK;
 }
@@ -442,7 +428,7 @@ static int migrate_impl(swem_plan* p, const float* d_dev, const float* sx, const
     SWEM_TRY(record(p, -1, st));
     for (int s0 = 0; s0 < nshot; s0 += chunk) {
         const int ns = std::min(chunk, nshot - s0);
-        SWEM_TRY(launch_sgemm(ns * (int)nm, 2 * (int)nf, (int)nt, d_dev + (size_t)s0 * nm * nt, (int)nt, p->basis_adj.p,
+        SWEM_TRY(launch_sgemm(ns * (int)nm, 2 * (int)nf, (int)nt, d_dev + (size_t)s0 * nm * nt, (int)nt, p->basis_adj,
                               2 * (int)nf, reinterpret_cast<float*>(p->spec_r.p), 2 * (int)nf, st));
         SWEM_TRY(record(p, 0, st));
         for (int c = 0; c < p->nchunks; c++) {
@@ -519,11 +505,11 @@ static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, con
 {
     const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
     const int F = p->F3;
-    const size_t per_item = 2 * (size_t)p->g.nmx * p->s.nky * sizeof(float2) + nz * nm * sizeof(float) / F + 1;
+    const size_t per_item = 2 * (size_t)p->g.nmx * p->s.nky * sizeof(float2) + nm * sizeof(float) / F + 1;
     const int nfc = chunk3(p, per_item, F);
     const size_t gmax = (nfc + F - 1) / F;
     SWEM_TRY(p->W.alloc((size_t)nfc * 2 * p->g.nmx * p->s.nky));
-    SWEM_TRY(p->partial.alloc(gmax * nz * nm));
+    SWEM_TRY(p->partial.alloc(gmax * nm));
     SWEM_TRY(p->spec_r.alloc(nm * nf));
     SWEM_TRY(set_igx(p, sx, sy, nshot, st));
     p->nev = 0;
@@ -532,20 +518,22 @@ static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, con
     p->last_stream = st;
     SWEM_TRY(record(p, -1, st));
     for (int s = 0; s < nshot; s++) {
-        SWEM_TRY(launch_sgemm((int)nm, 2 * (int)nf, (int)nt, d_dev + (size_t)s * nm * nt, (int)nt, p->basis_adj.p,
+        SWEM_TRY(launch_sgemm((int)nm, 2 * (int)nf, (int)nt, d_dev + (size_t)s * nm * nt, (int)nt, p->basis_adj,
                               2 * (int)nf, reinterpret_cast<float*>(p->spec_r.p), 2 * (int)nf, st));
         SWEM_TRY(record(p, 0, st));
         for (int f0 = 0; f0 < (int)nf; f0 += nfc) {
             March3DArgs a = base3(p);
             a.f0 = f0;
             a.nitem = std::min(nfc, (int)nf - f0);
-            a.igx = p->igx_pinned[s];
+            a.igx = p->igx_host[s];
             a.adj = 1;
             a.spec_r = p->spec_r.p;
             a.W = p->W.p;
             a.partial = p->partial.p;
             a.conj_src = 1;
             a.conj_rec = 0;
+            float* out = per_shot ? m_dev + (size_t)s * nm * nz : m_dev;
+            const int acc = f0 > 0 || (!per_shot && (accumulate || s > 0));
             SWEM_TRY(launch_march3d(0, a, F, 0, st));
             for (int iz = 0; iz < (int)nz; iz++) {
                 a.iz = iz;
@@ -556,14 +544,12 @@ static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, con
                 a.wmask = (a.ds ? 1 : 0) | (a.dr ? 2 : 0);
                 if (a.wmask) SWEM_TRY(launch_march3d(1, a, F, 0, st));
                 SWEM_TRY(launch_march3d(2, a, F, 0, st));
+                // this depth's image, summed over the chunk's frequency groups
+                SWEM_TRY(launch_reduce_depth(p->partial.p, (a.nitem + F - 1) / F, (int)nm, out + iz, (int)nz, acc, st));
             }
             SWEM_TRY(record(p, 1, st));
             p->last_launches++;
             p->last_flops += step_flops(p) * (double)a.nitem * nz;
-            float* out = per_shot ? m_dev + (size_t)s * nm * nz : m_dev;
-            const int acc = f0 > 0 || (!per_shot && (accumulate || s > 0));
-            SWEM_TRY(launch_reduce_image(p->partial.p, 1, (a.nitem + F - 1) / F, (int)nz, (int)nm, out, acc, 0, st));
-            SWEM_TRY(record(p, 2, st));
         }
     }
     return SWEM_OK;
@@ -593,7 +579,7 @@ static int model3d_impl(swem_plan* p, const float* m_dev, const float* sx, const
             March3DArgs a = base3(p);
             a.f0 = f0;
             a.nitem = std::min(nfc, (int)nf - f0);
-            a.igx = p->igx_pinned[s];
+            a.igx = p->igx_host[s];
             a.adj = 0;
             a.W = p->W.p;
             a.smig = p->smig.p;
@@ -627,7 +613,7 @@ static int model3d_impl(swem_plan* p, const float* m_dev, const float* sx, const
             p->last_flops += step_flops(p) * (double)a.nitem * nz;
         }
         SWEM_TRY(launch_sgemm((int)nm, (int)nt, 2 * (int)nf, reinterpret_cast<const float*>(p->dgw.p), 2 * (int)nf,
-                              p->basis_fwd.p, (int)nt, d_dev + (size_t)s * nm * nt, (int)nt, st));
+                              p->basis_fwd, (int)nt, d_dev + (size_t)s * nm * nt, (int)nt, st));
         SWEM_TRY(record(p, 2, st));
     }
     return SWEM_OK;
@@ -672,7 +658,7 @@ static int model_impl(swem_plan* p, const float* m_dev, const float* sx, const f
             p->last_flops += step_flops(p) * (double)a.nfc * nz * ns;
         }
         SWEM_TRY(launch_sgemm(ns * (int)nm, (int)nt, 2 * (int)nf, reinterpret_cast<const float*>(p->dgw.p), 2 * (int)nf,
-                              p->basis_fwd.p, (int)nt, d_dev + (size_t)s0 * nm * nt, (int)nt, st));
+                              p->basis_fwd, (int)nt, d_dev + (size_t)s0 * nm * nt, (int)nt, st));
         SWEM_TRY(record(p, 2, st));
     }
     return SWEM_OK;
@@ -708,11 +694,12 @@ int swem_device_count(int* count)
     return SWEM_OK;
 }
 
-int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav, const swem_options* opt,
-                     swem_plan** out)
+}  // extern "C"
+
+namespace swem {
+
+int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swem_options* opt)
 {
-    if (!g || !vel || !wav || !out) return set_error(SWEM_EINVAL, "plan_create: null argument");
-    *out = nullptr;
     swem_options o;
     swem_options_init(&o);
     if (opt) o = *opt;
@@ -721,74 +708,138 @@ int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav,
     if (o.precision != SWEM_PREC_KZ64 && o.precision != SWEM_PREC_FP32)
         return set_error(SWEM_EINVAL, "precision %d", o.precision);
     if (o.variant < SWEM_VARIANT_AUTO || o.variant > SWEM_VARIANT_SUBGROUP16) return set_error(SWEM_EINVAL, "variant %d", o.variant);
-    swem_plan* p = new swem_plan();
     p->g = *g;
     p->opt = o;
-    int rc = compute_sampling(p->g, p->s);
-    if (rc) { delete p; return rc; }
+    SWEM_TRY(compute_sampling(p->g, p->s));
     p->nm = g->nmx * g->nmy;
     p->nf = p->s.ifmax - p->s.ifmin;
     p->nkx = p->s.nkx;
     p->three_d = g->nmy > 1;
     if (p->three_d) {
-        if (g->pspi) { delete p; return set_error(SWEM_EUNSUPPORTED, "3-D PSPI is not available in this build (3-D SSF is)"); }
-        if (g->nmx < 2 || !march3d_supported(p->s.nkx, p->s.nky)) {
-            delete p;
+        if (g->pspi) return set_error(SWEM_EUNSUPPORTED, "3-D PSPI is not available in this build (3-D SSF is)");
+        if (g->nmx < 2 || !march3d_supported(p->s.nkx, p->s.nky))
             return set_error(SWEM_EUNSUPPORTED, "3-D: nkx=%d, nky=%d: this build handles powers of two 128..1024", p->s.nkx,
                              p->s.nky);
-        }
     } else if (!march2d_supported(p->nkx)) {
-        delete p;
         return set_error(SWEM_EUNSUPPORTED, "nkx = padx*nmx = %d: this build handles powers of two 128..4096", p->nkx);
     }
-    if (p->nf < 1) { delete p; return set_error(SWEM_EINVAL, "empty frequency band [ifmin=%d, ifmax=%d)", p->s.ifmin, p->s.ifmax); }
-    if (g->pspi && (g->nref < 2 || g->nref > 31)) { delete p; return set_error(SWEM_EINVAL, "PSPI needs 2 <= nref <= 31 (nref=%d)", g->nref); }
-    if (!(g->dz != 0.f)) { delete p; return set_error(SWEM_EINVAL, "dz must be non-zero"); }
-    rc = check_device(o.device);
-    if (rc) { delete p; return rc; }
+    if (p->nf < 1) return set_error(SWEM_EINVAL, "empty frequency band [ifmin=%d, ifmax=%d)", p->s.ifmin, p->s.ifmax);
+    if (g->pspi && (g->nref < 2 || g->nref > 31)) return set_error(SWEM_EINVAL, "PSPI needs 2 <= nref <= 31 (nref=%d)", g->nref);
+    if (!(g->dz != 0.f)) return set_error(SWEM_EINVAL, "dz must be non-zero");
+    SWEM_TRY(check_device(o.device));
     if (p->three_d) {
         // y-pass workgroups: F frequencies x 2 wavefields (migration), F x 1 (modelling)
         if (o.freqs_per_block > 0) p->F3 = p->F3f = o.freqs_per_block;
-        if (p->F3 < 1 || 128 * p->F3 > 1024 || 64 * p->F3f > 1024) {
-            delete p;
+        if (p->F3 < 1 || 128 * p->F3 > 1024 || 64 * p->F3f > 1024)
             return set_error(SWEM_EINVAL, "3-D: freqs_per_block=%d (1..8)", o.freqs_per_block);
-        }
     } else {
-    p->T = march2d_threads(p->nkx, o.variant);
-    p->F = choose_group(p, 2, o.freqs_per_block);
-    p->Ffwd = choose_group(p, 1, o.freqs_per_block);
-    if (2 * p->F * p->T > march2d_max_block(p->nkx, o.variant) ||
-        march2d_lds_bytes(p->nkx, 2 * p->F, o.variant) > 160 * 1024 ||
-        p->Ffwd * p->T > march2d_max_block(p->nkx, o.variant) ||
-        march2d_lds_bytes(p->nkx, p->Ffwd, o.variant) > 160 * 1024) {
-        delete p;
-        return set_error(SWEM_EINVAL, "freqs_per_block=%d does not fit a workgroup at nkx=%d", o.freqs_per_block, p->s.nkx);
-    }
+        p->T = march2d_threads(p->nkx, o.variant);
+        p->F = choose_group(p, 2, o.freqs_per_block);
+        p->Ffwd = choose_group(p, 1, o.freqs_per_block);
+        if (2 * p->F * p->T > march2d_max_block(p->nkx, o.variant) ||
+            march2d_lds_bytes(p->nkx, 2 * p->F, o.variant) > 160 * 1024 ||
+            p->Ffwd * p->T > march2d_max_block(p->nkx, o.variant) ||
+            march2d_lds_bytes(p->nkx, p->Ffwd, o.variant) > 160 * 1024)
+            return set_error(SWEM_EINVAL, "freqs_per_block=%d does not fit a workgroup at nkx=%d", o.freqs_per_block,
+                             p->s.nkx);
+        choose_chunks(p);
     }
     p->mode = mode_of(p);
     p->vel.assign(vel, vel + (size_t)p->nm * g->nz);
     p->budget = o.scratch_bytes ? o.scratch_bytes : default_budget();
-    rc = build_tables(p, wav);
-    if (!rc && !p->three_d) rc = alloc_tables(p);
-    if (rc) { swem_plan_destroy(p); return rc; }
+    return SWEM_OK;
+}
+
+int plan_build(swem_plan* p, const float* wav)
+{
+    SWEM_TRY(build_tables(p, wav));
+    if (!p->three_d) SWEM_TRY(alloc_tables(p));
+    if (!p->served) SWEM_HIP(hipDeviceSynchronize());
+    return SWEM_OK;
+}
+
+void plan_free(swem_plan* p)
+{
+    if (!p) return;
+    (void)hipSetDevice(p->opt.device);
+    if (!p->served && (p->last_stream || p->nev)) (void)hipStreamSynchronize(p->last_stream);
+    for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
+    if (p->igx_pinned) (void)hipHostFree(p->igx_pinned);
+    p->twy.release(); p->taper_y.release(); p->W.release(); p->wavd.release();
+    p->tw.release(); p->spec_s.release(); p->ktab.release(); p->stab.release(); p->pd.release(); p->pdf.release();
+    p->taper.release(); p->p2f.release(); p->w2.release();
+    p->own_bases.adj.release(); p->own_bases.fwd.release(); p->own_bases.src.release();
+    p->p2.release(); p->sel.release(); p->sel2.release();
+    p->used.release(); p->spec_r.release(); p->smig.release(); p->dgw.release(); p->partial.release();
+    p->mT.release(); p->igx.release();
+    delete p;
+}
+
+int plan_migrate(swem_plan* p, const float* d_dev, const float* sx, const float* sy, int nshot, float* m_dev,
+                 int accumulate, int per_shot, hipStream_t st)
+{
+    return migrate_impl(p, d_dev, sx, sy, nshot, m_dev, accumulate, per_shot, st);
+}
+
+static size_t pad256(size_t b) { return (b + 511) & ~(size_t)255; }
+
+size_t plan_upload_bytes(const swem_plan* p, int nshot)
+{
+    const size_t nz = p->g.nz, nm = p->nm, nmx = p->g.nmx, nref = p->g.pspi ? p->g.nref : 1;
+    size_t b = pad256(p->nkx * 8) + pad256(p->s.nky * 8) + pad256(nmx * 4) + pad256(p->g.nmy * 4) + pad256(p->g.nt * 4);
+    b += pad256(nz * nref * 8) + pad256(nz * nref * 4) + pad256(nz * 4);
+    if (p->g.pspi)
+        b += 2 * pad256(nz * nmx) + pad256(nz * nmx * 4);
+    else
+        b += pad256(nz * nm * 8) + pad256(nz * nm * 4);
+    return b + pad256((size_t)nshot * 4);
+}
+
+size_t plan_migrate_bytes(const swem_plan* p, int nshot)
+{
+    const size_t nz = p->g.nz, nm = p->nm, nmx = p->g.nmx, nf = p->nf, nref = p->g.pspi ? p->g.nref : 1;
+    size_t b = plan_upload_bytes(p, nshot) + pad256(nf * 8);
+    if (!p->basis_cache) b += 3 * pad256((size_t)p->g.nt * 2 * nf * 4);
+    if (p->three_d) {
+        const int F = p->F3;
+        const size_t per_item = 2 * nmx * p->s.nky * sizeof(float2) + nm * sizeof(float) / F + 1;
+        const size_t nfc = chunk3(p, per_item, F), gmax = (nfc + F - 1) / F;
+        b += pad256(nfc * 2 * nmx * p->s.nky * 8) + pad256(gmax * nm * 4) + pad256(nm * nf * 8);
+        return b;
+    }
+    const size_t nkh = p->nkx / 2 + 1;
+    b += pad256((size_t)p->nfc * nz * nref * nkh * 8);
+    if (!p->g.pspi) b += pad256((size_t)p->nfc * nz * nmx * 8);
+    const size_t gmax = (size_t)(p->nfc + p->F - 1) / p->F;
+    const size_t per = nm * nf * sizeof(float2) + gmax * nz * nmx * sizeof(float);
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(nshot, p->budget / std::max<size_t>(per, 1)));
+    b += pad256(chunk * nm * nf * 8) + pad256(chunk * gmax * nz * nmx * 4);
+    return b;
+}
+
+}  // namespace swem
+
+extern "C" {
+
+int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav, const swem_options* opt,
+                     swem_plan** out)
+{
+    if (!g || !vel || !wav || !out) return set_error(SWEM_EINVAL, "plan_create: null argument");
+    *out = nullptr;
+    swem_plan* p = new swem_plan();
+    int rc = plan_setup(p, g, vel, opt);
+    if (!rc) rc = plan_build(p, wav);
+    if (rc) {
+        plan_free(p);
+        return rc;
+    }
     *out = p;
     return SWEM_OK;
 }
 
 int swem_plan_destroy(swem_plan* p)
 {
-    if (!p) return SWEM_OK;
-    (void)hipSetDevice(p->opt.device);
-    if (p->last_stream || p->nev) (void)hipStreamSynchronize(p->last_stream);
-    for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
-    if (p->igx_pinned) (void)hipHostFree(p->igx_pinned);
-    p->twy.release(); p->taper_y.release(); p->W.release();
-    p->tw.release(); p->spec_s.release(); p->ktab.release(); p->stab.release(); p->pd.release(); p->pdf.release();
-    p->taper.release(); p->p2f.release(); p->w2.release();
-    p->basis_adj.release(); p->basis_fwd.release(); p->p2.release(); p->sel.release(); p->sel2.release();
-    p->used.release(); p->spec_r.release(); p->smig.release(); p->dgw.release(); p->partial.release();
-    p->mT.release(); p->igx.release();
-    delete p;
+    plan_free(p);
     return SWEM_OK;
 }
 
@@ -837,7 +888,7 @@ int swem_plan_bytes(const swem_plan* p, size_t* table_bytes, size_t* work_bytes)
     if (table_bytes)
         *table_bytes = p->tw.bytes() + p->spec_s.bytes() + p->ktab.bytes() + p->stab.bytes() + p->pd.bytes() +
                        p->pdf.bytes() + p->taper.bytes() + p->p2f.bytes() +
-                       p->w2.bytes() + p->basis_adj.bytes() + p->basis_fwd.bytes() + p->p2.bytes() + p->sel.bytes() +
+                       p->w2.bytes() + p->own_bases.adj.bytes() + p->own_bases.fwd.bytes() + p->own_bases.src.bytes() + p->wavd.bytes() + p->p2.bytes() + p->sel.bytes() +
                        p->sel2.bytes() + p->used.bytes();
     if (work_bytes)
         *work_bytes = p->spec_r.bytes() + p->smig.bytes() + p->dgw.bytes() + p->partial.bytes() + p->mT.bytes() +

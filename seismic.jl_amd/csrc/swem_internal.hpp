@@ -110,7 +110,7 @@ struct March3DArgs {
     const float2* spec_s;       // [nf]
     const float2* spec_r;       // [nm][nf] this shot's receiver spectra
     float2* W;                  // [nitem][2][nmx][nky] step state (y transformed)
-    float* partial;             // [ngroups][nz][nm] partial images of this shot
+    float* partial;             // [ngroups][nm] partial image of this depth
     const float* mT;            // [nz][nm] image (modelling input)
     float2* smig;               // [nitem][nz][nm] source wavefields (modelling)
     float2* dgw;                // [nm][nf] modelled spectra of this shot
@@ -128,6 +128,8 @@ int launch_sgemm(int M, int Nc, int K, const float* A, int lda, const float* B, 
 // fixed order.  per_shot = 1 writes one image per shot (no stacking).
 int launch_reduce_image(const float* partial, int nshot, int ngroups, int nz, int nmx, float* out, int accumulate,
                         int per_shot, hipStream_t st);
+// One depth of an image: out[x * ldo] (+)= sum_g partial[g][x] (compensated, fixed order).
+int launch_reduce_depth(const float* partial, int ngroups, int n, float* out, int ldo, int accumulate, hipStream_t st);
 // [nm][nz] -> [nz][nm]
 int launch_transpose(const float* in, int rows, int cols, float* out, hipStream_t st);
 

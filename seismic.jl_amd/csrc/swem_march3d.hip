@@ -255,7 +255,7 @@ __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
         for (int q = 0; q < P; ++q) red[t + 64 * q] = cc[q];
     }
     __syncthreads();
-    float* part = a.partial + ((size_t)g * a.nz + a.iz) * a.nm + (size_t)x * a.nmy;
+    float* part = a.partial + (size_t)g * a.nm + (size_t)x * a.nmy;
     for (int y = threadIdx.x; y < a.nmy; y += blockDim.x) {
         float sum = 0.f;
         for (int ff = 0; ff < F; ++ff) sum += reinterpret_cast<const float*>(smem + (2 * ff + 1) * L)[y];

@@ -108,6 +108,30 @@ int launch_reduce_image(const float* partial, int nshot, int ngroups, int nz, in
     return SWEM_OK;
 }
 
+// One depth of an image: out[x * ldo] (+)= sum_g partial[g][x], compensated, fixed order.
+__global__ __launch_bounds__(256) void reduce_depth(const float* __restrict__ partial, int ngroups, int n,
+                                                    float* __restrict__ out, int ldo, int accumulate)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    float sum = 0.f, comp = 0.f;
+    for (int g = 0; g < ngroups; ++g) {
+        const float y = partial[(size_t)g * n + x] - comp;
+        const float t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
+    }
+    float* o = out + (size_t)x * ldo;
+    *o = accumulate ? *o + sum : sum;
+}
+
+int launch_reduce_depth(const float* partial, int ngroups, int n, float* out, int ldo, int accumulate, hipStream_t st)
+{
+    hipLaunchKernelGGL(reduce_depth, dim3((n + 255) / 256), dim3(256), 0, st, partial, ngroups, n, out, ldo, accumulate);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
 __global__ __launch_bounds__(256) void transpose_k(const float* __restrict__ in, int rows, int cols,
                                                    float* __restrict__ out)
 {

@@ -27,7 +27,10 @@ EXPORTS = (
     "swem_malloc", "swem_free", "swem_memcpy_htod", "swem_memcpy_dtoh", "swem_memcpy_dtod", "swem_memset", "swem_synchronize",
     "swem_sampling", "swem_source_index", "swem_pspi_tables", "swem_taper",
     "swem_seis_extent_init", "swem_seis_read_header", "swem_seis_read", "swem_seis_write",
+    "swem_server_config_init", "swem_server_create", "swem_server_destroy", "swem_batch_submit",
+    "swem_batch_poll", "swem_batch_wait", "swem_server_get_stats",
 )
+REQ_PENDING, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(4)
 
 
 class SwemError(RuntimeError):
@@ -63,6 +66,39 @@ class Timing(ctypes.Structure):
     _fields_ = [
         ("total_ms", ctypes.c_double), ("spectra_ms", ctypes.c_double), ("march_ms", ctypes.c_double),
         ("reduce_ms", ctypes.c_double), ("march_launches", ctypes.c_int), ("march_flops", ctypes.c_double),
+    ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int), ("device_bytes", ctypes.c_size_t), ("pinned_bytes", ctypes.c_size_t),
+        ("streams", ctypes.c_int), ("max_inflight", ctypes.c_int), ("max_pending", ctypes.c_int),
+        ("skip_window_ms", ctypes.c_double),
+    ]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("g", Geometry), ("pspi_mode", ctypes.c_int), ("precision", ctypes.c_int),
+        ("vel", ctypes.POINTER(ctypes.c_float)), ("wav", ctypes.POINTER(ctypes.c_float)),
+        ("d", ctypes.POINTER(ctypes.c_float)), ("sx", ctypes.POINTER(ctypes.c_float)),
+        ("sy", ctypes.POINTER(ctypes.c_float)), ("nshot", ctypes.c_int),
+    ]
+
+
+class RequestStatus(ctypes.Structure):
+    _fields_ = [
+        ("state", ctypes.c_int), ("error", ctypes.c_int), ("submit_ms", ctypes.c_double),
+        ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double), ("message", ctypes.c_char * 256),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong),
+        ("pending", ctypes.c_int), ("inflight", ctypes.c_int), ("device_bytes", ctypes.c_size_t),
+        ("device_bytes_used", ctypes.c_size_t), ("device_bytes_peak", ctypes.c_size_t),
+        ("basis_bytes", ctypes.c_size_t), ("busy_ms", ctypes.c_double),
     ]
 
 
@@ -130,6 +166,13 @@ def load():
         "swem_seis_read_header": ([ctypes.c_char_p, P(SeisExtent)], ctypes.c_int),
         "swem_seis_read": ([ctypes.c_char_p, f32p, vp, P(SeisExtent)], ctypes.c_int),
         "swem_seis_write": ([ctypes.c_char_p, f32p, vp, P(SeisExtent)], ctypes.c_int),
+        "swem_server_config_init": ([P(ServerConfig)], None),
+        "swem_server_create": ([P(ServerConfig), P(vp)], ctypes.c_int),
+        "swem_server_destroy": ([vp], ctypes.c_int),
+        "swem_batch_submit": ([vp, P(Request), P(ctypes.c_longlong)], ctypes.c_int),
+        "swem_batch_poll": ([vp, ctypes.c_longlong, P(RequestStatus)], ctypes.c_int),
+        "swem_batch_wait": ([vp, ctypes.c_longlong, f32p, P(RequestStatus)], ctypes.c_int),
+        "swem_server_get_stats": ([vp, P(ServerStats)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

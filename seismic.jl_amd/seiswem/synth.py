@@ -115,6 +115,41 @@ def config3(seed: int = 3, nx: int = 256, ny: int = 256, nz: int = 300, nt: int 
     return g, vel, refl.reshape(nx * ny, nz), wav, sx, sy
 
 
+def config4_request(rid: int, seed: int = 4, records: dict | None = None):
+    """Request rid of the serving workload (seeded per request id): 80 % 2-D SSF with
+    nx in {128, 256, 512, 1024}, nz in {200, 400}, nt in {1000, 2000, 4000}, dx = dz = 10 m,
+    dt = 4 ms, 2-60 Hz, blob velocity (the 2-D generator); 20 % 3-D SSF with nx = ny in
+    {128, 256}, nz = 200, nt = 1000, dx = dy = 12.5 m, dz = 10 m, 3-50 Hz, the 3-D generator's
+    velocity.  One shot at the centre.  The records are random (the cost of a migration does not
+    depend on the record values); records of one shape are shared through `records`.
+    Returns (geom, vel, wav, d [1, nm, nt], sx, sy)."""
+    rng = np.random.default_rng([seed, rid])
+    if rng.uniform() < 0.8:
+        nx = int(rng.choice([128, 256, 512, 1024]))
+        nz = int(rng.choice([200, 400]))
+        nt = int(rng.choice([1000, 2000, 4000]))
+        g = WemGeometry(nt=nt, dt=0.004, nmx=nx, dmx=10.0, nz=nz, dz=10.0, fmin=2.0, fmax=60.0, pspi=False)
+        vel = blob_velocity(nx, nz, g.dmx, g.dz, rng)
+        wav = wavelet_trace(nt, g.dt, 20.0)
+        sx, sy = np.array([nx // 2 * g.dmx], np.float32), None
+    else:
+        nx = int(rng.choice([128, 256]))
+        g, vel, _, wav, sx, sy = config3(seed=int(rng.integers(1 << 30)), nx=nx, ny=nx, nz=200, nt=1000)
+    key = (g.nm, g.nt)
+    if records is not None and key in records:
+        d = records[key]
+    else:
+        d = np.random.default_rng(key).standard_normal((1, g.nm, g.nt)).astype(np.float32)
+        if records is not None:
+            records[key] = d
+    return g, vel, wav, d, sx, sy
+
+
+def poisson_arrivals(n: int, rate: float = 50.0, seed: int = 4) -> np.ndarray:
+    """Arrival times (s) of n requests of a Poisson process of the given rate."""
+    return np.cumsum(np.random.default_rng([seed, 1 << 20]).exponential(1.0 / rate, size=n))
+
+
 def config5_shots() -> np.ndarray:
     """Config-1 geometry with 32 shots at sx = 40 + 80 k."""
     return (40.0 + 80.0 * np.arange(32)).astype(np.float32)
