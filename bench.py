@@ -202,6 +202,75 @@ def bench_serving(args):
     print(json.dumps(out))
 
 
+def bench_lsm(args):
+    """Least-squares migration and precision study: the config-1 geometry with 32 shots
+    (sx = 40 + 80 k), 10 CG iterations (ConjugateGradients.jl, mu = 0) with the GPU operator pair,
+    model / data / residual resident on the device.  Observed records: FP64 oracle demigration of
+    the config-1 reflectivity (computed here on the host cores, checked against the fingerprint of
+    the committed FP64 reference).  Variants: FP64 end to end (the oracle; committed result of
+    tests/golden/make_cg_fp64.py), FP32 propagation + FP64 kz tables + compensated sums (the
+    default), naive FP32."""
+    import oracle as O
+    import seiswem
+    from seiswem import solvers, synth
+
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "cg_fp64_config5.npz"))
+    g, vel, refl, wav, _ = synth.config1()
+    sx = synth.config5_shots()
+    p = O.WemParams(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, fmin=g.fmin, fmax=g.fmax, padt=g.padt,
+                    padx=g.padx, pspi=False)
+    cores = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    d_obs = O.shots(p, vel, wav, sx, m=refl, adj=False, f64=True)
+    t_oracle = time.perf_counter() - t0
+    fp_ok = abs(d_obs.sum() - ref["d_sum"]) <= 1e-9 * abs(ref["d_sumsq"]) ** 0.5 * d_obs.size ** 0.5 and \
+        abs((d_obs ** 2).sum() / ref["d_sumsq"] - 1) <= 1e-9
+    d_obs = d_obs.astype(np.float32)
+    img64 = ref["image"]
+    variants = {}
+    rng = np.random.default_rng(5)
+    mr = rng.standard_normal((g.nm, g.nz)).astype(np.float32)
+    dr = rng.standard_normal((sx.size, g.nm, g.nt)).astype(np.float32)
+    for name, prec in (("fp32_prop_fp64_kz_compensated", seiswem.PREC_KZ64), ("naive_fp32", seiswem.PREC_FP32)):
+        plan = seiswem.Plan(g, vel, wav, precision=prec)
+        dd = seiswem.DeviceArray(d_obs.shape, d_obs)
+        solvers.ConjugateGradientsDevice(plan, dd, sx, Niter=1)  # warm-up
+        seiswem.synchronize()
+        t0 = time.perf_counter()
+        m, cost = solvers.ConjugateGradientsDevice(plan, dd, sx, Niter=10)
+        seiswem.synchronize()
+        t_cg = time.perf_counter() - t0
+        img = m.download().astype(np.float64)
+        lm = plan.model(seiswem.DeviceArray(mr.shape, mr), sx).download()
+        lhd = plan.migrate(seiswem.DeviceArray(dr.shape, dr), sx).download()
+        a, b = float(np.sum(dr.astype(np.float64) * lm)), float(np.sum(mr.astype(np.float64) * lhd))
+        napp = 1 + 2 * (len(cost) - 1)
+        variants[name] = {
+            "cost": [float(c) for c in cost],
+            "image_rel_l2_vs_fp64": float(np.linalg.norm(img - img64) / np.linalg.norm(img64)),
+            "dot_test_rel": abs(a - b) / abs(b),
+            "cg_s": t_cg, "s_per_iteration": t_cg / (len(cost) - 1),
+            "shots_per_s": sx.size * napp / t_cg,
+        }
+        plan.close()
+    variants["fp64_oracle"] = {
+        "cost": [float(c) for c in ref["cost"]], "image_rel_l2_vs_fp64": 0.0, "dot_test_rel": None,
+        "shots_per_s": sx.size / t_oracle, "cores": cores,
+        "note": "committed result of tests/golden/make_cg_fp64.py; throughput from one 32-shot FP64 oracle "
+                "demigration on this host",
+    }
+    main_v = variants["fp32_prop_fp64_kz_compensated"]
+    out = {
+        "metric": "LSM: migrated shots/sec at <=1e-4 rel-L2 vs FP64 (10 CG iterations)", "value": main_v["shots_per_s"],
+        "unit": "shots/s", "n_gpus": 1, "steps": 10, "warmup": 1, "ms_per_step": main_v["s_per_iteration"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (config-1 model; records by FP64 oracle demigration)",
+        "config": {"workload": "least-squares migration, config-1 geometry, 32 shots, 10 CG iterations, mu=0"},
+        "records_match_reference": bool(fp_ok), "variants": variants,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,7 +279,7 @@ def main():
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--freqs-per-block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving"), default="pspi2d")
+    ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm"), default="pspi2d")
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--plain-sample", type=int, default=24, help="serving: requests timed on the plain path")
@@ -227,6 +296,8 @@ def main():
         return bench_ssf3d(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "lsm":
+        return bench_lsm(args)
 
     import seiswem
     from seiswem import synth

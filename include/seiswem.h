@@ -221,6 +221,15 @@ int swem_memcpy_dtod(void* dst, const void* src, size_t bytes);
 int swem_memset(void* ptr, int value, size_t bytes);
 int swem_synchronize(void);
 
+/* Device vector operations of the least-squares solver, for vectors resident in device memory
+ * (models [nm][nz], records [nshot][nm][nt]):
+ *   swem_vec_cgstep  m1 = a*m1 + b*m2                       (CGStep, src/Solvers/CGStep.jl:1-4)
+ *   swem_vec_dot     *out = sum x*y accumulated in double in an order that depends only on n
+ *                    (InnerProduct, src/Tools/InnerProduct.jl:1-5, whose in-memory form rounds to
+ *                    Float32); waits for the stream. */
+int swem_vec_cgstep(float a, float* m1, float b, const float* m2, size_t n, void* stream);
+int swem_vec_dot(const float* x, const float* y, size_t n, double* out, void* stream);
+
 /* ------------------------------------------- discrete decisions (host only, no GPU) */
 
 /* ntfft, nw, nkx, nky, ifmin, ifmax, lmx, lmy with the reference's casts (wem.c:79-89,

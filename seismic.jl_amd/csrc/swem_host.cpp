@@ -2,6 +2,7 @@
 // (swem_wem, swem_shots, swem_plan_*) and the status helpers.
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -993,6 +994,36 @@ int swem_memcpy_dtoh(void* dst, const void* src, size_t bytes)
 int swem_memcpy_dtod(void* dst, const void* src, size_t bytes)
 {
     SWEM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return SWEM_OK;
+}
+
+int swem_vec_cgstep(float a, float* m1, float b, const float* m2, size_t n, void* stream)
+{
+    if ((!m1 || !m2) && n) return set_error(SWEM_EINVAL, "vec_cgstep: null vector");
+    return launch_vec_cgstep(a, m1, b, m2, n, (hipStream_t)stream);
+}
+
+int swem_vec_dot(const float* x, const float* y, size_t n, double* out, void* stream)
+{
+    if (!out || ((!x || !y) && n)) return set_error(SWEM_EINVAL, "vec_dot: null argument");
+    // scratch and pinned result slot per device, shared by the calls of the process (serialised)
+    static std::mutex mu;
+    static double* scratch[64] = {};
+    static double* host[64] = {};
+    std::lock_guard<std::mutex> lk(mu);
+    int dev = 0;
+    SWEM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_error(SWEM_EDEVICE, "vec_dot: device %d", dev);
+    if (!scratch[dev]) {
+        SWEM_HIP(hipMalloc((void**)&scratch[dev], sizeof(double) * vec_dot_scratch_doubles()));
+        SWEM_HIP(hipHostMalloc((void**)&host[dev], sizeof(double), hipHostMallocDefault));
+    }
+    hipStream_t st = (hipStream_t)stream;
+    SWEM_TRY(launch_vec_dot(x, y, n, scratch[dev], st));
+    SWEM_HIP(hipMemcpyAsync(host[dev], scratch[dev] + vec_dot_scratch_doubles() - 1, sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    SWEM_HIP(hipStreamSynchronize(st));
+    *out = host[dev][0];
     return SWEM_OK;
 }
 

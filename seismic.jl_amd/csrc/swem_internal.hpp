@@ -128,6 +128,11 @@ int launch_sgemm(int M, int Nc, int K, const float* A, int lda, const float* B, 
 // fixed order.  per_shot = 1 writes one image per shot (no stacking).
 int launch_reduce_image(const float* partial, int nshot, int ngroups, int nz, int nmx, float* out, int accumulate,
                         int per_shot, hipStream_t st);
+// Solver vector operations: m1 = a m1 + b m2; <x, y> in double into part[kDotBlocks] (part has
+// vec_dot_scratch_doubles() entries).
+int launch_vec_cgstep(float a, float* m1, float b, const float* m2, size_t n, hipStream_t st);
+int launch_vec_dot(const float* x, const float* y, size_t n, double* part, hipStream_t st);
+int vec_dot_scratch_doubles();
 // One depth of an image: out[x * ldo] (+)= sum_g partial[g][x] (compensated, fixed order).
 int launch_reduce_depth(const float* partial, int ngroups, int n, float* out, int ldo, int accumulate, hipStream_t st);
 // [nm][nz] -> [nz][nm]

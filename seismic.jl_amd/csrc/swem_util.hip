@@ -7,6 +7,8 @@
 // workloads), so the band is computed directly: spec[x][f] = sum_t d[x][t] B[t][f] with
 // B[t][2f] = c_f cos(2 pi iw t / n), B[t][2f+1] = -c_f sin(2 pi iw t / n), the angle reduced
 // exactly in integers and evaluated in FP64 on the host.
+#include <algorithm>
+
 #include "swem_internal.hpp"
 
 namespace swem {
@@ -107,6 +109,70 @@ int launch_reduce_image(const float* partial, int nshot, int ngroups, int nz, in
     SWEM_HIP(hipGetLastError());
     return SWEM_OK;
 }
+
+// ----------------------------------------------------------- solver vector operations
+// m1 = a m1 + b m2 (CGStep.jl:1-4)
+__global__ __launch_bounds__(256) void vec_cgstep_k(float a, float* __restrict__ m1, float b,
+                                                    const float* __restrict__ m2, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m1[i] = a * m1[i] + b * m2[i];
+}
+
+int launch_vec_cgstep(float a, float* m1, float b, const float* m2, size_t n, hipStream_t st)
+{
+    if (!n) return SWEM_OK;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(vec_cgstep_k, dim3(blocks), dim3(256), 0, st, a, m1, b, m2, n);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
+// <x, y> in double: each of kDotBlocks blocks sums a fixed grid-stride subset and reduces it in a
+// fixed tree; one block then sums the block results in order.  The result depends only on n.
+constexpr int kDotBlocks = 512;
+
+__global__ __launch_bounds__(256) void vec_dot_k(const float* __restrict__ x, const float* __restrict__ y, size_t n,
+                                                 double* __restrict__ part)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        acc += (double)x[i] * (double)y[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void vec_dot_final_k(const double* __restrict__ part, int nb, double* __restrict__ out)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// part: kDotBlocks + 1 doubles of device scratch; the result lands in part[kDotBlocks].
+int launch_vec_dot(const float* x, const float* y, size_t n, double* part, hipStream_t st)
+{
+    hipLaunchKernelGGL(vec_dot_k, dim3(kDotBlocks), dim3(256), 0, st, x, y, n, part);
+    SWEM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(vec_dot_final_k, dim3(1), dim3(256), 0, st, part, kDotBlocks, part + kDotBlocks);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
+int vec_dot_scratch_doubles() { return kDotBlocks + 1; }
 
 // One depth of an image: out[x * ldo] (+)= sum_g partial[g][x], compensated, fixed order.
 __global__ __launch_bounds__(256) void reduce_depth(const float* __restrict__ partial, int ngroups, int n,

@@ -28,7 +28,7 @@ EXPORTS = (
     "swem_sampling", "swem_source_index", "swem_pspi_tables", "swem_taper",
     "swem_seis_extent_init", "swem_seis_read_header", "swem_seis_read", "swem_seis_write",
     "swem_server_config_init", "swem_server_create", "swem_server_destroy", "swem_batch_submit",
-    "swem_batch_poll", "swem_batch_wait", "swem_server_get_stats",
+    "swem_batch_poll", "swem_batch_wait", "swem_server_get_stats", "swem_vec_cgstep", "swem_vec_dot",
 )
 REQ_PENDING, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(4)
 
@@ -173,6 +173,8 @@ def load():
         "swem_batch_poll": ([vp, ctypes.c_longlong, P(RequestStatus)], ctypes.c_int),
         "swem_batch_wait": ([vp, ctypes.c_longlong, f32p, P(RequestStatus)], ctypes.c_int),
         "swem_server_get_stats": ([vp, P(ServerStats)], ctypes.c_int),
+        "swem_vec_cgstep": ([ctypes.c_float, vp, ctypes.c_float, vp, ctypes.c_size_t, vp], ctypes.c_int),
+        "swem_vec_dot": ([vp, vp, ctypes.c_size_t, P(ctypes.c_double), vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
