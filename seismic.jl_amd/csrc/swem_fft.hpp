@@ -12,8 +12,8 @@
 // LDS: each subgroup owns two buffers of LdsLen<N> complex and alternates between them
 // (parity `pp`, carried across calls), so one barrier per round trip suffices: a buffer written
 // in round trip k is rewritten in round trip k+2, after every thread has passed barrier k+1 and
-// hence finished reading it.  Indices are padded by one complex every 8 to spread the
-// stride-R stores over the banks.
+// hence finished reading it.  Indices are padded per pass (PassPad) to spread the stores and
+// loads over the banks.
 // Sign convention as FFTW: SIGN = -1 forward exp(-i...), SIGN = +1 backward, unnormalised.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -122,10 +122,31 @@ struct Dft<16, SIGN> {
     }
 };
 
-__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 3); }
+// Bank-conflict padding: one complex inserted every 2^SH.  The LDS has 64 banks of 4 bytes (32
+// complex per row).  A Stockham pass writes at lane stride R (first pass) or in groups of NS
+// lanes, then reads back at unit lane stride; SH is chosen per pass so both the writes and the
+// reads of a 64-lane instruction touch every bank exactly twice (the minimum for 8-byte
+// accesses), except the radix-8 NS = 8 and radix-4 NS > 1 passes, which keep one 3-way access.
+template <int SH>
+__device__ __forceinline__ int padsh(int i)
+{
+    return i + (i >> SH);
+}
+// Padding of the strided-mapping accesses x[t + q T] outside a transform (hand-offs).
+__device__ __forceinline__ int lds_pad(int i) { return padsh<5>(i); }
+template <int NS, int R>
+struct PassPad {
+    static constexpr int value = (R == 8)   ? (NS == 8 ? 3 : 5)
+                                 : (R == 4) ? (NS == 1 ? 5 : 3)
+                                 : (R == 2) ? ((NS == 8 || NS == 16) ? 3 : 5)
+                                            : 5;
+};
+// Complex elements per LDS buffer: the largest padded index (shift 3), rounded to a multiple of
+// 32 plus 4, so consecutive buffers start 8 banks apart (tiles staged across buffers spread
+// over all banks).
 template <int N>
 struct LdsLen {
-    static constexpr int value = N + (N >> 3);
+    static constexpr int value = ((N + N / 8 + 31) & ~31) + 4;
 };
 
 // Number of radix passes of fft<N,P> (for flop accounting and tests).
@@ -153,6 +174,7 @@ __device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* _
     constexpr int B = P / R;
     constexpr bool LAST = (NS * R == N);
     constexpr int L = LdsLen<N>::value;
+    constexpr int SH = PassPad<NS, R>::value;
     float2* buf = lds + pp * L;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
@@ -180,13 +202,13 @@ __device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* _
         } else {
             const int base = (j - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[lds_pad(base + r * NS)] = x[r];
+            for (int r = 0; r < R; ++r) buf[padsh<SH>(base + r * NS)] = x[r];
         }
     }
     if constexpr (!LAST) {
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < P; ++q) v[q] = buf[lds_pad(t + q * T)];
+        for (int q = 0; q < P; ++q) v[q] = buf[padsh<SH>(t + q * T)];
         pp ^= 1;
         fft<N, P, SIGN, NS * R>(v, lds, tw, t, pp);
     }
@@ -208,6 +230,7 @@ __device__ __forceinline__ void wfft(float2 (&v)[N / 64], float2* buf, const flo
     constexpr int R = REM >= RMAX ? RMAX : REM;
     constexpr int B = P / R;
     constexpr bool LAST = (NS * R == N);
+    constexpr int SH = PassPad<NS, R>::value;
     static_assert(P % R == 0, "radix must divide the points per lane");
 #pragma unroll
     for (int b = 0; b < B; ++b) {
@@ -232,7 +255,7 @@ __device__ __forceinline__ void wfft(float2 (&v)[N / 64], float2* buf, const flo
         } else {
             const int base = (j - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[lds_pad(base + r * NS)] = x[r];
+            for (int r = 0; r < R; ++r) buf[padsh<SH>(base + r * NS)] = x[r];
         }
     }
     if constexpr (!LAST) {
@@ -240,7 +263,7 @@ __device__ __forceinline__ void wfft(float2 (&v)[N / 64], float2* buf, const flo
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-        for (int q = 0; q < P; ++q) v[q] = buf[lds_pad(t + q * T)];
+        for (int q = 0; q < P; ++q) v[q] = buf[padsh<SH>(t + q * T)];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -486,7 +486,8 @@ int march2d_max_block(int nkx, int variant)
 size_t march2d_lds_bytes(int nkx, int nsub, int variant)
 {
     const int bufs = march2d_resolve_variant(nkx, variant) == SWEM_VARIANT_WAVE ? 1 : 2;
-    return (size_t)nsub * bufs * (size_t)(nkx + (nkx >> 3)) * sizeof(float2);
+    const size_t len = (size_t)(((nkx + nkx / 8 + 31) & ~31) + 4);  // LdsLen<nkx>
+    return (size_t)nsub * bufs * len * sizeof(float2);
 }
 
 int march2d_grid_blocks(int ngroups, int nshot) { return ((ngroups + 7) / 8) * 8 * nshot; }
