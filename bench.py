@@ -278,6 +278,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--freqs-per-block", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0, help="2-D kernel variant (SWEM_VARIANT_*, 0 = automatic)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm"), default="pspi2d")
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
@@ -304,7 +305,7 @@ def main():
 
     g, vel, refl, wav, sx = synth.config2(nshot=args.shots)
     nshot = int(sx.size)
-    plan = seiswem.Plan(g, vel, wav, freqs_per_block=args.freqs_per_block)
+    plan = seiswem.Plan(g, vel, wav, freqs_per_block=args.freqs_per_block, variant=args.variant)
     m_true = seiswem.DeviceArray(refl.shape, refl)
     t0 = time.perf_counter()
     d = plan.model(m_true, sx)  # records by demigration (setup, untimed)

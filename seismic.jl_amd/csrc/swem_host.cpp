@@ -96,11 +96,12 @@ static double step_flops(const swem_plan* p)
 // Frequencies per migration workgroup (2F subgroups of T threads) or subgroups per modelling
 // workgroup (one frequency each).  Preferred: at most 512 threads and 80 KiB of LDS so two
 // workgroups share a CU; fewer when a single shot would leave CUs idle.
-static int choose_group(const swem_plan* p, int per_freq, int forced)
+static int choose_group(const swem_plan* p, int adj, int forced)
 {
     if (forced > 0) return forced;
-    int F = std::max(1, std::min(512, march2d_max_block(p->nkx, p->opt.variant)) / (p->T * per_freq));
-    while (F > 1 && march2d_lds_bytes(p->nkx, F * per_freq, p->opt.variant) > 80 * 1024) F--;
+    const March2DShape sh = march2d_shape(p->nkx, p->opt.variant, adj);
+    int F = std::max(1, std::min(512, sh.max_block) / (sh.T * sh.per_freq));
+    while (F > 1 && march2d_lds_bytes(p->nkx, sh, F) > 80 * 1024) F--;
     while (F > 1 && (p->nf + F - 1) / F < 512) F--;
     return F;
 }
@@ -212,12 +213,13 @@ static int build_tables(swem_plan* p, const float* wav)
                 p2f[(size_t)iz * nref + ir] = 1 / (vf * vf);
             }
         }
-        std::vector<uint8_t> sel((size_t)nz * nmx), sel2((size_t)nz * nmx);
-        std::vector<float> w2((size_t)nz * nmx, 0.f);
+        // [nz][nkx]: rows padded to the transform length (0x7f: no vref at x >= nmx)
+        std::vector<uint8_t> sel((size_t)nz * nkx, 0x7f), sel2((size_t)nz * nkx, 0x7f);
+        std::vector<float> w2((size_t)nz * nkx, 0.f);
         std::vector<uint32_t> used(nz, 0u);
         for (int ix = 0; ix < nm; ix++)
             for (int iz = 0; iz < nz; iz++) {
-                const size_t i = (size_t)ix * nz + iz, o = (size_t)iz * nmx + ix;
+                const size_t i = (size_t)ix * nz + iz, o = (size_t)iz * nkx + ix;
                 sel[o] = (uint8_t)(i1[i] | (cut[i] ? 0x80 : 0));
                 sel2[o] = (uint8_t)i2[i];
                 used[iz] |= 1u << i1[i];
@@ -708,7 +710,7 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
         return set_error(SWEM_EINVAL, "pspi_mode %d", o.pspi_mode);
     if (o.precision != SWEM_PREC_KZ64 && o.precision != SWEM_PREC_FP32)
         return set_error(SWEM_EINVAL, "precision %d", o.precision);
-    if (o.variant < SWEM_VARIANT_AUTO || o.variant > SWEM_VARIANT_SUBGROUP16) return set_error(SWEM_EINVAL, "variant %d", o.variant);
+    if (o.variant < SWEM_VARIANT_AUTO || o.variant > SWEM_VARIANT_PAIR) return set_error(SWEM_EINVAL, "variant %d", o.variant);
     p->g = *g;
     p->opt = o;
     SWEM_TRY(compute_sampling(p->g, p->s));
@@ -734,13 +736,12 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
         if (p->F3 < 1 || 128 * p->F3 > 1024 || 64 * p->F3f > 1024)
             return set_error(SWEM_EINVAL, "3-D: freqs_per_block=%d (1..8)", o.freqs_per_block);
     } else {
-        p->T = march2d_threads(p->nkx, o.variant);
-        p->F = choose_group(p, 2, o.freqs_per_block);
-        p->Ffwd = choose_group(p, 1, o.freqs_per_block);
-        if (2 * p->F * p->T > march2d_max_block(p->nkx, o.variant) ||
-            march2d_lds_bytes(p->nkx, 2 * p->F, o.variant) > 160 * 1024 ||
-            p->Ffwd * p->T > march2d_max_block(p->nkx, o.variant) ||
-            march2d_lds_bytes(p->nkx, p->Ffwd, o.variant) > 160 * 1024)
+        const March2DShape sa = march2d_shape(p->nkx, o.variant, 1), sf = march2d_shape(p->nkx, o.variant, 0);
+        p->T = sa.T;
+        p->F = choose_group(p, 1, o.freqs_per_block);
+        p->Ffwd = choose_group(p, 0, o.freqs_per_block);
+        if (sa.per_freq * p->F * sa.T > sa.max_block || march2d_lds_bytes(p->nkx, sa, p->F) > 160 * 1024 ||
+            p->Ffwd * sf.T > sf.max_block || march2d_lds_bytes(p->nkx, sf, p->Ffwd) > 160 * 1024)
             return set_error(SWEM_EINVAL, "freqs_per_block=%d does not fit a workgroup at nkx=%d", o.freqs_per_block,
                              p->s.nkx);
         choose_chunks(p);
@@ -790,7 +791,7 @@ size_t plan_upload_bytes(const swem_plan* p, int nshot)
     size_t b = pad256(p->nkx * 8) + pad256(p->s.nky * 8) + pad256(nmx * 4) + pad256(p->g.nmy * 4) + pad256(p->g.nt * 4);
     b += pad256(nz * nref * 8) + pad256(nz * nref * 4) + pad256(nz * 4);
     if (p->g.pspi)
-        b += 2 * pad256(nz * nmx) + pad256(nz * nmx * 4);
+        b += 2 * pad256(nz * p->nkx) + pad256(nz * p->nkx * 4);
     else
         b += pad256(nz * nm * 8) + pad256(nz * nm * 4);
     return b + pad256((size_t)nshot * 4);

@@ -66,9 +66,9 @@ struct March2DArgs {
     const int* igx;         // [nshot]
     const float2* spec_r;   // [nshot][nmx][nf]  receiver spectra w^2 D / sqrt(ntfft)
     const float* taper;     // [nmx]
-    const uint8_t* sel;     // [nz][nmx] iref1 | cut << 7 (PSPI)
-    const uint8_t* sel2;    // [nz][nmx] iref2 (PSPI linear)
-    const float* w2;        // [nz][nmx] weight of iref2 (PSPI linear)
+    const uint8_t* sel;     // [nz][nkx] iref1 | cut << 7 (PSPI; 0x7f beyond nmx)
+    const uint8_t* sel2;    // [nz][nkx] iref2 (PSPI linear)
+    const float* w2;        // [nz][nkx] weight of iref2 (PSPI linear)
     const uint32_t* used;   // [nz] bit ir set when some x selects vref ir at iz
     float* partial;         // [nshot][ngroups][nz][nmx] migration partial images
     const float* mT;        // [nz][nmx] image (modelling input, depth-major)
@@ -76,16 +76,22 @@ struct March2DArgs {
     float2* dgw;            // [nshot][nmx][nf] modelled spectra (modelling output)
 };
 
-// mode: 0 = SSF, 1 = PSPI compat, 2 = PSPI linear.  variant: SWEM_VARIANT_*.
+// mode: 0 = SSF, 1 = PSPI compat, 2 = PSPI linear.  variant: SWEM_VARIANT_*.  F: frequencies
+// per workgroup.
 int launch_march2d(const March2DArgs& a, int nkx, int F, int mode, int adj, int variant, hipStream_t st);
 // Number of blocks of the XCD-aware 1-D grid of a launch.
 int march2d_grid_blocks(int ngroups, int nshot);
-int march2d_resolve_variant(int nkx, int variant);
-// Threads per wavefield subgroup for a given nkx (0 if unsupported).
-int march2d_threads(int nkx, int variant);
+// Shape of the 2-D kernel a variant selects (T = 0: nkx unsupported).
+struct March2DShape {
+    int variant;    // resolved SWEM_VARIANT_*
+    int T;          // threads per wavefield subgroup
+    int per_freq;   // subgroups per frequency
+    int max_block;  // largest workgroup
+    int bufs;       // LDS buffers per subgroup
+};
+March2DShape march2d_shape(int nkx, int variant, int adj);
 bool march2d_supported(int nkx);
-int march2d_max_block(int nkx, int variant);
-size_t march2d_lds_bytes(int nkx, int nsub, int variant);
+size_t march2d_lds_bytes(int nkx, const March2DShape& sh, int F);
 
 // Arguments of the 3-D split-step kernels (swem_march3d.hip).  A launch covers the items
 // (frequencies f0 .. f0 + nitem of one shot) of a frequency chunk at depth iz.

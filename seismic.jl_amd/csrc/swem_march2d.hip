@@ -52,11 +52,37 @@ __device__ __forceinline__ void decode_block(int ngroups, int nshot, int& g, int
     if (g >= ngroups) g = -1;
 }
 
-// kz phase of wavenumber index k from a table row (L is even in kx: row holds kk = 0..N/2).
-template <int N>
-__device__ __forceinline__ float2 kload(const float2* __restrict__ row, int k)
+// Raw buffer access: a resource of `bytes` bytes at `base` (wave-uniform); loads beyond it
+// return zero and stores beyond it are dropped by the hardware, so edge lanes need no branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes)
 {
-    return row[k <= N / 2 ? k : N - k];
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 buf_load2(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ float buf_load1(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void buf_store1(__amdgpu_buffer_rsrc_t r, uint32_t off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void buf_store2(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 v)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
+                                          (int)off, 0, 0);
+}
+
+// kz phase of the point k = t + q T of the strided mapping from a table row (L is even in kx,
+// the row holds kk = 0..N/2): kk = k for q < P/2, N - k = (P - q) T - t above.
+template <int N, int P>
+__device__ __forceinline__ float2 kload(const float2* __restrict__ row, int t, int q)
+{
+    constexpr int T = N / P;
+    return row[q < P / 2 ? t + q * T : (P - q) * T - t];
 }
 
 __device__ __forceinline__ float2 cmul_c(float2 x, float2 L, bool conj) { return conj ? cmulc(x, L) : cmul(x, L); }
@@ -75,6 +101,7 @@ __device__ __forceinline__ void xfft(float2 (&v)[P], float2* lds, const float2* 
 
 struct StepCtx {
     int t, iz, fc;  // thread within the subgroup, depth index, frequency index within the chunk
+                    // (fc and iz are wave-uniform: table rows are addressed from scalar registers)
 };
 
 __device__ __forceinline__ const float2* krow(const March2DArgs& a, const StepCtx& c, int ir)
@@ -82,30 +109,37 @@ __device__ __forceinline__ const float2* krow(const March2DArgs& a, const StepCt
     return a.ktab + (((size_t)c.fc * a.nz + c.iz) * a.nref + ir) * a.nkh;
 }
 
-// ssop adjoint branch (wem.c:313-320, 334-359) on one wavefield.  act = false turns the step
+// Split-step phase row of (fc, iz) as a buffer of nmx entries: lanes x >= nmx read zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ss_row(const March2DArgs& a, const StepCtx& c)
+{
+    return buf_rsrc(a.stab + ((size_t)c.fc * a.nz + c.iz) * a.nmx, (uint32_t)a.nmx * 8u);
+}
+
+// ssop adjoint branch (wem.c:313-320, 334-359) on one wavefield.  ACT = false turns the step
 // into FFT / IFFT with unit phases (a numerical identity) so every subgroup of a workgroup runs
 // the same barriers when only one of the two wavefields is stepped at a depth.
-template <int N, int P, bool WAVE>
+template <int N, int P, bool WAVE, bool ACT>
 __device__ __forceinline__ void ssf_adj_step(float2 (&u)[P], float2* lds, int& pp, const March2DArgs& a,
-                                             const StepCtx& c, bool act, bool conj, const float (&tap)[P])
+                                             const StepCtx& c, bool conj, const float (&tap)[P])
 {
     constexpr int T = N / P;
     xfft<N, P, -1, WAVE>(u, lds, a.tw, c.t, pp);
     const float2* row = krow(a, c, 0);
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-        const float2 L = act ? kload<N>(row, c.t + q * T) : make_float2(a.inv_nk, 0.f);
+        const float2 L = ACT ? kload<N, P>(row, c.t, q) : make_float2(a.inv_nk, 0.f);
         u[q] = cmul_c(u[q], L, conj);
     }
     xfft<N, P, +1, WAVE>(u, lds, a.tw, c.t, pp);
-    const float2* srow = a.stab + ((size_t)c.fc * a.nz + c.iz) * a.nmx;
+    if constexpr (ACT) {
+        const auto sr = ss_row(a, c);
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-        const int x = c.t + q * T;
-        if (x < a.nmx) {
-            if (act) u[q] = cscale(cmul_c(u[q], srow[x], conj), tap[q]);
-        } else
-            u[q] = make_float2(0.f, 0.f);
+        for (int q = 0; q < P; ++q)  // zero beyond nmx: the split-step row reads zero there
+            u[q] = cscale(cmul_c(u[q], buf_load2(sr, (uint32_t)(c.t + q * T) * 8u), conj), tap[q]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+            if (c.t + q * T >= a.nmx) u[q] = make_float2(0.f, 0.f);
     }
 }
 
@@ -116,16 +150,13 @@ __device__ __forceinline__ void ssf_fwd_step(float2 (&u)[P], float2* lds, int& p
                                              const StepCtx& c, const float (&tap)[P])
 {
     constexpr int T = N / P;
-    const float2* srow = a.stab + ((size_t)c.fc * a.nz + c.iz) * a.nmx;
+    const auto sr = ss_row(a, c);
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-        const int x = c.t + q * T;
-        u[q] = x < a.nmx ? cmulc(cscale(u[q], tap[q]), srow[x]) : make_float2(0.f, 0.f);
-    }
+    for (int q = 0; q < P; ++q) u[q] = cmulc(cscale(u[q], tap[q]), buf_load2(sr, (uint32_t)(c.t + q * T) * 8u));
     xfft<N, P, -1, WAVE>(u, lds, a.tw, c.t, pp);
     const float2* row = krow(a, c, 0);
 #pragma unroll
-    for (int q = 0; q < P; ++q) u[q] = cmulc(u[q], kload<N>(row, c.t + q * T));
+    for (int q = 0; q < P; ++q) u[q] = cmulc(u[q], kload<N, P>(row, c.t, q));
     xfft<N, P, +1, WAVE>(u, lds, a.tw, c.t, pp);
 #pragma unroll
     for (int q = 0; q < P; ++q)
@@ -146,24 +177,24 @@ __device__ __forceinline__ float pspi_linear_weight(int ir, uint8_t s1, uint8_t 
 
 // pspiop with the reference structure (wem.c:409-460): FFT, for each vref phase + IFFT, then
 // per-x selection.  Compat: u = Re(dref[iref1]) when cut else dref[iref1]; linear: weights.
-template <int N, int P, int MODE, bool WAVE>
+// The selection rows are padded to N columns (0x7f beyond nmx: no vref selected).
+template <int N, int P, int MODE, bool WAVE, bool ACT>
 __device__ __forceinline__ void pspi_select_step(float2 (&u)[P], float2* lds, int& pp, const March2DArgs& a,
-                                                 const StepCtx& c, bool act, bool conj)
+                                                 const StepCtx& c, bool conj)
 {
     constexpr int T = N / P;
     xfft<N, P, -1, WAVE>(u, lds, a.tw, c.t, pp);
     float2 U[P];
     uint8_t s1[P], s2[P];
     float w2[P];
-    const size_t rowx = (size_t)c.iz * a.nmx;
+    const size_t rowx = (size_t)c.iz * N;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const int x = c.t + q * T;
-        const bool in = x < a.nmx;
-        s1[q] = in ? a.sel[rowx + x] : (uint8_t)0x7f;
+        s1[q] = a.sel[rowx + x];
         if constexpr (MODE == kModePspiLinear) {
-            s2[q] = in ? a.sel2[rowx + x] : (uint8_t)0x7f;
-            w2[q] = in ? a.w2[rowx + x] : 0.f;
+            s2[q] = a.sel2[rowx + x];
+            w2[q] = a.w2[rowx + x];
         }
         U[q] = make_float2(0.f, 0.f);
     }
@@ -175,13 +206,13 @@ __device__ __forceinline__ void pspi_select_step(float2 (&u)[P], float2* lds, in
         float2 B[P];
 #pragma unroll
         for (int q = 0; q < P; ++q) {
-            const float2 L = act ? kload<N>(row, c.t + q * T) : make_float2(a.inv_nk, 0.f);
+            const float2 L = ACT ? kload<N, P>(row, c.t, q) : make_float2(a.inv_nk, 0.f);
             B[q] = cmul_c(u[q], L, conj);
         }
         xfft<N, P, +1, WAVE>(B, lds, a.tw, c.t, pp);
 #pragma unroll
         for (int q = 0; q < P; ++q) {
-            if (!act) {
+            if constexpr (!ACT) {
                 if (ir == first) U[q] = B[q];
             } else if constexpr (MODE == kModePspiCompat) {
                 if ((s1[q] & 0x7f) == ir) U[q] = (s1[q] >> 7) ? make_float2(B[q].x, 0.f) : B[q];
@@ -204,14 +235,13 @@ __device__ __forceinline__ void pspi_weight_step(float2 (&u)[P], float2* lds, in
     float2 acc[P];
     uint8_t s1[P], s2[P];
     float w2[P];
-    const size_t rowx = (size_t)c.iz * a.nmx;
+    const size_t rowx = (size_t)c.iz * N;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const int x = c.t + q * T;
-        const bool in = x < a.nmx;
-        s1[q] = in ? a.sel[rowx + x] : (uint8_t)0x7f;
-        s2[q] = in ? a.sel2[rowx + x] : (uint8_t)0x7f;
-        w2[q] = in ? a.w2[rowx + x] : 0.f;
+        s1[q] = a.sel[rowx + x];
+        s2[q] = a.sel2[rowx + x];
+        w2[q] = a.w2[rowx + x];
         acc[q] = make_float2(0.f, 0.f);
     }
     const uint32_t used = a.used[c.iz];
@@ -223,22 +253,30 @@ __device__ __forceinline__ void pspi_weight_step(float2 (&u)[P], float2* lds, in
         xfft<N, P, -1, WAVE>(B, lds, a.tw, c.t, pp);
         const float2* row = krow(a, c, ir);
 #pragma unroll
-        for (int q = 0; q < P; ++q) acc[q] = cadd(acc[q], cmulc(B[q], kload<N>(row, c.t + q * T)));
+        for (int q = 0; q < P; ++q) acc[q] = cadd(acc[q], cmulc(B[q], kload<N, P>(row, c.t, q)));
     }
     xfft<N, P, +1, WAVE>(acc, lds, a.tw, c.t, pp);
 #pragma unroll
     for (int q = 0; q < P; ++q) u[q] = (c.t + q * T < a.nmx) ? acc[q] : make_float2(0.f, 0.f);
 }
 
-// Source / adjoint-receiver step of either method (select-after for PSPI).
+// Source / adjoint-receiver step of either method (select-after for PSPI).  act is uniform
+// over the subgroup: the branch is taken once per step, not per element.
 template <int N, int P, int MODE, bool WAVE>
 __device__ __forceinline__ void adj_step(float2 (&u)[P], float2* lds, int& pp, const March2DArgs& a, const StepCtx& c,
                                          bool act, bool conj, const float (&tap)[P])
 {
-    if constexpr (MODE == kModeSSF)
-        ssf_adj_step<N, P, WAVE>(u, lds, pp, a, c, act, conj, tap);
-    else
-        pspi_select_step<N, P, MODE, WAVE>(u, lds, pp, a, c, act, conj);
+    if constexpr (MODE == kModeSSF) {
+        if (act)
+            ssf_adj_step<N, P, WAVE, true>(u, lds, pp, a, c, conj, tap);
+        else
+            ssf_adj_step<N, P, WAVE, false>(u, lds, pp, a, c, conj, tap);
+    } else {
+        if (act)
+            pspi_select_step<N, P, MODE, WAVE, true>(u, lds, pp, a, c, conj);
+        else
+            pspi_select_step<N, P, MODE, WAVE, false>(u, lds, pp, a, c, conj);
+    }
 }
 
 // Forward-receiver step (dz < 0).
@@ -249,7 +287,7 @@ __device__ __forceinline__ void fwd_step(float2 (&u)[P], float2* lds, int& pp, c
     if constexpr (MODE == kModeSSF)
         ssf_fwd_step<N, P, WAVE>(u, lds, pp, a, c, tap);
     else if constexpr (MODE == kModePspiCompat)
-        pspi_select_step<N, P, MODE, WAVE>(u, lds, pp, a, c, true, true);
+        pspi_select_step<N, P, MODE, WAVE, true>(u, lds, pp, a, c, true);
     else
         pspi_weight_step<N, P, WAVE>(u, lds, pp, a, c);
 }
@@ -288,7 +326,7 @@ __device__ __forceinline__ void adj_body(const March2DArgs& a)
     const int iw = a.ifmin + fl;
     StepCtx c;
     c.t = t;
-    c.fc = active ? fc : 0;
+    c.fc = __builtin_amdgcn_readfirstlane(active ? fc : 0);
     const float factor = iw == 0 ? 1.f : 2.f;
     float2* lds = smem + (size_t)s * BUFS * L;
     const float2* xsrc = smem + (size_t)(2 * f) * BUFS * L;
@@ -329,11 +367,9 @@ __device__ __forceinline__ void adj_body(const March2DArgs& a)
         if constexpr (!WAVE) pp ^= 1;
         if (F == 1) {
             if (!src) {
+                const auto pr = buf_rsrc(part + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
 #pragma unroll
-                for (int q = 0; q < P; ++q) {
-                    const int x = t + q * T;
-                    if (x < a.nmx) part[(size_t)iz * a.nmx + x] = cc[q];
-                }
+                for (int q = 0; q < P; ++q) buf_store1(pr, (uint32_t)(t + q * T) * 4u, cc[q]);
             }
         } else {
             if constexpr (WAVE) __syncthreads();  // single buffer: the source copy is read
@@ -376,7 +412,7 @@ __device__ __forceinline__ void fwd_body(const March2DArgs& a)
     const int fl = a.f0 + fc;
     StepCtx c;
     c.t = t;
-    c.fc = active ? fc : 0;
+    c.fc = __builtin_amdgcn_readfirstlane(active ? fc : 0);
     float2* lds = smem + (size_t)s * BUFS * L;
     int pp = 0;
     float tap[P];
@@ -397,11 +433,10 @@ __device__ __forceinline__ void fwd_body(const March2DArgs& a)
         const bool ds = z >= a.sz;
         if (ds) adj_step<N, P, MODE, WAVE>(u, lds, pp, a, c, true, true, tap);
         if (active) {
+            const auto sr = buf_rsrc(sm + (size_t)iz * a.nmx, (uint32_t)a.nmx * 8u);
 #pragma unroll
-            for (int q = 0; q < P; ++q) {
-                const int x = t + q * T;
-                if (x < a.nmx) sm[(size_t)iz * a.nmx + x] = ds ? u[q] : make_float2(0.f, 0.f);
-            }
+            for (int q = 0; q < P; ++q)
+                buf_store2(sr, (uint32_t)(t + q * T) * 8u, ds ? u[q] : make_float2(0.f, 0.f));
         }
     }
     // receiver march upward with injection (wem.c:266-276)
@@ -412,20 +447,66 @@ __device__ __forceinline__ void fwd_body(const March2DArgs& a)
         const float z = a.oz + a.dz * (float)iz;
         if (!(z >= a.gz)) continue;
         if (active) {
+            const auto sr = buf_rsrc(sm + (size_t)iz * a.nmx, (uint32_t)a.nmx * 8u);
+            const auto mr = buf_rsrc(a.mT + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
 #pragma unroll
             for (int q = 0; q < P; ++q) {
-                const int x = t + q * T;
-                if (x < a.nmx) u[q] = cadd(u[q], cscale(sm[(size_t)iz * a.nmx + x], a.mT[(size_t)iz * a.nmx + x]));
+                const uint32_t x = (uint32_t)(t + q * T);
+                u[q] = cadd(u[q], cscale(buf_load2(sr, x * 8u), buf_load1(mr, x * 4u)));
             }
         }
         fwd_step<N, P, MODE, WAVE>(u, lds, pp, a, c, tap);
     }
     if (active) {
+        const auto dr = buf_rsrc(a.dgw + (size_t)shot * a.nmx * a.nf, (uint32_t)a.nmx * a.nf * 8u);
 #pragma unroll
-        for (int q = 0; q < P; ++q) {
-            const int x = t + q * T;
-            if (x < a.nmx) a.dgw[((size_t)shot * a.nmx + x) * a.nf + fl] = u[q];
-        }
+        for (int q = 0; q < P; ++q) buf_store2(dr, ((uint32_t)(t + q * T) * (uint32_t)a.nf + (uint32_t)fl) * 8u, u[q]);
+    }
+}
+
+// Migration march with one subgroup per frequency that steps the source, then the receiver,
+// through the same LDS buffers (variant PAIR).  One frequency per workgroup; the imaging
+// condition is register-local.
+template <int N, int P, int MODE, bool WAVE>
+__device__ __forceinline__ void adj_body_pair(const March2DArgs& a)
+{
+    constexpr int T = N / P;
+    extern __shared__ float2 smem[];
+    int g, shot;
+    decode_block(a.ngroups, a.nshot, g, shot);
+    if (g < 0) return;  // padding block of the XCD-aware grid
+    const int t = threadIdx.x;
+    const int fc = g;  // one frequency per workgroup
+    const bool active = fc < a.nfc;
+    const int fl = a.f0 + fc;
+    const int iw = a.ifmin + fl;
+    StepCtx c;
+    c.t = t;
+    c.fc = __builtin_amdgcn_readfirstlane(active ? fc : 0);
+    const float factor = iw == 0 ? 1.f : 2.f;
+    int pp = 0;
+    float tap[P];
+    load_taper<N, P, MODE>(tap, a, t);
+    const int igx = a.igx[shot];
+    float2 us[P], ur[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int x = t + q * T;
+        us[q] = (active && x == igx) ? a.spec_s[fl] : make_float2(0.f, 0.f);
+        ur[q] = (active && x < a.nmx) ? a.spec_r[((size_t)shot * a.nmx + x) * a.nf + fl] : make_float2(0.f, 0.f);
+    }
+    float* part = a.partial + ((size_t)shot * a.ngroups + g) * (size_t)a.nz * a.nmx;
+    for (int iz = 0; iz < a.nz; ++iz) {
+        c.iz = iz;
+        const float z = a.oz + a.dz * (float)iz;
+        const bool ds = z >= a.sz, dr = z >= a.gz;
+        if (ds) adj_step<N, P, MODE, WAVE>(us, smem, pp, a, c, true, true, tap);
+        if (dr) adj_step<N, P, MODE, WAVE>(ur, smem, pp, a, c, true, false, tap);
+        // imaging condition (wem.c:243-247)
+        const auto pr = buf_rsrc(part + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+            buf_store1(pr, (uint32_t)(t + q * T) * 4u, dr ? factor * (ur[q].x * us[q].x + ur[q].y * us[q].y) : 0.f);
     }
 }
 
@@ -449,45 +530,64 @@ __global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(2)))
     fwd_body<N, N / 64, MODE, true>(a);
 }
 
+// PAIR: 16 points per thread from nkx = 1024 (nkx/16 threads, radix-16 passes), one wave with
+// nkx/64 points per lane below it.
+template <int N, int MODE>
+__global__ __launch_bounds__(N / 16) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_pair(March2DArgs a)
+{
+    adj_body_pair<N, 16, MODE, false>(a);
+}
+
+template <int N, int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_pair_w(March2DArgs a)
+{
+    adj_body_pair<N, N / 64, MODE, true>(a);
+}
+
 // ------------------------------------------------------------------------ dispatch
-// variant: SWEM_VARIANT_WAVE (one wave per wavefield, nkx <= 2048), SWEM_VARIANT_SUBGROUP
-// (nkx/8 threads per wavefield), SWEM_VARIANT_SUBGROUP16 (nkx/16 threads, nkx >= 1024).
-int march2d_resolve_variant(int nkx, int variant)
+// Kernel shape of a variant: threads per wavefield subgroup T, subgroups per frequency
+// (2 = source and receiver in separate subgroups, 1 = one subgroup marches both), the largest
+// workgroup, LDS buffers per subgroup.
+//   WAVE         one 64-lane wave per wavefield, no barrier inside a transform (nkx <= 2048)
+//   SUBGROUP     nkx/8 threads per wavefield (nkx/4, nkx/2 below 512), ping-pong LDS
+//   SUBGROUP16   nkx/16 threads per wavefield, radix-16 passes (nkx >= 1024)
+//   PAIR         (migration) one subgroup per frequency marches source and receiver in turn:
+//                the imaging needs no hand-off and a barrier holds only that subgroup's waves;
+//                a wave (nkx <= 1024) or nkx/16 threads.  Modelling uses the AUTO shape.
+March2DShape march2d_shape(int nkx, int variant, int adj)
 {
-    if (variant == SWEM_VARIANT_AUTO) return nkx <= 1024 ? SWEM_VARIANT_WAVE : SWEM_VARIANT_SUBGROUP16;
-    if (variant == SWEM_VARIANT_WAVE && nkx > 2048) return SWEM_VARIANT_SUBGROUP16;
-    if (variant == SWEM_VARIANT_SUBGROUP16 && nkx < 1024) return SWEM_VARIANT_SUBGROUP;
-    return variant;
+    March2DShape sh{};
+    if (nkx < 128 || nkx > 4096 || (nkx & (nkx - 1))) return sh;
+    int v = variant;
+    if (v == SWEM_VARIANT_PAIR && !adj) v = SWEM_VARIANT_AUTO;
+    if (v == SWEM_VARIANT_AUTO) v = adj ? SWEM_VARIANT_PAIR : (nkx <= 512 ? SWEM_VARIANT_WAVE : SWEM_VARIANT_SUBGROUP16);
+    if (v == SWEM_VARIANT_WAVE && nkx > 2048) v = SWEM_VARIANT_SUBGROUP16;
+    if (v == SWEM_VARIANT_SUBGROUP16 && nkx < 1024) v = SWEM_VARIANT_SUBGROUP;
+    sh.variant = v;
+    sh.per_freq = adj ? 2 : 1;
+    switch (v) {
+    case SWEM_VARIANT_WAVE:
+        sh.T = 64, sh.max_block = kWaveBlock, sh.bufs = 1;
+        break;
+    case SWEM_VARIANT_SUBGROUP16:
+        sh.T = nkx / 16, sh.max_block = 512, sh.bufs = 2;
+        break;
+    case SWEM_VARIANT_PAIR:
+        sh.T = nkx >= 1024 ? nkx / 16 : 64, sh.per_freq = 1, sh.max_block = sh.T, sh.bufs = nkx >= 1024 ? 2 : 1;
+        break;
+    default:
+        sh.variant = SWEM_VARIANT_SUBGROUP;
+        sh.T = nkx / (nkx >= 512 ? 8 : (nkx >= 256 ? 4 : 2)), sh.max_block = kMaxBlock, sh.bufs = 2;
+    }
+    return sh;
 }
 
-static int subgroup_points(int nkx, int v)
-{
-    if (v == SWEM_VARIANT_SUBGROUP16) return 16;
-    return nkx >= 512 ? 8 : (nkx >= 256 ? 4 : 2);
-}
+bool march2d_supported(int nkx) { return march2d_shape(nkx, SWEM_VARIANT_AUTO, 1).T > 0; }
 
-int march2d_threads(int nkx, int variant)
+size_t march2d_lds_bytes(int nkx, const March2DShape& sh, int F)
 {
-    if (nkx < 128 || nkx > 4096 || (nkx & (nkx - 1))) return 0;
-    const int v = march2d_resolve_variant(nkx, variant);
-    if (v == SWEM_VARIANT_WAVE) return 64;
-    return nkx / subgroup_points(nkx, v);
-}
-
-bool march2d_supported(int nkx) { return march2d_threads(nkx, SWEM_VARIANT_AUTO) > 0; }
-
-int march2d_max_block(int nkx, int variant)
-{
-    const int v = march2d_resolve_variant(nkx, variant);
-    return v == SWEM_VARIANT_WAVE ? kWaveBlock : (v == SWEM_VARIANT_SUBGROUP16 ? 512 : kMaxBlock);
-}
-
-// LDS of one workgroup of nsub subgroups (one buffer per wave, two per ping-pong subgroup).
-size_t march2d_lds_bytes(int nkx, int nsub, int variant)
-{
-    const int bufs = march2d_resolve_variant(nkx, variant) == SWEM_VARIANT_WAVE ? 1 : 2;
     const size_t len = (size_t)(((nkx + nkx / 8 + 31) & ~31) + 4);  // LdsLen<nkx>
-    return (size_t)nsub * bufs * len * sizeof(float2);
+    return (size_t)F * sh.per_freq * sh.bufs * len * sizeof(float2);
 }
 
 int march2d_grid_blocks(int ngroups, int nshot) { return ((ngroups + 7) / 8) * 8 * nshot; }
@@ -503,18 +603,23 @@ static int launch_kernel(K kernel, const March2DArgs& a, dim3 block, size_t lds,
 }
 
 template <int N, int MODE>
-static int launch_one(const March2DArgs& a, int nsub, int adj, int variant, hipStream_t st)
+static int launch_one(const March2DArgs& a, const March2DShape& sh, int F, int adj, hipStream_t st)
 {
-    const size_t lds = march2d_lds_bytes(N, nsub, variant);
-    dim3 block(march2d_threads(N, variant) * nsub);
-    const int v = march2d_resolve_variant(N, variant);
+    const size_t lds = march2d_lds_bytes(N, sh, F);
+    dim3 block(sh.T * sh.per_freq * F);
+    if (sh.variant == SWEM_VARIANT_PAIR) {
+        if constexpr (N < 1024)
+            return launch_kernel(march2d_adj_pair_w<N, MODE>, a, block, lds, st);
+        else
+            return launch_kernel(march2d_adj_pair<N, MODE>, a, block, lds, st);
+    }
     if constexpr (N <= 2048) {
-        if (v == SWEM_VARIANT_WAVE)
+        if (sh.variant == SWEM_VARIANT_WAVE)
             return adj ? launch_kernel(march2d_adj_w<N, MODE>, a, block, lds, st)
                        : launch_kernel(march2d_fwd_w<N, MODE>, a, block, lds, st);
     }
     if constexpr (N >= 1024) {
-        if (v == SWEM_VARIANT_SUBGROUP16)
+        if (sh.variant == SWEM_VARIANT_SUBGROUP16)
             return adj ? launch_kernel(march2d_adj<N, MODE, 16>, a, block, lds, st)
                        : launch_kernel(march2d_fwd<N, MODE, 16>, a, block, lds, st);
     }
@@ -524,35 +629,33 @@ static int launch_one(const March2DArgs& a, int nsub, int adj, int variant, hipS
 }
 
 template <int N>
-static int launch_n(const March2DArgs& a, int nsub, int mode, int adj, int variant, hipStream_t st)
+static int launch_n(const March2DArgs& a, const March2DShape& sh, int F, int mode, int adj, hipStream_t st)
 {
-    if (mode == kModeSSF) return launch_one<N, kModeSSF>(a, nsub, adj, variant, st);
-    if (mode == kModePspiCompat) return launch_one<N, kModePspiCompat>(a, nsub, adj, variant, st);
-    return launch_one<N, kModePspiLinear>(a, nsub, adj, variant, st);
+    if (mode == kModeSSF) return launch_one<N, kModeSSF>(a, sh, F, adj, st);
+    if (mode == kModePspiCompat) return launch_one<N, kModePspiCompat>(a, sh, F, adj, st);
+    return launch_one<N, kModePspiLinear>(a, sh, F, adj, st);
 }
 
-// F = frequencies per workgroup for migration (2F wavefield subgroups), subgroups per
-// workgroup for modelling (one frequency each).
+// F = frequencies per workgroup (migration: 2F subgroups, or F for PAIR; modelling: F subgroups).
 int launch_march2d(const March2DArgs& a, int nkx, int F, int mode, int adj, int variant, hipStream_t st)
 {
-    const int T = march2d_threads(nkx, variant);
-    if (T <= 0) return set_error(SWEM_EUNSUPPORTED, "2-D march: nkx=%d is not one of 128..4096 (powers of two)", nkx);
-    const int nsub = adj ? 2 * F : F;
-    if (F < 1 || T * nsub > march2d_max_block(nkx, variant))
-        return set_error(SWEM_EINVAL, "2-D march: %d subgroups of %d threads exceed %d", nsub, T,
-                         march2d_max_block(nkx, variant));
+    const March2DShape sh = march2d_shape(nkx, variant, adj);
+    if (sh.T <= 0) return set_error(SWEM_EUNSUPPORTED, "2-D march: nkx=%d is not one of 128..4096 (powers of two)", nkx);
+    if (F < 1 || sh.T * sh.per_freq * F > sh.max_block)
+        return set_error(SWEM_EINVAL, "2-D march: %d frequencies of %d subgroups of %d threads exceed %d", F,
+                         sh.per_freq, sh.T, sh.max_block);
     if (a.nmx > nkx || a.nref < 1 || a.nref > 31 || a.ngroups < 1 || a.nshot < 1 || a.nkh != nkx / 2 + 1 ||
         a.f0 < 0 || a.nfc < 1 || a.f0 + a.nfc > a.nf)
         return set_error(SWEM_EINVAL, "2-D march: inconsistent launch (nmx=%d nkx=%d nref=%d)", a.nmx, nkx, a.nref);
-    if (march2d_lds_bytes(nkx, nsub, variant) > 160 * 1024)
-        return set_error(SWEM_EINVAL, "2-D march: LDS for %d subgroups at nkx=%d exceeds 160 KiB", nsub, nkx);
+    if (march2d_lds_bytes(nkx, sh, F) > 160 * 1024)
+        return set_error(SWEM_EINVAL, "2-D march: LDS for %d frequencies at nkx=%d exceeds 160 KiB", F, nkx);
     switch (nkx) {
-    case 128: return launch_n<128>(a, nsub, mode, adj, variant, st);
-    case 256: return launch_n<256>(a, nsub, mode, adj, variant, st);
-    case 512: return launch_n<512>(a, nsub, mode, adj, variant, st);
-    case 1024: return launch_n<1024>(a, nsub, mode, adj, variant, st);
-    case 2048: return launch_n<2048>(a, nsub, mode, adj, variant, st);
-    case 4096: return launch_n<4096>(a, nsub, mode, adj, variant, st);
+    case 128: return launch_n<128>(a, sh, F, mode, adj, st);
+    case 256: return launch_n<256>(a, sh, F, mode, adj, st);
+    case 512: return launch_n<512>(a, sh, F, mode, adj, st);
+    case 1024: return launch_n<1024>(a, sh, F, mode, adj, st);
+    case 2048: return launch_n<2048>(a, sh, F, mode, adj, st);
+    case 4096: return launch_n<4096>(a, sh, F, mode, adj, st);
     }
     return set_error(SWEM_EUNSUPPORTED, "2-D march: nkx=%d", nkx);
 }
