@@ -731,10 +731,10 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
     if (!(g->dz != 0.f)) return set_error(SWEM_EINVAL, "dz must be non-zero");
     SWEM_TRY(check_device(o.device));
     if (p->three_d) {
-        // y-pass workgroups: F frequencies x 2 wavefields (migration), F x 1 (modelling)
+        // y-pass workgroups: one wave per frequency (both wavefields in migration)
         if (o.freqs_per_block > 0) p->F3 = p->F3f = o.freqs_per_block;
-        if (p->F3 < 1 || 128 * p->F3 > 1024 || 64 * p->F3f > 1024)
-            return set_error(SWEM_EINVAL, "3-D: freqs_per_block=%d (1..8)", o.freqs_per_block);
+        if (p->F3 < 1 || 64 * p->F3 > 1024 || 64 * p->F3f > 1024)
+            return set_error(SWEM_EINVAL, "3-D: freqs_per_block=%d (1..16)", o.freqs_per_block);
     } else {
         const March2DShape sa = march2d_shape(p->nkx, o.variant, 1), sf = march2d_shape(p->nkx, o.variant, 0);
         p->T = sa.T;

@@ -48,13 +48,17 @@ __device__ __forceinline__ float2 kz_phase(const March3DArgs& a, double w, doubl
     const int kxi = ikx <= a.nkx / 2 ? ikx : a.nkx - ikx;
     const int kyi = iky <= a.nky / 2 ? iky : a.nky - iky;
     if (a.prec == SWEM_PREC_KZ64) {
+        // s in FP64 (the cancellation w^2 p^2 - k^2 near the evanescent cutoff), then the
+        // correctly-signed float s, whose square root and phase carry float relative error only
         const double kx = kxi * a.dkx, ky = kyi * a.dky;
-        const double s = w * w * p2 - kx * kx - ky * ky;
-        if (s >= 0.0) {
-            const float2 c = cis_rev(reduce_rev(sqrt(s) * a.dzd));
-            return make_float2(c.x * a.inv_nk, c.y * a.inv_nk);
+        const float s = (float)(w * w * p2 - kx * kx - ky * ky);
+        const float dzf = (float)a.dzd;
+        if (s >= 0.f) {
+            float r = sqrtf(s) * dzf * (float)(1.0 / kTwoPi);
+            r -= rintf(r);
+            return make_float2(__builtin_amdgcn_cosf(r) * a.inv_nk, __builtin_amdgcn_sinf(r) * a.inv_nk);
         }
-        return make_float2(expf((float)(-0.2 * sqrt(-s) * fabs(a.dzd))) * a.inv_nk, 0.f);
+        return make_float2(expf(-0.2f * sqrtf(-s) * fabsf(dzf)) * a.inv_nk, 0.f);
     }
     // the reference's float expressions (wem.c:336-343)
     const float wf = (float)w;
@@ -128,70 +132,79 @@ __global__ __launch_bounds__(256) void march3d_init(March3DArgs a)
     for (int q = 0; q < P; ++q) dst[t + 64 * q] = v[q];
 }
 
-// x-pass.  Workgroup: C columns x nwf wavefields, one wave per (wavefield, column); the tile of
-// C consecutive ky columns is staged through LDS so global reads and writes are row segments of
-// C complex.  wmask: bit 0 source, bit 1 receiver.  conj_src / conj_rec: use conj(L) (-dz).
-template <int NX>
+// x-pass.  Workgroup: C waves, wave w owns column ky0 + w of every stepped wavefield (NWF = 2:
+// source and receiver, which share the kz phase; NWF = 1: the one wmask names).  The tile of C
+// consecutive ky columns is staged through LDS so the global reads and writes are row segments
+// of C complex.  conj_src / conj_rec: use conj(L) (-dz).
+template <int NX, int NWF>
 __global__ __launch_bounds__(NX >= 1024 ? 512 : 1024) void march3d_cols(March3DArgs a, int C)
 {
     constexpr int P = NX / 64;
     constexpr int L = LdsLen<NX>::value;
     extern __shared__ float2 smem[];
-    const int nwf = (a.wmask & 1) + ((a.wmask >> 1) & 1);
     const int tiles = a.nky / C;
     const int item = blockIdx.x / tiles;
     const int ky0 = (blockIdx.x - item * tiles) * C;
     const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
     const int nthr = blockDim.x;
-    // stage: LDS buffer of wave (k, c) holds column ky0 + c of wavefield k (k-th active one)
-    for (int i = threadIdx.x; i < a.nmx * C * nwf; i += nthr) {
+    const int wf1 = (a.wmask & 1) ? 0 : 1;  // the wavefield of an NWF = 1 launch
+    auto wf_of = [&](int k) { return NWF == 2 ? k : wf1; };
+    // stage: buffer (c * NWF + k) holds column ky0 + c of the k-th stepped wavefield
+    for (int i = threadIdx.x; i < a.nmx * C * NWF; i += nthr) {
         const int c = i % C, r = i / C;
-        const int k = r % nwf, x = r / nwf;
-        const int wf = nwf == 2 ? k : ((a.wmask & 1) ? 0 : 1);
-        smem[(k * C + c) * L + lds_pad(x)] = state(a, item, wf)[(size_t)x * a.nky + ky0 + c];
+        const int k = r % NWF, x = r / NWF;
+        smem[(c * NWF + k) * L + lds_pad(x)] = state(a, item, wf_of(k))[(size_t)x * a.nky + ky0 + c];
     }
     __syncthreads();
-    const int k = w / C, c = w - k * C;
-    const int wf = nwf == 2 ? k : ((a.wmask & 1) ? 0 : 1);
-    float2* buf = smem + w * L;
-    float2 v[P];
+    float2 v[NWF][P];
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-        const int x = t + 64 * q;
-        v[q] = x < a.nmx ? buf[lds_pad(x)] : make_float2(0.f, 0.f);
+    for (int k = 0; k < NWF; ++k) {
+        float2* buf = smem + (w * NWF + k) * L;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int x = t + 64 * q;
+            v[k][q] = x < a.nmx ? buf[lds_pad(x)] : make_float2(0.f, 0.f);
+        }
+        wave_sync_lds();
+        wfft<NX, -1>(v[k], buf, a.twx, t);
     }
-    wave_sync_lds();
-    wfft<NX, -1>(v, buf, a.twx, t);
     const int iw = a.ifmin + a.f0 + item;
     const double wd = (double)iw * a.dw;
     const double p2 = a.p2[a.iz];
-    const bool cj = wf == 0 ? a.conj_src : a.conj_rec;
-    const int iky = ky0 + c;
+    const int iky = ky0 + w;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const float2 ph = kz_phase(a, wd, p2, t + 64 * q, iky);
-        v[q] = cj ? cmulc(v[q], ph) : cmul(v[q], ph);
-    }
-    wave_sync_lds();
-    wfft<NX, +1>(v, buf, a.twx, t);
-    wave_sync_lds();
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-        const int x = t + 64 * q;
-        if (x < a.nmx) buf[lds_pad(x)] = v[q];
+        for (int k = 0; k < NWF; ++k) {
+            const bool cj = wf_of(k) == 0 ? a.conj_src : a.conj_rec;
+            v[k][q] = cj ? cmulc(v[k][q], ph) : cmul(v[k][q], ph);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) {
+        float2* buf = smem + (w * NWF + k) * L;
+        wave_sync_lds();
+        wfft<NX, +1>(v[k], buf, a.twx, t);
+        wave_sync_lds();
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int x = t + 64 * q;
+            if (x < a.nmx) buf[lds_pad(x)] = v[k][q];
+        }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < a.nmx * C * nwf; i += nthr) {
-        const int cc = i % C, r = i / C;
-        const int kk = r % nwf, x = r / nwf;
-        const int wff = nwf == 2 ? kk : ((a.wmask & 1) ? 0 : 1);
-        state(a, item, wff)[(size_t)x * a.nky + ky0 + cc] = smem[(kk * C + cc) * L + lds_pad(x)];
+    for (int i = threadIdx.x; i < a.nmx * C * NWF; i += nthr) {
+        const int c = i % C, r = i / C;
+        const int k = r % NWF, x = r / NWF;
+        state(a, item, wf_of(k))[(size_t)x * a.nky + ky0 + c] = smem[(c * NWF + k) * L + lds_pad(x)];
     }
 }
 
-// y-pass of the migration (adjoint): F frequencies x {source, receiver} of one row x.
-// Wave s = 2f + wf.  ds / dr: the source / receiver was stepped at this depth; next_s / next_r:
-// it is stepped at the next depth (its forward y-FFT is needed); the imaging runs when dr.
+// y-pass of the migration (adjoint): F waves, wave f owns row x of the source and the receiver
+// of frequency item g F + f.  ds / dr: the source / receiver was stepped at this depth; next_s /
+// next_r: it is stepped at the next depth (its forward y-FFT is needed); the imaging runs when dr.
+// The split-step phase is computed once and applied conjugated to the source.
 template <int NY>
 __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
 {
@@ -201,74 +214,72 @@ __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
     const int ngroups = (a.nitem + F - 1) / F;
     const int g = blockIdx.x % ngroups, x = blockIdx.x / ngroups;
     const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
-    const int f = w >> 1, wf = w & 1;
-    const int item = g * F + f;
+    const int item = g * F + w;
     const bool active = item < a.nitem;
     const int iw = a.ifmin + a.f0 + (active ? item : 0);
     const double wd = (double)iw * a.dw;
     float2* buf = smem + w * L;
-    const bool stepped = wf == 0 ? a.ds : a.dr;
-    const bool need = active && (stepped || (wf == 0 && a.dr));  // spatial values needed
-    float2 v[P];
-    if (need) {
-        const float2* src = state(a, item, wf) + (size_t)x * a.nky;
+    const bool need_s = active && (a.ds || a.dr), need_r = active && a.dr;
+    float2 vs[P], vr[P];
 #pragma unroll
-        for (int q = 0; q < P; ++q) v[q] = src[t + 64 * q];
-        wfft<NY, +1>(v, buf, a.twy, t);
-        const float tx = a.taper_x[x];
+    for (int q = 0; q < P; ++q) vs[q] = vr[q] = make_float2(0.f, 0.f);
+    if (need_s) {
+        const float2* src = state(a, item, 0) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) {
-            const int y = t + 64 * q;
-            if (y < a.nmy) {
-                if (stepped) {
-                    const float2 s = ss_phase(a, wd, (size_t)x * a.nmy + y);
-                    v[q] = cscale(wf == 0 ? cmulc(v[q], s) : cmul(v[q], s), tx * a.taper_y[y]);
-                }
-            } else
-                v[q] = make_float2(0.f, 0.f);
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);
+        for (int q = 0; q < P; ++q) vs[q] = src[t + 64 * q];
+        wfft<NY, +1>(vs, buf, a.twy, t);
+        wave_sync_lds();
     }
-    // imaging (wem.c:243-247): the source wave hands its row to the receiver wave
-    wave_sync_lds();
-    if (wf == 0) {
+    if (need_r) {
+        const float2* src = state(a, item, 1) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) buf[lds_pad(t + 64 * q)] = v[q];
+        for (int q = 0; q < P; ++q) vr[q] = src[t + 64 * q];
+        wfft<NY, +1>(vr, buf, a.twy, t);
     }
-    __syncthreads();
     float cc[P];
     const float factor = iw == 0 ? 1.f : 2.f;
-    if (wf == 1) {
-        const float2* us = smem + (w - 1) * L;
+    const float tx = a.taper_x[x];
 #pragma unroll
-        for (int q = 0; q < P; ++q) {
-            const float2 s = us[lds_pad(t + 64 * q)];
-            cc[q] = (a.dr && active) ? factor * (v[q].x * s.x + v[q].y * s.y) : 0.f;
+    for (int q = 0; q < P; ++q) {
+        const int y = t + 64 * q;
+        if (y < a.nmy) {
+            if (a.ds || a.dr) {
+                const float2 sp = ss_phase(a, wd, (size_t)x * a.nmy + y);
+                const float tp = tx * a.taper_y[y];
+                if (a.ds) vs[q] = cscale(cmulc(vs[q], sp), tp);
+                if (a.dr) vr[q] = cscale(cmul(vr[q], sp), tp);
+            }
+        } else {
+            vs[q] = vr[q] = make_float2(0.f, 0.f);
         }
+        cc[q] = need_r ? factor * (vr[q].x * vs[q].x + vr[q].y * vs[q].y) : 0.f;
     }
-    __syncthreads();
-    if (wf == 1) {
-        float* red = reinterpret_cast<float*>(buf);
+    // image of the workgroup's frequencies, summed in a fixed order (wem.c:243-247)
+    wave_sync_lds();
+    float* red = reinterpret_cast<float*>(buf);
 #pragma unroll
-        for (int q = 0; q < P; ++q) red[t + 64 * q] = cc[q];
-    }
+    for (int q = 0; q < P; ++q) red[t + 64 * q] = cc[q];
     __syncthreads();
     float* part = a.partial + (size_t)g * a.nm + (size_t)x * a.nmy;
     for (int y = threadIdx.x; y < a.nmy; y += blockDim.x) {
         float sum = 0.f;
-        for (int ff = 0; ff < F; ++ff) sum += reinterpret_cast<const float*>(smem + (2 * ff + 1) * L)[y];
+        for (int ff = 0; ff < F; ++ff) sum += reinterpret_cast<const float*>(smem + ff * L)[y];
         part[y] = sum;
     }
     __syncthreads();
-    // forward y-FFT that starts the next step
-    const bool next = wf == 0 ? a.next_s : a.next_r;
-    if (active && stepped && next) {
-        wfft<NY, -1>(v, buf, a.twy, t);
-        float2* dst = state(a, item, wf) + (size_t)x * a.nky;
+    // forward y-FFTs that start the next step
+    if (active && a.ds && a.next_s) {
+        wfft<NY, -1>(vs, buf, a.twy, t);
+        float2* dst = state(a, item, 0) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) dst[t + 64 * q] = v[q];
+        for (int q = 0; q < P; ++q) dst[t + 64 * q] = vs[q];
+        wave_sync_lds();
+    }
+    if (active && a.dr && a.next_r) {
+        wfft<NY, -1>(vr, buf, a.twy, t);
+        float2* dst = state(a, item, 1) + (size_t)x * a.nky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) dst[t + 64 * q] = vr[q];
     }
 }
 
@@ -344,8 +355,8 @@ bool march3d_supported(int nkx, int nky)
     return ok(nkx) && ok(nky);
 }
 
-// Columns per x-pass workgroup: 16 waves (8 per wavefield) below nkx = 1024, 8 waves at 1024
-// (16 points per lane need more than the 128 registers of a 1024-thread workgroup).
+// Columns (waves) per x-pass workgroup: 16 buffers of staging per workgroup (8 columns of two
+// wavefields or 16 of one) below nkx = 1024, half that at 1024.
 static int march3d_cols_tile(int nwf, int nkx) { return (nkx >= 1024 ? 8 : 16) / nwf; }
 
 template <typename K>
@@ -372,12 +383,14 @@ static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStrea
         const int nwf = (a.wmask & 1) + ((a.wmask >> 1) & 1);
         const int C = march3d_cols_tile(nwf, N);
         const size_t lds = (size_t)C * nwf * L * sizeof(float2);
-        return launch3(march3d_cols<N>, dim3(a.nitem * (a.nky / C)), dim3(64 * C * nwf), lds, st, a, C);
+        const dim3 grid(a.nitem * (a.nky / C)), block(64 * C);
+        return nwf == 2 ? launch3(march3d_cols<N, 2>, grid, block, lds, st, a, C)
+                        : launch3(march3d_cols<N, 1>, grid, block, lds, st, a, C);
     }
     const int ngroups = (a.nitem + F - 1) / F;
     if (which == 2) {
-        const size_t lds = (size_t)2 * F * L * sizeof(float2);
-        return launch3(march3d_rows_adj<N>, dim3(ngroups * a.nmx), dim3(128 * F), lds, st, a, F);
+        const size_t lds = (size_t)F * L * sizeof(float2);
+        return launch3(march3d_rows_adj<N>, dim3(ngroups * a.nmx), dim3(64 * F), lds, st, a, F);
     }
     const size_t lds = (size_t)F * L * sizeof(float2);
     SWEM_HIP(hipFuncSetAttribute((const void*)march3d_rows_fwd<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -394,7 +407,7 @@ int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_
     const int n = which == 1 ? a.nkx : a.nky;
     if (!march3d_supported(a.nkx, a.nky) || a.nmx > a.nkx || a.nmy > a.nky || a.nitem < 1)
         return set_error(SWEM_EINVAL, "3-D march: inconsistent launch (nkx=%d nky=%d)", a.nkx, a.nky);
-    if ((which == 2 && 128 * F > 1024) || (which == 3 && 64 * F > 1024) || F < 1)
+    if (((which == 2 || which == 3) && 64 * F > 1024) || F < 1)
         return set_error(SWEM_EINVAL, "3-D march: %d frequencies per workgroup", F);
     switch (n) {
     case 128: return launch3_n<128>(which, a, F, phase, st);

@@ -86,7 +86,7 @@ struct swem_plan {
     // frequency chunks: [c * nfc, min(nf, (c + 1) * nfc)); the phase tables hold one chunk
     int nfc = 0, nchunks = 1, resident = -1;
     bool three_d = false;
-    int F3 = 4, F3f = 8;  // frequencies per 3-D y-pass workgroup (migration, modelling)
+    int F3 = 8, F3f = 8;  // frequencies per 3-D y-pass workgroup (migration, modelling)
     std::vector<float> vel;  // host copy (source of every table)
     // construction context: stream of the table builds, optional shared bases
     hipStream_t build_stream = nullptr;
