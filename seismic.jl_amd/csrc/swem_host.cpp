@@ -710,7 +710,7 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
         return set_error(SWEM_EINVAL, "pspi_mode %d", o.pspi_mode);
     if (o.precision != SWEM_PREC_KZ64 && o.precision != SWEM_PREC_FP32)
         return set_error(SWEM_EINVAL, "precision %d", o.precision);
-    if (o.variant < SWEM_VARIANT_AUTO || o.variant > SWEM_VARIANT_PAIR) return set_error(SWEM_EINVAL, "variant %d", o.variant);
+    if (o.variant < SWEM_VARIANT_AUTO || o.variant > SWEM_VARIANT_PAIR8) return set_error(SWEM_EINVAL, "variant %d", o.variant);
     p->g = *g;
     p->opt = o;
     SWEM_TRY(compute_sampling(p->g, p->s));

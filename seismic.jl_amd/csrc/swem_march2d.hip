@@ -532,10 +532,10 @@ __global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(2)))
 
 // PAIR: 16 points per thread from nkx = 1024 (nkx/16 threads, radix-16 passes), one wave with
 // nkx/64 points per lane below it.
-template <int N, int MODE>
-__global__ __launch_bounds__(N / 16) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_pair(March2DArgs a)
+template <int N, int MODE, int P>
+__global__ __launch_bounds__(N / P) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_pair(March2DArgs a)
 {
-    adj_body_pair<N, 16, MODE, false>(a);
+    adj_body_pair<N, P, MODE, false>(a);
 }
 
 template <int N, int MODE>
@@ -559,7 +559,8 @@ March2DShape march2d_shape(int nkx, int variant, int adj)
     March2DShape sh{};
     if (nkx < 128 || nkx > 4096 || (nkx & (nkx - 1))) return sh;
     int v = variant;
-    if (v == SWEM_VARIANT_PAIR && !adj) v = SWEM_VARIANT_AUTO;
+    if ((v == SWEM_VARIANT_PAIR || v == SWEM_VARIANT_PAIR8) && !adj) v = SWEM_VARIANT_AUTO;
+    if (v == SWEM_VARIANT_PAIR8 && nkx < 512) v = SWEM_VARIANT_PAIR;
     if (v == SWEM_VARIANT_AUTO) v = adj ? SWEM_VARIANT_PAIR : (nkx <= 512 ? SWEM_VARIANT_WAVE : SWEM_VARIANT_SUBGROUP16);
     if (v == SWEM_VARIANT_WAVE && nkx > 2048) v = SWEM_VARIANT_SUBGROUP16;
     if (v == SWEM_VARIANT_SUBGROUP16 && nkx < 1024) v = SWEM_VARIANT_SUBGROUP;
@@ -574,6 +575,9 @@ March2DShape march2d_shape(int nkx, int variant, int adj)
         break;
     case SWEM_VARIANT_PAIR:
         sh.T = nkx >= 1024 ? nkx / 16 : 64, sh.per_freq = 1, sh.max_block = sh.T, sh.bufs = nkx >= 1024 ? 2 : 1;
+        break;
+    case SWEM_VARIANT_PAIR8:
+        sh.T = nkx / 8, sh.per_freq = 1, sh.max_block = sh.T, sh.bufs = 2;
         break;
     default:
         sh.variant = SWEM_VARIANT_SUBGROUP;
@@ -611,7 +615,11 @@ static int launch_one(const March2DArgs& a, const March2DShape& sh, int F, int a
         if constexpr (N < 1024)
             return launch_kernel(march2d_adj_pair_w<N, MODE>, a, block, lds, st);
         else
-            return launch_kernel(march2d_adj_pair<N, MODE>, a, block, lds, st);
+            return launch_kernel(march2d_adj_pair<N, MODE, 16>, a, block, lds, st);
+    }
+    if (sh.variant == SWEM_VARIANT_PAIR8) {
+        if constexpr (N >= 512)
+            return launch_kernel(march2d_adj_pair<N, MODE, 8>, a, block, lds, st);
     }
     if constexpr (N <= 2048) {
         if (sh.variant == SWEM_VARIANT_WAVE)

@@ -33,7 +33,8 @@ def oparams(g: seiswem.WemGeometry, pspi_mode: int = 0) -> O.WemParams:
                        padx=g.padx, pspi=g.pspi, pspi_mode=pspi_mode)
 
 
-VARIANTS = [seiswem.VARIANT_SUBGROUP, seiswem.VARIANT_WAVE, seiswem.VARIANT_SUBGROUP16, seiswem.VARIANT_PAIR]
+VARIANTS = [seiswem.VARIANT_SUBGROUP, seiswem.VARIANT_WAVE, seiswem.VARIANT_SUBGROUP16, seiswem.VARIANT_PAIR,
+            seiswem.VARIANT_PAIR8]
 CASES = [(False, 0), (True, 0), (True, 1)]
 
 
