@@ -6,4 +6,6 @@ from ._lib import (PREC_FP32, PREC_KZ64, PSPI_COMPAT, PSPI_LINEAR, VARIANT_AUTO,
                    SWEM_EIO, SWEM_EUNSUPPORTED, SWEM_EBUSY)
 from .wem import DeviceArray, Plan, ShotProfileWEM, WemGeometry, copy_device, synchronize, wem  # noqa: F401
 from .serve import Server  # noqa: F401
+from .files import ShotProfileWEMFiles  # noqa: F401
+from . import seis, solvers  # noqa: F401
 from . import synth  # noqa: F401

@@ -197,3 +197,30 @@ def test_shotwem_so_main_callable_in_process(tmp_path, monkeypatch):
     so.main.restype = ctypes.c_int
     argv = [b"0", b"adj=y"]
     assert so.main(len(argv), (ctypes.c_char_p * 2)(*argv)) == _lib.SWEM_EINVAL
+
+
+def test_seis_module_counts_traces_from_the_files(tmp_path, monkeypatch):
+    """seiswem.seis writes every trace it is given and reads the trace count from the headers file
+    (SeisWrite.jl:31-40, Header.jl:223-230), also when n2*n3*n4*n5 differs from it."""
+    from seiswem import seis
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("DATAPATH", raising=False)
+    assert seis.HEADER_DTYPE.itemsize == 124
+    d = np.arange(30, dtype=np.float32).reshape(6, 5)
+    h = seis.headers(6, tracenum=np.arange(1, 7), sx=[0, 0, 0, 10, 10, 10], gx=np.arange(6) * 2.5)
+    fh = seis.extent((5, 3, 1, 2, 2), (0.0, 0.0, 0.0, 0.0, 0.0), (0.004, 2.5, 2.5, 10.0, 1.0),
+                     ("Time", "gx", "gy", "sx", "sy"), ("s", "m", "m", "m", "m"))
+    seis.write("c.seis", d, h, fh)
+    d2, h2, fh2 = seis.read("c.seis")
+    assert np.array_equal(d2, d) and np.array_equal(h2, h)
+    assert (fh2.n4, fh2.n5, fh2.label5, fh2.unit1) == (2, 2, b"sy", b"s")
+    # a consistent extent reads back identically through the C reader
+    fh3 = seis.extent((5, 6))
+    seis.write("e.seis", d, h, fh3)
+    d3, h3, _ = seis.read("e.seis")
+    data = np.zeros((6, 5), np.float32)
+    hc = header_array(6)
+    fh4 = _lib.SeisExtent()
+    _lib.check(seiswem.load().swem_seis_read(b"e.seis", data.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                             hc.ctypes.data, ctypes.byref(fh4)))
+    assert np.array_equal(data, d3) and hc.tobytes() == h3.tobytes()

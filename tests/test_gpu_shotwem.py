@@ -66,3 +66,63 @@ def test_shotwem_main_migrates_and_models_like_the_oracle(gpu, tmp_path, monkeyp
     assert np.allclose(hd["gx"], mx) and np.allclose(hd["hx"], mx - 200.0)  # shotwem.c:86-98
     d_ref = O.wem(p, vel, wav, 200.0, m=m, adj=False)
     assert rel_l2(dm, d_ref) <= 1e-4
+
+
+def _grid_files(lib, nx, nz, nt, dx, dz, dt, rng):
+    vel = synth.blob_velocity(nx, nz, dx, dz, rng, nblobs=3, v0=2000.0)
+    wav = synth.wavelet_trace(nt, dt, 15.0)
+    mx = dx * np.arange(nx)
+    write_seis(lib, "vel.seis", vel, header_array(nx, tracenum=np.arange(1, nx + 1), n1=nz, d1=dz, mx=mx,
+                                                  imx=np.arange(nx)), nz, nx, o1=0.0, d1=dz, o2=0.0, d2=dx)
+    write_seis(lib, "wav.seis", wav[None], header_array(1, n1=nt, d1=dt), nt, 1, o1=0.0, d1=dt)
+    return vel, wav, mx
+
+
+def test_shot_profile_wem_files_stacks_and_concatenates_like_the_reference(gpu, tmp_path, monkeypatch):
+    """ShotProfileWEM(m, d, adj) on .seis files (ShotProfileWEM.jl:39-310): per-shot windows by
+    sx/sy, images stacked at imx(gx) * ny + imy, modelled shots concatenated with their headers."""
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("DATAPATH", raising=False)
+    lib = seiswem.load()
+    nx, nz, nt, dx, dz, dt = 64, 40, 256, 10.0, 10.0, 0.004
+    rng = np.random.default_rng(5)
+    vel, wav, mx = _grid_files(lib, nx, nz, nt, dx, dz, dt, rng)
+    sx = np.array([150.0, 320.0, 480.0], np.float32)
+    d = rng.standard_normal((3, nx, nt)).astype(np.float32)
+    # shot 1's traces are stored with their receivers in reverse order: wem images trace ix at grid
+    # point ix and the stack then places it at the receiver's imx (shotwem.c:113-117).
+    gx = np.stack([mx, mx[::-1], mx])
+    order = np.array([0, 2, 1])  # shots interleaved in the file, windowed by their sx header
+    h = np.concatenate([header_array(nx, n1=nt, d1=dt, sx=sx[k], gx=gx[k]) for k in order])
+    write_seis(lib, "d.seis", d[order].reshape(-1, nt), h, nt, 3 * nx, o1=0.0, d1=dt)
+    common = dict(vel="vel.seis", wav="wav.seis", fmax=40.0, nref=4, padt=2, padx=2, pspi=True)
+    m = seiswem.ShotProfileWEMFiles("m.seis", "d.seis", True, sx=sx, sy=np.zeros(3, np.float32), **common)
+    p = O.WemParams(nt=nt, dt=dt, nmx=nx, dmx=dx, nz=nz, dz=dz, fmin=0.0, fmax=40.0, nref=4, padt=2, padx=2,
+                    pspi=True, pspi_mode=0)
+    m_ref = np.zeros((nx, nz))
+    for k in range(3):
+        img = O.wem(p, vel, wav, float(sx[k]), d=d[k], adj=True)
+        m_ref[np.trunc(gx[k] / dx).astype(int)] += img
+    assert rel_l2(m, m_ref) <= 1e-4
+    mr, hm, fh = seiswem.seis.read("m.seis")
+    assert np.array_equal(mr, m)
+    assert (fh.n1, fh.n2, fh.n3, fh.n4, fh.n5) == (nz, nx, 1, 1, 1)
+    assert [fh.label1, fh.label2, fh.label4] == [b"Depth", b"mx", b"angx"] and fh.unit4 == b"degrees"
+    assert np.array_equal(hm["tracenum"], np.arange(1, nx + 1))  # SeisWrite.jl:36-39
+    assert np.array_equal(hm["imx"], np.arange(nx)) and np.array_equal(hm["mx"], mx.astype(np.float32))
+    assert np.all(hm["n1"] == nz) and np.all(hm["d1"] == np.float32(dz))
+
+    # demigration of the stacked image, two shots concatenated (ShotProfileWEM.jl:203-309)
+    dm = seiswem.ShotProfileWEMFiles("m.seis", "dm.seis", False, sx=sx[:2], sy=np.zeros(2, np.float32), **common)
+    dr, hd, fh = seiswem.seis.read("dm.seis")
+    assert dr.shape == (2 * nx, nt) and np.array_equal(dr.reshape(2, nx, nt), dm)
+    assert (fh.n1, fh.n2, fh.n3, fh.n4, fh.n5) == (nt, nx, 1, 2, 2)
+    assert [fh.label1, fh.label2, fh.label4, fh.label5] == [b"Time", b"gx", b"sx", b"sy"]
+    assert abs(fh.o4 - sx[0]) < 1e-3 and abs(fh.d4 - (sx[1] - sx[0])) < 1e-3 and fh.d5 == 1.0
+    assert np.array_equal(hd["tracenum"], np.arange(1, 2 * nx + 1))
+    for k in range(2):
+        hk = hd[k * nx:(k + 1) * nx]
+        assert np.all(hk["sx"] == sx[k]) and np.array_equal(hk["gx"], mx.astype(np.float32))
+        assert np.array_equal(hk["hx"], mx.astype(np.float32) - sx[k]) and np.all(hk["n1"] == nt)
+        d_ref = O.wem(p, vel, wav, float(sx[k]), m=m, adj=False)
+        assert rel_l2(dm[k], d_ref) <= 1e-4
