@@ -350,7 +350,7 @@ def main():
         os.environ.setdefault("OMP_NUM_THREADS", str(cores))
 
         def m_gpu_fn(gs, dtr):
-            return seiswem.ShotProfileWEM(dtr[None], True, vel=vel, wav=wav, geom=gs, sx=sx[:1])
+            return seiswem.ShotProfileWEM(dtr[None], True, vel=vel, wav=wav, geom=gs, sx=sx[:1], variant=args.variant)
 
         cb, err, nf_samp = cpu_baseline(g, vel, wav, float(sx[0]), m_gpu_fn, cores)
         out["cpu_baseline"] = cb

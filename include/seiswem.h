@@ -48,6 +48,7 @@ extern "C" {
 #define SWEM_VARIANT_SUBGROUP16 3 /* nkx/16 threads per wavefield, radix-16 passes (nkx >= 1024) */
 #define SWEM_VARIANT_PAIR 4      /* migration: one subgroup per frequency marches source and receiver */
 #define SWEM_VARIANT_PAIR8 5     /* PAIR with nkx/8 threads (8 points each), nkx >= 512 */
+#define SWEM_VARIANT_PACKED 6    /* PSPI compat migration: both real wavefields in one transform (AUTO where valid) */
 
 /* Geometry of one migration problem: the scalar arguments of wem() (wem.c:3-11). */
 typedef struct swem_geometry {

@@ -189,11 +189,14 @@ __device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* _
             // transform would pin 42 registers for the whole kernel.
             int ti = k * (N / (NS * R));
             asm volatile("" : "+v"(ti));
+            // every twiddle load in flight before the first use: under register pressure the
+            // scheduler would otherwise pair each load with its use and serialise the latencies
+            float2 w[R];
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const float2 w = tw[r * ti];
-                x[r] = cmul(x[r], SIGN > 0 ? make_float2(w.x, -w.y) : w);
-            }
+            for (int r = 1; r < R; ++r) w[r] = tw[r * ti];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 1; r < R; ++r) x[r] = cmul(x[r], SIGN > 0 ? make_float2(w[r].x, -w[r].y) : w[r]);
         }
         Dft<R, SIGN>::run(x);
         if constexpr (LAST) {
@@ -220,18 +223,51 @@ __device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* _
 // read issued after a write observes it and one buffer (LdsLen<N> complex, private to the wave)
 // suffices with no s_barrier; wave_barrier only stops the compiler from reordering the LDS
 // accesses across the exchange.
-template <int N, int SIGN, int NS = 1>
-__device__ __forceinline__ void wfft(float2 (&v)[N / 64], float2* buf, const float2* __restrict__ tw, int t)
+template <int N, int NS>
+struct WPass {
+    static constexpr int P = N / 64;
+    static constexpr int REM = N / NS;
+    static constexpr int RMAX = P < 8 ? P : 8;
+    static constexpr int R = REM >= RMAX ? RMAX : REM;
+    static constexpr int B = P / R;
+    static constexpr bool LAST = (NS * R == N);
+    static constexpr int NTW = NS > 1 ? B * (R - 1) : 1;  // twiddles per lane (1: unused slot)
+};
+
+// The twiddles of pass NS of wfft<N>: w[b (R - 1) + r - 1] = tw[r k N / (NS R)].
+template <int N, int NS>
+__device__ __forceinline__ void wfft_twiddles(const float2* __restrict__ tw, int t, float2 (&w)[WPass<N, NS>::NTW])
 {
-    constexpr int P = N / 64;
+    using S = WPass<N, NS>;
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int b = 0; b < S::B; ++b) {
+            const int k = (t + b * 64) & (NS - 1);
+            int ti = k * (N / (NS * S::R));
+            asm volatile("" : "+v"(ti));  // keep the loads in the transform (see fft)
+#pragma unroll
+            for (int r = 1; r < S::R; ++r) w[b * (S::R - 1) + r - 1] = tw[r * ti];
+        }
+    }
+}
+
+// One pass and the passes after it; w holds this pass's twiddles.  The next pass's twiddles are
+// loaded at the start of this one, so their latency overlaps this pass's butterflies and exchange.
+template <int N, int SIGN, int NS>
+__device__ __forceinline__ void wfft_pass(float2 (&v)[N / 64], float2* buf, const float2* __restrict__ tw, int t,
+                                          const float2 (&w)[WPass<N, NS>::NTW])
+{
+    using S = WPass<N, NS>;
+    constexpr int P = S::P, R = S::R, B = S::B;
     constexpr int T = 64;
-    constexpr int REM = N / NS;
-    constexpr int RMAX = P < 8 ? P : 8;
-    constexpr int R = REM >= RMAX ? RMAX : REM;
-    constexpr int B = P / R;
-    constexpr bool LAST = (NS * R == N);
     constexpr int SH = PassPad<NS, R>::value;
     static_assert(P % R == 0, "radix must divide the points per lane");
+    constexpr int NSN = S::LAST ? NS : NS * R;
+    // prefetch only where it costs at most 7 complex registers (N <= 512); larger transforms load
+    // their twiddles in the pass that uses them
+    constexpr bool PRE = WPass<N, NSN>::NTW <= 7;
+    float2 wn[WPass<N, NSN>::NTW];
+    if constexpr (!S::LAST && PRE) wfft_twiddles<N, NSN>(tw, t, wn);
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = t + b * T;
@@ -240,16 +276,14 @@ __device__ __forceinline__ void wfft(float2 (&v)[N / 64], float2* buf, const flo
 #pragma unroll
         for (int r = 0; r < R; ++r) x[r] = v[b + r * B];
         if constexpr (NS > 1) {
-            int ti = k * (N / (NS * R));
-            asm volatile("" : "+v"(ti));
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                const float2 w = tw[r * ti];
-                x[r] = cmul(x[r], SIGN > 0 ? make_float2(w.x, -w.y) : w);
+                const float2 wr = w[b * (R - 1) + r - 1];
+                x[r] = cmul(x[r], SIGN > 0 ? make_float2(wr.x, -wr.y) : wr);
             }
         }
         Dft<R, SIGN>::run(x);
-        if constexpr (LAST) {
+        if constexpr (S::LAST) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b + r * B] = x[r];
         } else {
@@ -258,7 +292,7 @@ __device__ __forceinline__ void wfft(float2 (&v)[N / 64], float2* buf, const flo
             for (int r = 0; r < R; ++r) buf[padsh<SH>(base + r * NS)] = x[r];
         }
     }
-    if constexpr (!LAST) {
+    if constexpr (!S::LAST) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -267,8 +301,16 @@ __device__ __forceinline__ void wfft(float2 (&v)[N / 64], float2* buf, const flo
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        wfft<N, SIGN, NS * R>(v, buf, tw, t);
+        if constexpr (!PRE) wfft_twiddles<N, NSN>(tw, t, wn);
+        wfft_pass<N, SIGN, NSN>(v, buf, tw, t, wn);
     }
+}
+
+template <int N, int SIGN>
+__device__ __forceinline__ void wfft(float2 (&v)[N / 64], float2* buf, const float2* __restrict__ tw, int t)
+{
+    const float2 none[1] = {make_float2(0.f, 0.f)};
+    wfft_pass<N, SIGN, 1>(v, buf, tw, t, none);
 }
 
 }  // namespace swem

@@ -70,6 +70,7 @@ struct March2DArgs {
     const uint8_t* sel2;    // [nz][nkx] iref2 (PSPI linear)
     const float* w2;        // [nz][nkx] weight of iref2 (PSPI linear)
     const uint32_t* used;   // [nz] bit ir set when some x selects vref ir at iz
+    const int* cref;        // [nz] vref of the uncut points of depth iz (PSPI compat), -1: none
     float* partial;         // [nshot][ngroups][nz][nmx] migration partial images
     const float* mT;        // [nz][nmx] image (modelling input, depth-major)
     float2* smig;           // [nshot][nfc][nz][nmx] source wavefield scratch (modelling)
@@ -103,7 +104,7 @@ struct March3DArgs {
     int conj_src, conj_rec;     // x-pass: conj(L) for a -dz step
     int ds, dr, next_s, next_r; // stepped at this depth / at the next depth
     int first;                  // modelling receiver: first step (zero state)
-    double dw, dkx, dky, dzd;
+    double dw, dkx, dky, dzd, dkx2;  // dkx2 = dkx^2
     float dkxf, dkyf, inv_nk;
     const double* p2;           // [nz] po^2
     const float* p2f;           // [nz] float po^2 (naive FP32)

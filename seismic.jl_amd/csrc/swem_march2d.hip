@@ -87,6 +87,17 @@ __device__ __forceinline__ float2 kload(const float2* __restrict__ row, int t, i
 
 __device__ __forceinline__ float2 cmul_c(float2 x, float2 L, bool conj) { return conj ? cmulc(x, L) : cmul(x, L); }
 
+// A table row of the strided mapping into registers with every load issued before the first
+// use (under register pressure the scheduler otherwise pairs each load with its use and
+// serialises the latencies).
+template <int N, int P>
+__device__ __forceinline__ void kload_row(float2 (&L)[P], const float2* __restrict__ row, int t)
+{
+#pragma unroll
+    for (int q = 0; q < P; ++q) L[q] = kload<N, P>(row, t, q);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
 // One x-transform of the wavefield: a wave-level transform (WAVE, P = N/64, no barriers) or a
 // subgroup transform through ping-pong buffers.
 template <int N, int P, int SIGN, bool WAVE>
@@ -124,18 +135,25 @@ __device__ __forceinline__ void ssf_adj_step(float2 (&u)[P], float2* lds, int& p
 {
     constexpr int T = N / P;
     xfft<N, P, -1, WAVE>(u, lds, a.tw, c.t, pp);
-    const float2* row = krow(a, c, 0);
+    if constexpr (ACT) {
+        float2 L[P];
+        kload_row<N, P>(L, krow(a, c, 0), c.t);
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-        const float2 L = ACT ? kload<N, P>(row, c.t, q) : make_float2(a.inv_nk, 0.f);
-        u[q] = cmul_c(u[q], L, conj);
+        for (int q = 0; q < P; ++q) u[q] = cmul_c(u[q], L[q], conj);
+    } else {
+#pragma unroll
+        for (int q = 0; q < P; ++q) u[q] = cscale(u[q], a.inv_nk);
     }
     xfft<N, P, +1, WAVE>(u, lds, a.tw, c.t, pp);
     if constexpr (ACT) {
         const auto sr = ss_row(a, c);
+        float2 sp[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) sp[q] = buf_load2(sr, (uint32_t)(c.t + q * T) * 8u);
+        __builtin_amdgcn_sched_barrier(0);  // all loads in flight before the first use
 #pragma unroll
         for (int q = 0; q < P; ++q)  // zero beyond nmx: the split-step row reads zero there
-            u[q] = cscale(cmul_c(u[q], buf_load2(sr, (uint32_t)(c.t + q * T) * 8u), conj), tap[q]);
+            u[q] = cscale(cmul_c(u[q], sp[q], conj), tap[q]);
     } else {
 #pragma unroll
         for (int q = 0; q < P; ++q)
@@ -151,12 +169,21 @@ __device__ __forceinline__ void ssf_fwd_step(float2 (&u)[P], float2* lds, int& p
 {
     constexpr int T = N / P;
     const auto sr = ss_row(a, c);
+    {
+        float2 sp[P];
 #pragma unroll
-    for (int q = 0; q < P; ++q) u[q] = cmulc(cscale(u[q], tap[q]), buf_load2(sr, (uint32_t)(c.t + q * T) * 8u));
+        for (int q = 0; q < P; ++q) sp[q] = buf_load2(sr, (uint32_t)(c.t + q * T) * 8u);
+        __builtin_amdgcn_sched_barrier(0);  // all loads in flight before the first use
+#pragma unroll
+        for (int q = 0; q < P; ++q) u[q] = cmulc(cscale(u[q], tap[q]), sp[q]);
+    }
     xfft<N, P, -1, WAVE>(u, lds, a.tw, c.t, pp);
-    const float2* row = krow(a, c, 0);
+    {
+        float2 L[P];
+        kload_row<N, P>(L, krow(a, c, 0), c.t);
 #pragma unroll
-    for (int q = 0; q < P; ++q) u[q] = cmulc(u[q], kload<N, P>(row, c.t, q));
+        for (int q = 0; q < P; ++q) u[q] = cmulc(u[q], L[q]);
+    }
     xfft<N, P, +1, WAVE>(u, lds, a.tw, c.t, pp);
 #pragma unroll
     for (int q = 0; q < P; ++q)
@@ -202,12 +229,14 @@ __device__ __forceinline__ void pspi_select_step(float2 (&u)[P], float2* lds, in
     const int first = __builtin_ctz(used | 0x80000000u);
     for (int ir = 0; ir < a.nref; ++ir) {
         if (!((used >> ir) & 1u)) continue;
-        const float2* row = krow(a, c, ir);
         float2 B[P];
+        if constexpr (ACT) {
+            kload_row<N, P>(B, krow(a, c, ir), c.t);
 #pragma unroll
-        for (int q = 0; q < P; ++q) {
-            const float2 L = ACT ? kload<N, P>(row, c.t, q) : make_float2(a.inv_nk, 0.f);
-            B[q] = cmul_c(u[q], L, conj);
+            for (int q = 0; q < P; ++q) B[q] = cmul_c(u[q], B[q], conj);
+        } else {
+#pragma unroll
+            for (int q = 0; q < P; ++q) B[q] = cscale(u[q], a.inv_nk);
         }
         xfft<N, P, +1, WAVE>(B, lds, a.tw, c.t, pp);
 #pragma unroll
@@ -252,8 +281,10 @@ __device__ __forceinline__ void pspi_weight_step(float2 (&u)[P], float2* lds, in
         for (int q = 0; q < P; ++q) B[q] = cscale(u[q], pspi_linear_weight(ir, s1[q], s2[q], w2[q]));
         xfft<N, P, -1, WAVE>(B, lds, a.tw, c.t, pp);
         const float2* row = krow(a, c, ir);
+        float2 L[P];
+        kload_row<N, P>(L, row, c.t);
 #pragma unroll
-        for (int q = 0; q < P; ++q) acc[q] = cadd(acc[q], cmulc(B[q], kload<N, P>(row, c.t, q)));
+        for (int q = 0; q < P; ++q) acc[q] = cadd(acc[q], cmulc(B[q], L[q]));
     }
     xfft<N, P, +1, WAVE>(acc, lds, a.tw, c.t, pp);
 #pragma unroll
@@ -449,11 +480,17 @@ __device__ __forceinline__ void fwd_body(const March2DArgs& a)
         if (active) {
             const auto sr = buf_rsrc(sm + (size_t)iz * a.nmx, (uint32_t)a.nmx * 8u);
             const auto mr = buf_rsrc(a.mT + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
+            float2 sv[P];
+            float mv[P];
 #pragma unroll
             for (int q = 0; q < P; ++q) {
                 const uint32_t x = (uint32_t)(t + q * T);
-                u[q] = cadd(u[q], cscale(buf_load2(sr, x * 8u), buf_load1(mr, x * 4u)));
+                sv[q] = buf_load2(sr, x * 8u);
+                mv[q] = buf_load1(mr, x * 4u);
             }
+            __builtin_amdgcn_sched_barrier(0);  // all loads in flight before the first use
+#pragma unroll
+            for (int q = 0; q < P; ++q) u[q] = cadd(u[q], cscale(sv[q], mv[q]));
         }
         fwd_step<N, P, MODE, WAVE>(u, lds, pp, a, c, tap);
     }
@@ -510,6 +547,176 @@ __device__ __forceinline__ void adj_body_pair(const March2DArgs& a)
     }
 }
 
+// ---------------------------------------------------------------- packed PSPI compat migration
+// In compat mode pspiop keeps only the real part of its result wherever vref2 - vref1 > 10
+// (wem.c:450-458 with linear_interp, wem.c:473-478), so after a step both wavefields are real
+// except at the few points of each depth with iref1 == iref2 (v at the depth's maximum, or every
+// point of a depth with vmax <= vmin + 10).  With s = s_R + i s_I, r = r_R + i r_I the state is
+//   z = s_R + i a r_R        e = s_I - i a r_I          (a: balances the two magnitudes)
+// and, because L_ir(k) is even in kx and FFTs of real sequences are Hermitian,
+//   Re P_src(s) + i a Re P_rec(r) = IFFT( Re L Z + Im L E )                        (all x)
+//   Im P_src(s) - i a Im P_rec(r) = IFFT( Re L E - Im L Z )                        (all x)
+// with Z = FFT(z), E = FFT(e), P_src using conj(L) (-dz) and P_rec using L.  One inverse
+// transform per vref thus steps both wavefields; one more, with the vref of the uncut points,
+// gives their new e: 2 + nref + 1 transforms per depth instead of 2 (1 + nref).  The image is
+// factor Re(r conj s) = factor (Re z Im z - Re e Im e) / a.
+
+// Sum over the T threads of a subgroup in a fixed order (red: T/64 floats of LDS that no
+// transform touches: the last four complex of a buffer lie beyond every padded index).
+template <int T>
+__device__ __forceinline__ float subgroup_sum(float v, float* red)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if constexpr (T == 64) {
+        return v;
+    } else {
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        float s = 0.f;
+#pragma unroll
+        for (int i = 0; i < T / 64; ++i) s += red[i];
+        __syncthreads();
+        return s;
+    }
+}
+
+
+template <int N, int P, bool WAVE>
+__device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
+{
+    constexpr int T = N / P;
+    extern __shared__ float2 smem[];
+    int g, shot;
+    decode_block(a.ngroups, a.nshot, g, shot);
+    if (g < 0) return;  // padding block of the XCD-aware grid
+    const int t = threadIdx.x;
+    const int fc = g;  // one frequency per workgroup
+    const bool active = fc < a.nfc;
+    const int fl = a.f0 + fc;
+    const int iw = a.ifmin + fl;
+    StepCtx c;
+    c.t = t;
+    c.fc = __builtin_amdgcn_readfirstlane(active ? fc : 0);
+    const float factor = iw == 0 ? 1.f : 2.f;
+    int pp = 0;
+    const int igx = a.igx[shot];
+    const float2 ss = active ? a.spec_s[fl] : make_float2(0.f, 0.f);
+    float2 z[P], e[P];
+    float nr = 0.f;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int x = t + q * T;
+        e[q] = (active && x < a.nmx) ? a.spec_r[((size_t)shot * a.nmx + x) * a.nf + fl] : make_float2(0.f, 0.f);
+        nr += e[q].x * e[q].x + e[q].y * e[q].y;
+    }
+    float* red = reinterpret_cast<float*>(smem + LdsLen<N>::value - 4);
+    static_assert(T / 64 <= 8, "reduction slots");
+    nr = subgroup_sum<T>(nr, red);
+    const float ns = sqrtf(ss.x * ss.x + ss.y * ss.y);
+    float alpha = (nr > 0.f && ns > 0.f) ? ns / sqrtf(nr) : 1.f;
+    float inv_alpha = 1.f / alpha;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int x = t + q * T;
+        const float2 us = x == igx ? ss : make_float2(0.f, 0.f);
+        z[q] = make_float2(us.x, alpha * e[q].x);
+        e[q] = make_float2(us.y, -alpha * e[q].y);
+    }
+    float* part = a.partial + ((size_t)shot * a.ngroups + g) * (size_t)a.nz * a.nmx;
+    bool has_e = true;  // e may be nonzero (initial wavefields are complex); false: e == 0
+    for (int iz = 0; iz < a.nz; ++iz) {
+        c.iz = iz;
+        const int cr = a.cref[iz];
+        xfft<N, P, -1, WAVE>(z, smem, a.tw, t, pp);
+        if (has_e) xfft<N, P, -1, WAVE>(e, smem, a.tw, t, pp);
+        // selection bytes packed four to a register: (sel(q) = s1w[q / 4] >> 8 (q % 4)) & 0xff
+        uint32_t s1w[(P + 3) / 4];
+        float2 out[P];
+#pragma unroll
+        for (int q = 0; q < (P + 3) / 4; ++q) s1w[q] = 0u;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            s1w[q >> 2] |= (uint32_t)a.sel[(size_t)iz * N + t + q * T] << (8 * (q & 3));
+            out[q] = make_float2(0.f, 0.f);
+        }
+        auto s1 = [&](int q) { return (s1w[q >> 2] >> (8 * (q & 3))) & 0xffu; };
+        const uint32_t used = a.used[iz];
+        for (int ir = 0; ir < a.nref; ++ir) {
+            if (!((used >> ir) & 1u)) continue;
+            const float2* row = krow(a, c, ir);
+            float2 w[P];
+            kload_row<N, P>(w, row, c.t);
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const float2 L = w[q];
+                w[q] = make_float2(z[q].x * L.x + e[q].x * L.y, z[q].y * L.x + e[q].y * L.y);
+            }
+            xfft<N, P, +1, WAVE>(w, smem, a.tw, t, pp);
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+                if ((int)(s1(q) & 0x7fu) == ir) out[q] = w[q];
+        }
+        if (cr >= 0) {
+            // imaginary parts at the uncut points of this depth (all select vref cr):
+            // e_new = IFFT(Re L E - Im L Z), the conjugate of IFFT(V) by L(-k) = L(k)
+            const float2* row = krow(a, c, cr);
+            float2 lv[P];
+            kload_row<N, P>(lv, row, c.t);
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const float2 L = lv[q];
+                e[q] = make_float2(L.x * e[q].x - L.y * z[q].x, L.x * e[q].y - L.y * z[q].y);
+            }
+            xfft<N, P, +1, WAVE>(e, smem, a.tw, t, pp);
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int x = t + q * T;
+                if (!(x < a.nmx && !(s1(q) >> 7))) e[q] = make_float2(0.f, 0.f);
+            }
+        }
+        has_e = cr >= 0;
+        // imaging condition (wem.c:243-247): factor Re(ur conj(us)).  Without uncut points e is
+        // zeroed, so the arithmetic above needs no branch (its transform is skipped instead).
+        const auto pr = buf_rsrc(part + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
+        float n2s = 0.f, n2r = 0.f;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            z[q] = out[q];
+            if (!has_e) e[q] = make_float2(0.f, 0.f);
+            buf_store1(pr, (uint32_t)(t + q * T) * 4u, factor * (z[q].x * z[q].y - e[q].x * e[q].y) * inv_alpha);
+            n2s += z[q].x * z[q].x + e[q].x * e[q].x;
+            n2r += z[q].y * z[q].y + e[q].y * e[q].y;
+        }
+        // keep the two packed wavefields of comparable size (the rounding of each transform is
+        // relative to their sum): rescale the receiver part when the ratio leaves [1/2, 2]
+        n2s = subgroup_sum<T>(n2s, red);
+        n2r = subgroup_sum<T>(n2r, red);
+        if (n2s > 0.f && n2r > 0.f && (n2s > 4.f * n2r || n2r > 4.f * n2s)) {
+            const float beta = sqrtf(n2s / n2r);
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                z[q].y *= beta;
+                e[q].y *= beta;
+            }
+            alpha *= beta;
+            inv_alpha = 1.f / alpha;
+        }
+    }
+}
+
+template <int N, int P>
+__global__ __launch_bounds__(N / P) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_packed(March2DArgs a)
+{
+    adj_body_packed<N, P, false>(a);
+}
+
+template <int N>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_packed_w(March2DArgs a)
+{
+    adj_body_packed<N, N / 64, true>(a);
+}
+
 #define SWEM_SUBGROUP_BOUNDS(P) __launch_bounds__((P) == 16 ? 512 : kMaxBlock)
 
 template <int N, int MODE, int P>
@@ -554,12 +761,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ma
 //   PAIR         (migration) one subgroup per frequency marches source and receiver in turn:
 //                the imaging needs no hand-off and a barrier holds only that subgroup's waves;
 //                a wave (nkx <= 1024) or nkx/16 threads.  Modelling uses the AUTO shape.
+//   PACKED       PAIR's shape; PSPI compat migration with both wavefields in one transform
+//                (the host selects it only where every depth steps both wavefields and the
+//                uncut points of a depth share one vref; other modes run PAIR).
 March2DShape march2d_shape(int nkx, int variant, int adj)
 {
     March2DShape sh{};
     if (nkx < 128 || nkx > 4096 || (nkx & (nkx - 1))) return sh;
     int v = variant;
-    if ((v == SWEM_VARIANT_PAIR || v == SWEM_VARIANT_PAIR8) && !adj) v = SWEM_VARIANT_AUTO;
+    if ((v == SWEM_VARIANT_PAIR || v == SWEM_VARIANT_PAIR8 || v == SWEM_VARIANT_PACKED) && !adj) v = SWEM_VARIANT_AUTO;
     if (v == SWEM_VARIANT_PAIR8 && nkx < 512) v = SWEM_VARIANT_PAIR;
     if (v == SWEM_VARIANT_AUTO) v = adj ? SWEM_VARIANT_PAIR : (nkx <= 512 ? SWEM_VARIANT_WAVE : SWEM_VARIANT_SUBGROUP16);
     if (v == SWEM_VARIANT_WAVE && nkx > 2048) v = SWEM_VARIANT_SUBGROUP16;
@@ -574,6 +784,7 @@ March2DShape march2d_shape(int nkx, int variant, int adj)
         sh.T = nkx / 16, sh.max_block = 512, sh.bufs = 2;
         break;
     case SWEM_VARIANT_PAIR:
+    case SWEM_VARIANT_PACKED:
         sh.T = nkx >= 1024 ? nkx / 16 : 64, sh.per_freq = 1, sh.max_block = sh.T, sh.bufs = nkx >= 1024 ? 2 : 1;
         break;
     case SWEM_VARIANT_PAIR8:
@@ -611,7 +822,13 @@ static int launch_one(const March2DArgs& a, const March2DShape& sh, int F, int a
 {
     const size_t lds = march2d_lds_bytes(N, sh, F);
     dim3 block(sh.T * sh.per_freq * F);
-    if (sh.variant == SWEM_VARIANT_PAIR) {
+    if (sh.variant == SWEM_VARIANT_PACKED && MODE == kModePspiCompat && adj) {
+        if constexpr (N < 1024)
+            return launch_kernel(march2d_adj_packed_w<N>, a, block, lds, st);
+        else
+            return launch_kernel(march2d_adj_packed<N, 16>, a, block, lds, st);
+    }
+    if (sh.variant == SWEM_VARIANT_PAIR || sh.variant == SWEM_VARIANT_PACKED) {
         if constexpr (N < 1024)
             return launch_kernel(march2d_adj_pair_w<N, MODE>, a, block, lds, st);
         else

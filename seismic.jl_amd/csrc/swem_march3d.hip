@@ -132,28 +132,61 @@ __global__ __launch_bounds__(256) void march3d_init(March3DArgs a)
     for (int q = 0; q < P; ++q) dst[t + 64 * q] = v[q];
 }
 
+// kz phase of the KZ64 path with the column's part hoisted: c0 = w^2 po^2 - ky^2 (FP64, once per
+// column), s = c0 - kxi^2 dkx^2 per point (kxi^2 exact in FP64).
+__device__ __forceinline__ float2 kz_phase64(const March3DArgs& a, double c0, int ikx)
+{
+    const int kxi = ikx <= a.nkx / 2 ? ikx : a.nkx - ikx;
+    const float s = (float)fma(-(double)(kxi * kxi), a.dkx2, c0);
+    const float dzf = (float)a.dzd;
+    if (s >= 0.f) {
+        float r = sqrtf(s) * dzf * (float)(1.0 / kTwoPi);
+        r -= rintf(r);
+        return make_float2(__builtin_amdgcn_cosf(r) * a.inv_nk, __builtin_amdgcn_sinf(r) * a.inv_nk);
+    }
+    return make_float2(expf(-0.2f * sqrtf(-s) * fabsf(dzf)) * a.inv_nk, 0.f);
+}
+
 // x-pass.  Workgroup: C waves, wave w owns column ky0 + w of every stepped wavefield (NWF = 2:
 // source and receiver, which share the kz phase; NWF = 1: the one wmask names).  The tile of C
 // consecutive ky columns is staged through LDS so the global reads and writes are row segments
-// of C complex.  conj_src / conj_rec: use conj(L) (-dz).
-template <int NX, int NWF>
-__global__ __launch_bounds__(NX >= 1024 ? 512 : 1024) void march3d_cols(March3DArgs a, int C)
+// of C complex.  conj_src / conj_rec: use conj(L) (-dz).  Workgroups are renumbered so that each
+// XCD (consecutive workgroup ids go round robin over the 8 XCDs) takes a contiguous run of tiles:
+// neighbouring tiles share 128-byte lines, which then meet in one L2.
+template <int NX, int NWF, int C>
+__global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 1024 ? 2 : 4))) void march3d_cols(March3DArgs a)
 {
     constexpr int P = NX / 64;
     constexpr int L = LdsLen<NX>::value;
     extern __shared__ float2 smem[];
     const int tiles = a.nky / C;
-    const int item = blockIdx.x / tiles;
-    const int ky0 = (blockIdx.x - item * tiles) * C;
+    const int nblk = a.nitem * tiles;
+    const int full = nblk & ~7;
+    int b = blockIdx.x;
+    if (b < full) b = (b & 7) * (full >> 3) + (b >> 3);
+    const int item = b / tiles;
+    const int ky0 = (b - item * tiles) * C;
     const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
-    const int nthr = blockDim.x;
     const int wf1 = (a.wmask & 1) ? 0 : 1;  // the wavefield of an NWF = 1 launch
     auto wf_of = [&](int k) { return NWF == 2 ? k : wf1; };
-    // stage: buffer (c * NWF + k) holds column ky0 + c of the k-th stepped wavefield
-    for (int i = threadIdx.x; i < a.nmx * C * NWF; i += nthr) {
-        const int c = i % C, r = i / C;
-        const int k = r % NWF, x = r / NWF;
-        smem[(c * NWF + k) * L + lds_pad(x)] = state(a, item, wf_of(k))[(size_t)x * a.nky + ky0 + c];
+    // stage: buffer (c * NWF + k) holds column ky0 + c of the k-th stepped wavefield.  The loads of
+    // one wavefield are all issued before its LDS writes (rows x0 + 64 j, j < P cover x < nmx).
+    const int c = threadIdx.x % C, x0 = threadIdx.x / C;
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) {
+        const float2* src = state(a, item, wf_of(k)) + ky0 + c;
+        float2* dst = smem + (c * NWF + k) * L;
+        float2 st[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int x = x0 + 64 * j;
+            st[j] = x < a.nmx ? src[(size_t)x * a.nky] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int x = x0 + 64 * j;
+            if (x < a.nmx) dst[lds_pad(x)] = st[j];
+        }
     }
     __syncthreads();
     float2 v[NWF][P];
@@ -172,14 +205,17 @@ __global__ __launch_bounds__(NX >= 1024 ? 512 : 1024) void march3d_cols(March3DA
     const double wd = (double)iw * a.dw;
     const double p2 = a.p2[a.iz];
     const int iky = ky0 + w;
+    const int kyi = iky <= a.nky / 2 ? iky : a.nky - iky;
+    const double ky = kyi * a.dky;
+    const double c0 = wd * wd * p2 - ky * ky;
+    bool cj[NWF];
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) cj[k] = wf_of(k) == 0 ? a.conj_src : a.conj_rec;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-        const float2 ph = kz_phase(a, wd, p2, t + 64 * q, iky);
+        const float2 ph = a.prec == SWEM_PREC_KZ64 ? kz_phase64(a, c0, t + 64 * q) : kz_phase(a, wd, p2, t + 64 * q, iky);
 #pragma unroll
-        for (int k = 0; k < NWF; ++k) {
-            const bool cj = wf_of(k) == 0 ? a.conj_src : a.conj_rec;
-            v[k][q] = cj ? cmulc(v[k][q], ph) : cmul(v[k][q], ph);
-        }
+        for (int k = 0; k < NWF; ++k) v[k][q] = cmul(v[k][q], make_float2(ph.x, cj[k] ? -ph.y : ph.y));
     }
 #pragma unroll
     for (int k = 0; k < NWF; ++k) {
@@ -194,10 +230,18 @@ __global__ __launch_bounds__(NX >= 1024 ? 512 : 1024) void march3d_cols(March3DA
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < a.nmx * C * NWF; i += nthr) {
-        const int c = i % C, r = i / C;
-        const int k = r % NWF, x = r / NWF;
-        state(a, item, wf_of(k))[(size_t)x * a.nky + ky0 + c] = smem[(c * NWF + k) * L + lds_pad(x)];
+    // recomputed rather than kept live through the transforms (register pressure)
+    int c1 = c, x1 = x0;
+    asm volatile("" : "+v"(c1), "+v"(x1));
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) {
+        float2* dst = state(a, item, wf_of(k)) + ky0 + c1;
+        const float2* src = smem + (c1 * NWF + k) * L;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int x = x1 + 64 * j;
+            if (x < a.nmx) dst[(size_t)x * a.nky] = src[lds_pad(x)];
+        }
     }
 }
 
@@ -223,19 +267,22 @@ __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
     float2 vs[P], vr[P];
 #pragma unroll
     for (int q = 0; q < P; ++q) vs[q] = vr[q] = make_float2(0.f, 0.f);
+    // both rows are loaded before the first transform (their latencies overlap)
     if (need_s) {
         const float2* src = state(a, item, 0) + (size_t)x * a.nky;
 #pragma unroll
         for (int q = 0; q < P; ++q) vs[q] = src[t + 64 * q];
-        wfft<NY, +1>(vs, buf, a.twy, t);
-        wave_sync_lds();
     }
     if (need_r) {
         const float2* src = state(a, item, 1) + (size_t)x * a.nky;
 #pragma unroll
         for (int q = 0; q < P; ++q) vr[q] = src[t + 64 * q];
-        wfft<NY, +1>(vr, buf, a.twy, t);
     }
+    if (need_s) {
+        wfft<NY, +1>(vs, buf, a.twy, t);
+        wave_sync_lds();
+    }
+    if (need_r) wfft<NY, +1>(vr, buf, a.twy, t);
     float cc[P];
     const float factor = iw == 0 ? 1.f : 2.f;
     const float tx = a.taper_x[x];
@@ -359,12 +406,12 @@ bool march3d_supported(int nkx, int nky)
 // wavefields or 16 of one) below nkx = 1024, half that at 1024.
 static int march3d_cols_tile(int nwf, int nkx) { return (nkx >= 1024 ? 8 : 16) / nwf; }
 
-template <typename K>
-static int launch3(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, const March3DArgs& a, int extra)
+template <typename K, typename... X>
+static int launch3(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, const March3DArgs& a, X... extra)
 {
     if (lds > 48 * 1024)
         SWEM_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kernel, grid, block, lds, st, a, extra);
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, a, extra...);
     SWEM_HIP(hipGetLastError());
     return SWEM_OK;
 }
@@ -384,8 +431,9 @@ static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStrea
         const int C = march3d_cols_tile(nwf, N);
         const size_t lds = (size_t)C * nwf * L * sizeof(float2);
         const dim3 grid(a.nitem * (a.nky / C)), block(64 * C);
-        return nwf == 2 ? launch3(march3d_cols<N, 2>, grid, block, lds, st, a, C)
-                        : launch3(march3d_cols<N, 1>, grid, block, lds, st, a, C);
+        constexpr int C2 = N >= 1024 ? 4 : 8, C1 = 2 * C2;
+        return nwf == 2 ? launch3(march3d_cols<N, 2, C2>, grid, block, lds, st, a)
+                        : launch3(march3d_cols<N, 1, C1>, grid, block, lds, st, a);
     }
     const int ngroups = (a.nitem + F - 1) / F;
     if (which == 2) {

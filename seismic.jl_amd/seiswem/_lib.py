@@ -16,7 +16,7 @@ SHOTWEM_PATH = os.path.join(LIB_DIR, "shotwem.so")
 SWEM_OK, SWEM_EINVAL, SWEM_ENOMEM, SWEM_EDEVICE, SWEM_EIO, SWEM_EUNSUPPORTED, SWEM_EBUSY = range(7)
 PSPI_COMPAT, PSPI_LINEAR = 0, 1
 PREC_KZ64, PREC_FP32 = 0, 1
-VARIANT_AUTO, VARIANT_SUBGROUP, VARIANT_WAVE, VARIANT_SUBGROUP16, VARIANT_PAIR, VARIANT_PAIR8 = 0, 1, 2, 3, 4, 5
+VARIANT_AUTO, VARIANT_SUBGROUP, VARIANT_WAVE, VARIANT_SUBGROUP16, VARIANT_PAIR, VARIANT_PAIR8, VARIANT_PACKED = range(7)
 
 # Every symbol declared in include/seiswem.h.
 EXPORTS = (

@@ -34,7 +34,7 @@ def oparams(g: seiswem.WemGeometry, pspi_mode: int = 0) -> O.WemParams:
 
 
 VARIANTS = [seiswem.VARIANT_SUBGROUP, seiswem.VARIANT_WAVE, seiswem.VARIANT_SUBGROUP16, seiswem.VARIANT_PAIR,
-            seiswem.VARIANT_PAIR8]
+            seiswem.VARIANT_PAIR8, seiswem.VARIANT_PACKED]
 CASES = [(False, 0), (True, 0), (True, 1)]
 
 
@@ -131,11 +131,14 @@ def test_shots_stack_equals_sum_and_each_matches_alone(gpu):
     assert rel_l2(stack, each.astype(np.float64).sum(0)) <= 1e-6
 
 
-def test_linearity_full_config2_shape(gpu):
+@pytest.mark.parametrize("variant,tol", [(seiswem.VARIANT_PAIR, 1e-5), (seiswem.VARIANT_PACKED, 2e-5)])
+def test_linearity_full_config2_shape(gpu, variant, tol):
     """Size-independent check at the full config-2 shape (nkx=2048, nz=500, 641 frequencies):
-    migration is linear, image(a d1 + d2) = a image(d1) + image(d2)."""
+    migration is linear, image(a d1 + d2) = a image(d1) + image(d2), to the FP32 rounding of a
+    500-step march.  The packed kernel rounds both wavefields in one transform (rounding relative
+    to their combined size): its bound is twice PAIR's (measured 1.0e-5)."""
     g, vel, refl, wav, sx = synth.config2(nshot=1)
-    plan = seiswem.Plan(g, vel, wav)
+    plan = seiswem.Plan(g, vel, wav, variant=variant)
     rng = np.random.default_rng(11)
     d1 = rng.standard_normal((1, g.nm, g.nt)).astype(np.float32)
     d2 = rng.standard_normal((1, g.nm, g.nt)).astype(np.float32)
@@ -144,8 +147,8 @@ def test_linearity_full_config2_shape(gpu):
     m2 = plan.migrate(seiswem.DeviceArray(d2.shape, d2), sx[:1]).download()
     m12 = plan.migrate(seiswem.DeviceArray(d1.shape, a * d1 + d2), sx[:1]).download()
     err = rel_l2(m12, a * m1.astype(np.float64) + m2)
-    print(f"config2 linearity rel-L2 {err:.3e}")
-    assert err <= 1e-5
+    print(f"config2 linearity (variant {variant}) rel-L2 {err:.3e}")
+    assert err <= tol
 
 
 @pytest.mark.parametrize("pspi,mode", CASES)
@@ -175,3 +178,30 @@ def test_frequency_chunks_match_single_chunk(gpu, pspi, mode):
     r1 = whole.model(md, sx).download()
     r2 = chunked.model(md, sx).download()
     assert rel_l2(r2, r1) <= 1e-6
+
+
+@pytest.mark.parametrize("case", ["plateau", "vz", "source_below_surface", "nkx2048"])
+def test_packed_compat_migration(gpu, case):
+    """The packed PSPI compat kernel (both real wavefields in one transform) against the oracle
+    and the PAIR kernel: a velocity clipped to a plateau at its maximum (many uncut points per
+    depth), a v(z) model (vmax <= vmin + 10 at every depth: every point uncut), a source below
+    the surface (not every depth steps the source: the host falls back to PAIR) and nkx = 2048
+    (the 128-thread subgroup kernel)."""
+    nx, nz, nt = (1024, 12, 128) if case == "nkx2048" else (256, 30, 256)
+    g, vel, wav, rng = small_case(True, seed=5, nx=nx, nz=nz, nt=nt, lateral=case != "vz")
+    if case == "plateau":
+        vel = np.minimum(vel, np.float32(np.percentile(vel, 90)))
+    if case == "source_below_surface":
+        g.sz = 55.0
+    sx = 640.0
+    d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
+    m_ref = O.wem(oparams(g, 0), vel, wav, sx, d=d, adj=True)
+    m_packed = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=[sx], pspi_mode=0,
+                                      variant=seiswem.VARIANT_PACKED)
+    m_pair = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=[sx], pspi_mode=0,
+                                    variant=seiswem.VARIANT_PAIR)
+    m_auto = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=[sx], pspi_mode=0)
+    print(f"packed {case}: rel-L2 vs oracle {rel_l2(m_packed, m_ref):.3e}, vs PAIR {rel_l2(m_packed, m_pair):.3e}")
+    assert rel_l2(m_packed, m_ref) <= TOL
+    assert rel_l2(m_packed, m_pair) <= 1e-5
+    assert np.array_equal(m_auto, m_packed)
