@@ -64,7 +64,7 @@ COPY_BYTES = 1 << 30
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "march2d_adj_pspi_nkx2048_64shots_traffic.json")
 
 
-def traffic_per_launch(nshot):
+def traffic_per_launch(nshot, kernel):
     """HBM bytes per march launch from the committed FETCH_SIZE / WRITE_SIZE passes (KiB per
     dispatch), each scaled by the same counter's reading for a 1 GiB device copy; None when no
     pass of this workload is on file."""
@@ -73,7 +73,7 @@ def traffic_per_launch(nshot):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("nshot") != nshot:
+    if t.get("nshot") != nshot or kernel not in t.get("kernel", ""):
         return None
     fetch = t["march_fetch_kib"] * (COPY_BYTES / 1024) / t["copy_fetch_kib"]
     write = t["march_write_kib"] * (COPY_BYTES / 1024) / t["copy_write_kib"]
@@ -330,10 +330,14 @@ def main():
     avg_launch_s = march_ms / launches / 1e3
     flops_per_launch = flops / launches
     achieved = flops_per_launch / avg_launch_s / 1e12
-    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_per_launch(nshot),
-                "kernel": "march2d_adj (PSPI, nkx=2048)", "avg_launch_ms": march_ms / launches,
-                "flops_per_launch": flops_per_launch}
+    packed = args.variant in (0, 6)  # automatic selection takes the packed kernel for PSPI compat
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_per_launch(nshot, "march2d_adj_packed" if packed else "march2d_adj_pair"),
+                "kernel": "march2d_adj_packed<2048,16>" if packed else "march2d_adj (PSPI, nkx=2048)",
+                "compute_unit": "FP32 vector ALU (no stage is a matrix product); peak = MI355X FP32 157.3 TFLOP/s",
+                "flops_basis": "reference operation count: 2 (1 + nref) transforms per depth and frequency"
+                               + ("; the packed kernel executes 3 + nref" if packed else ""),
+                "avg_launch_ms": march_ms / launches, "flops_per_launch": flops_per_launch}
     out = {
         "metric": "migrated shots/sec (2D & 3D) at <=1e-4 rel-L2 vs FP64; p50/p99 request latency",
         "value": value, "unit": "shots/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
