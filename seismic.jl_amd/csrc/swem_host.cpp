@@ -194,7 +194,7 @@ static int build_tables(swem_plan* p, const float* wav)
         SWEM_TRY(p->p2f.upload(p2f.data(), p2f.size()));
         std::vector<uint32_t> used(nz, 1u);
         SWEM_TRY(p->used.upload(used.data(), used.size()));
-        std::vector<int> cref(nz, -1);
+        std::vector<int> cref((size_t)nz * kCrefStride, 0);
         SWEM_TRY(p->cref.upload(cref.data(), cref.size()));
         p->packed_ok = false;
     } else {
@@ -240,16 +240,22 @@ static int build_tables(swem_plan* p, const float* wav)
         SWEM_TRY(p->used.upload(used.data(), used.size()));
         // Packed compat migration: every depth steps both wavefields (z >= sz and z >= gz,
         // wem.c:235-241) and the uncut points of a depth share one vref.
-        std::vector<int> cref(nz, -1);
+        // Per depth: vref of the uncut points (-1: none), their count and the first kUncutSparse
+        // positions.
+        std::vector<int> cref((size_t)nz * kCrefStride, 0);
         bool packed = p->opt.pspi_mode == SWEM_PSPI_COMPAT && !p->three_d;
         for (int iz = 0; iz < nz; iz++) {
             const float z = g.oz + g.dz * (float)iz;
             if (!(z >= g.sz && z >= g.gz)) packed = false;
+            int* row = cref.data() + (size_t)iz * kCrefStride;
+            row[0] = -1;
             for (int ix = 0; ix < nm; ix++) {
                 const size_t i = (size_t)ix * nz + iz;
                 if (cut[i]) continue;
-                if (cref[iz] < 0) cref[iz] = i1[i];
-                else if (cref[iz] != i1[i]) packed = false;
+                if (row[0] < 0) row[0] = i1[i];
+                else if (row[0] != i1[i]) packed = false;
+                if (row[1] < kUncutSparse) row[2 + row[1]] = ix;
+                row[1]++;
             }
         }
         SWEM_TRY(p->cref.upload(cref.data(), cref.size()));

@@ -54,6 +54,13 @@ int launch_phase_tables(const PhaseTableArgs& a, bool ss, hipStream_t st);
 // [f0, f0 + nfc) of the band (indices relative to ifmin) for nshot shots; the block index is
 // decoded XCD-aware into (frequency group g, shot) so the shots that share a frequency run on
 // the same XCD and share its L2 copy of the phase tables.
+// Uncut-point table of the packed PSPI compat kernel: depths with at most kUncutSparse uncut points
+// use direct sums over them instead of two transforms.
+constexpr int kUncutSparse = 4;
+constexpr int kCrefStride = 2 + kUncutSparse;
+// Complex LDS scratch after the transform buffers of a packed-kernel workgroup.
+constexpr int kPackedScratch = 32;
+
 struct March2DArgs {
     int nmx, nz, nf, nref, ngroups, ifmin, nshot;
     int f0, nfc, nkh;
@@ -70,7 +77,8 @@ struct March2DArgs {
     const uint8_t* sel2;    // [nz][nkx] iref2 (PSPI linear)
     const float* w2;        // [nz][nkx] weight of iref2 (PSPI linear)
     const uint32_t* used;   // [nz] bit ir set when some x selects vref ir at iz
-    const int* cref;        // [nz] vref of the uncut points of depth iz (PSPI compat), -1: none
+    const int* cref;        // [nz][kCrefStride] uncut points of depth iz (PSPI compat): vref (-1: none),
+                            // count, the first kUncutSparse positions x
     float* partial;         // [nshot][ngroups][nz][nmx] migration partial images
     const float* mT;        // [nz][nmx] image (modelling input, depth-major)
     float2* smig;           // [nshot][nfc][nz][nmx] source wavefield scratch (modelling)

@@ -561,8 +561,7 @@ __device__ __forceinline__ void adj_body_pair(const March2DArgs& a)
 // gives their new e: 2 + nref + 1 transforms per depth instead of 2 (1 + nref).  The image is
 // factor Re(r conj s) = factor (Re z Im z - Re e Im e) / a.
 
-// Sum over the T threads of a subgroup in a fixed order (red: T/64 floats of LDS that no
-// transform touches: the last four complex of a buffer lie beyond every padded index).
+// Sum over the T threads of a subgroup in a fixed order (red: T/64 floats of LDS scratch).
 template <int T>
 __device__ __forceinline__ float subgroup_sum(float v, float* red)
 {
@@ -582,10 +581,25 @@ __device__ __forceinline__ float subgroup_sum(float v, float* red)
 }
 
 
+template <int T>
+__device__ __forceinline__ void subgroup_sync()
+{
+    if constexpr (T == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 template <int N, int P, bool WAVE>
 __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
 {
     constexpr int T = N / P;
+    constexpr int K = kUncutSparse;
+    constexpr int NW = T / 64;  // waves of the subgroup
+    static_assert(4 + K + NW * K <= kPackedScratch, "packed scratch");
     extern __shared__ float2 smem[];
     int g, shot;
     decode_block(a.ngroups, a.nshot, g, shot);
@@ -602,6 +616,12 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
     int pp = 0;
     const int igx = a.igx[shot];
     const float2 ss = active ? a.spec_s[fl] : make_float2(0.f, 0.f);
+    // scratch after the transform buffers: reduction slots, e at the previous depth's uncut
+    // points, per-wave partial sums of the new e at this depth's uncut points
+    float2* scr = smem + (WAVE ? 1 : 2) * LdsLen<N>::value;
+    float* red = reinterpret_cast<float*>(scr);
+    float2* scr_e = scr + 4;
+    float2* scr_v = scr + 4 + K;
     float2 z[P], e[P];
     float nr = 0.f;
 #pragma unroll
@@ -610,8 +630,6 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
         e[q] = (active && x < a.nmx) ? a.spec_r[((size_t)shot * a.nmx + x) * a.nf + fl] : make_float2(0.f, 0.f);
         nr += e[q].x * e[q].x + e[q].y * e[q].y;
     }
-    float* red = reinterpret_cast<float*>(smem + LdsLen<N>::value - 4);
-    static_assert(T / 64 <= 8, "reduction slots");
     nr = subgroup_sum<T>(nr, red);
     const float ns = sqrtf(ss.x * ss.x + ss.y * ss.y);
     float alpha = (nr > 0.f && ns > 0.f) ? ns / sqrtf(nr) : 1.f;
@@ -624,12 +642,44 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
         e[q] = make_float2(us.y, -alpha * e[q].y);
     }
     float* part = a.partial + ((size_t)shot * a.ngroups + g) * (size_t)a.nz * a.nmx;
-    bool has_e = true;  // e may be nonzero (initial wavefields are complex); false: e == 0
+    // e before the step: dense (the initial imaginary parts), sparse (nonzero only at the
+    // listed uncut points of the previous depth: prev_cnt <= K) or zero
+    int prev_cnt = -1;  // -1: dense
+    const int* prev_row = a.cref;
     for (int iz = 0; iz < a.nz; ++iz) {
         c.iz = iz;
-        const int cr = a.cref[iz];
+        const int* crow = a.cref + (size_t)iz * kCrefStride;
+        const int cr = crow[0], cnt = crow[1];
         xfft<N, P, -1, WAVE>(z, smem, a.tw, t, pp);
-        if (has_e) xfft<N, P, -1, WAVE>(e, smem, a.tw, t, pp);
+        if (prev_cnt < 0 || prev_cnt > K) {
+            xfft<N, P, -1, WAVE>(e, smem, a.tw, t, pp);
+        } else if (prev_cnt > 0) {
+            // E(k) = sum_j e(x_j) w^(k x_j), w = exp(-2 pi i / N): the owners publish e(x_j)
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (j >= prev_cnt) break;
+                const int xj = prev_row[2 + j];
+#pragma unroll
+                for (int q = 0; q < P; ++q)
+                    if (t + q * T == xj) scr_e[j] = e[q];
+            }
+            subgroup_sync<T>();
+#pragma unroll
+            for (int q = 0; q < P; ++q) e[q] = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (j >= prev_cnt) break;
+                const int xj = prev_row[2 + j];
+                const float2 ev = scr_e[j];
+                float2 w = a.tw[(t * xj) & (N - 1)];
+                const float2 st = a.tw[(T * xj) & (N - 1)];
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    e[q] = cadd(e[q], cmul(ev, w));
+                    w = cmul(w, st);
+                }
+            }
+        }  // prev_cnt == 0: e == 0, E == 0
         // selection bytes packed four to a register: (sel(q) = s1w[q / 4] >> 8 (q % 4)) & 0xff
         uint32_t s1w[(P + 3) / 4];
         float2 out[P];
@@ -657,7 +707,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
             for (int q = 0; q < P; ++q)
                 if ((int)(s1(q) & 0x7fu) == ir) out[q] = w[q];
         }
-        if (cr >= 0) {
+        if (cnt > 0) {
             // imaginary parts at the uncut points of this depth (all select vref cr):
             // e_new = IFFT(Re L E - Im L Z), the conjugate of IFFT(V) by L(-k) = L(k)
             const float2* row = krow(a, c, cr);
@@ -668,22 +718,69 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
                 const float2 L = lv[q];
                 e[q] = make_float2(L.x * e[q].x - L.y * z[q].x, L.x * e[q].y - L.y * z[q].y);
             }
-            xfft<N, P, +1, WAVE>(e, smem, a.tw, t, pp);
+            if (cnt > K) {
+                xfft<N, P, +1, WAVE>(e, smem, a.tw, t, pp);
 #pragma unroll
-            for (int q = 0; q < P; ++q) {
-                const int x = t + q * T;
-                if (!(x < a.nmx && !(s1(q) >> 7))) e[q] = make_float2(0.f, 0.f);
+                for (int q = 0; q < P; ++q) {
+                    const int x = t + q * T;
+                    if (!(x < a.nmx && !(s1(q) >> 7))) e[q] = make_float2(0.f, 0.f);
+                }
+            } else {
+                // e_new(x_j) = sum_k U(k) conj(w^(k x_j)) at the few uncut points: partial sums
+                // per thread, then over the subgroup in a fixed order
+                float2 acc[K];
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    acc[j] = make_float2(0.f, 0.f);
+                    if (j >= cnt) continue;
+                    const int xj = crow[2 + j];
+                    float2 w = a.tw[(t * xj) & (N - 1)];
+                    const float2 st = a.tw[(T * xj) & (N - 1)];
+#pragma unroll
+                    for (int q = 0; q < P; ++q) {
+                        acc[j] = cadd(acc[j], cmulc(e[q], w));
+                        w = cmul(w, st);
+                    }
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) {
+                        acc[j].x += __shfl_xor(acc[j].x, o, 64);
+                        acc[j].y += __shfl_xor(acc[j].y, o, 64);
+                    }
+                    if constexpr (NW > 1)
+                        if ((t & 63) == 0) scr_v[(t >> 6) * K + j] = acc[j];
+                }
+                if constexpr (NW > 1) {
+                    subgroup_sync<T>();
+#pragma unroll
+                    for (int j = 0; j < K; ++j) {
+                        if (j >= cnt) break;
+                        acc[j] = scr_v[j];
+#pragma unroll
+                        for (int wv = 1; wv < NW; ++wv) acc[j] = cadd(acc[j], scr_v[wv * K + j]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < P; ++q) e[q] = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    if (j >= cnt) break;
+                    const int xj = crow[2 + j];
+#pragma unroll
+                    for (int q = 0; q < P; ++q)
+                        if (t + q * T == xj) e[q] = acc[j];
+                }
             }
         }
-        has_e = cr >= 0;
+        prev_cnt = cnt;
+        prev_row = crow;
         // imaging condition (wem.c:243-247): factor Re(ur conj(us)).  Without uncut points e is
-        // zeroed, so the arithmetic above needs no branch (its transform is skipped instead).
+        // zeroed, so the arithmetic above needs no branch.
         const auto pr = buf_rsrc(part + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
         float n2s = 0.f, n2r = 0.f;
 #pragma unroll
         for (int q = 0; q < P; ++q) {
             z[q] = out[q];
-            if (!has_e) e[q] = make_float2(0.f, 0.f);
+            if (cnt == 0) e[q] = make_float2(0.f, 0.f);
             buf_store1(pr, (uint32_t)(t + q * T) * 4u, factor * (z[q].x * z[q].y - e[q].x * e[q].y) * inv_alpha);
             n2s += z[q].x * z[q].x + e[q].x * e[q].x;
             n2r += z[q].y * z[q].y + e[q].y * e[q].y;
@@ -823,10 +920,11 @@ static int launch_one(const March2DArgs& a, const March2DShape& sh, int F, int a
     const size_t lds = march2d_lds_bytes(N, sh, F);
     dim3 block(sh.T * sh.per_freq * F);
     if (sh.variant == SWEM_VARIANT_PACKED && MODE == kModePspiCompat && adj) {
+        const size_t lds_p = lds + kPackedScratch * sizeof(float2);
         if constexpr (N < 1024)
-            return launch_kernel(march2d_adj_packed_w<N>, a, block, lds, st);
+            return launch_kernel(march2d_adj_packed_w<N>, a, block, lds_p, st);
         else
-            return launch_kernel(march2d_adj_packed<N, 16>, a, block, lds, st);
+            return launch_kernel(march2d_adj_packed<N, 16>, a, block, lds_p, st);
     }
     if (sh.variant == SWEM_VARIANT_PAIR || sh.variant == SWEM_VARIANT_PACKED) {
         if constexpr (N < 1024)

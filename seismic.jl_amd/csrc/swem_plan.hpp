@@ -101,7 +101,7 @@ struct swem_plan {
     swem::DevBuf<double> p2, pd;
     swem::DevBuf<uint8_t> sel, sel2;
     swem::DevBuf<uint32_t> used;
-    swem::DevBuf<int> cref;  // [nz] vref of the uncut points of a depth (PSPI compat), -1: none
+    swem::DevBuf<int> cref;  // [nz][kCrefStride] uncut points of a depth (PSPI compat)
     bool packed_ok = false;  // PSPI compat migration may run the packed kernel
     // work
     swem::DevBuf<float2> spec_r, smig, dgw, W;

@@ -180,17 +180,22 @@ def test_frequency_chunks_match_single_chunk(gpu, pspi, mode):
     assert rel_l2(r2, r1) <= 1e-6
 
 
-@pytest.mark.parametrize("case", ["plateau", "vz", "source_below_surface", "nkx2048"])
+@pytest.mark.parametrize("case", ["plateau", "vz", "source_below_surface", "nkx2048", "three_uncut", "five_uncut"])
 def test_packed_compat_migration(gpu, case):
     """The packed PSPI compat kernel (both real wavefields in one transform) against the oracle
     and the PAIR kernel: a velocity clipped to a plateau at its maximum (many uncut points per
     depth), a v(z) model (vmax <= vmin + 10 at every depth: every point uncut), a source below
-    the surface (not every depth steps the source: the host falls back to PAIR) and nkx = 2048
-    (the 128-thread subgroup kernel)."""
+    the surface (not every depth steps the source: the host falls back to PAIR), nkx = 2048
+    (the 128-thread subgroup kernel) and exactly 3 / 5 uncut points per depth (the direct sums
+    for at most 4, the transforms above)."""
     nx, nz, nt = (1024, 12, 128) if case == "nkx2048" else (256, 30, 256)
     g, vel, wav, rng = small_case(True, seed=5, nx=nx, nz=nz, nt=nt, lateral=case != "vz")
     if case == "plateau":
         vel = np.minimum(vel, np.float32(np.percentile(vel, 90)))
+    if case in ("three_uncut", "five_uncut"):  # the depth maximum at 3 / 5 points: sparse / dense paths
+        top = vel.max(axis=0) + np.float32(100.0)
+        for x in rng.choice(nx, size=3 if case == "three_uncut" else 5, replace=False):
+            vel[x] = top
     if case == "source_below_surface":
         g.sz = 55.0
     sx = 640.0

@@ -68,3 +68,36 @@ def test_packed_compat_steps_equal_the_reference_steps(seed):
         image = (r * np.conj(s)).real
         np.testing.assert_allclose((z.real * z.imag - e.real * e.imag) / alpha, image,
                                    atol=1e-9 * np.abs(image).max())
+
+
+def test_sparse_sums_match_transforms_in_the_strided_mapping():
+    """Depths with few uncut points replace FFT(e) and the inverse transform of the new e by
+    direct sums: E(k) = sum_j e_j w^(k x_j) and e_new(x_j) = sum_k U(k) conj(w^(k x_j)),
+    w = exp(-2 pi i / N), evaluated per thread t over k = t + q T with the recurrence
+    w^(k x) = w^(t x) (w^(T x))^q from two twiddle-table entries."""
+    rng = np.random.default_rng(9)
+    n, P = 256, 16
+    T = n // P
+    tw = np.exp(-2j * np.pi * np.arange(n) / n)
+    xs = np.array([3, 77, 200])
+    ev = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+    e = np.zeros(n, complex)
+    e[xs] = ev
+    E = np.zeros(n, complex)
+    for t in range(T):
+        for j, x in enumerate(xs):
+            w, st = tw[(t * x) % n], tw[(T * x) % n]
+            for q in range(P):
+                E[t + q * T] += ev[j] * w
+                w *= st
+    np.testing.assert_allclose(E, np.fft.fft(e), atol=1e-12)
+    U = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.ifft(U) * n
+    for x in xs:
+        acc = 0j
+        for t in range(T):
+            w, st = tw[(t * x) % n], tw[(T * x) % n]
+            for q in range(P):
+                acc += U[t + q * T] * np.conj(w)
+                w *= st
+        assert abs(acc - ref[x]) < 1e-9 * np.abs(ref).max()
