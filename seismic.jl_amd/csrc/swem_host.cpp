@@ -216,13 +216,15 @@ static int build_tables(swem_plan* p, const float* wav)
                 p2f[(size_t)iz * nref + ir] = 1 / (vf * vf);
             }
         }
-        // [nz][nkx]: rows padded to the transform length (0x7f: no vref at x >= nmx)
-        std::vector<uint8_t> sel((size_t)nz * nkx, 0x7f), sel2((size_t)nz * nkx, 0x7f);
-        std::vector<float> w2((size_t)nz * nkx, 0.f);
+        // 2-D: [nz][nkx], rows padded to the transform length (0x7f: no vref at x >= nmx);
+        // 3-D: [nz][nm], trace ix = imx * nmy + imy
+        const size_t srow = p->three_d ? (size_t)nm : (size_t)nkx;
+        std::vector<uint8_t> sel((size_t)nz * srow, 0x7f), sel2((size_t)nz * srow, 0x7f);
+        std::vector<float> w2((size_t)nz * srow, 0.f);
         std::vector<uint32_t> used(nz, 0u);
         for (int ix = 0; ix < nm; ix++)
             for (int iz = 0; iz < nz; iz++) {
-                const size_t i = (size_t)ix * nz + iz, o = (size_t)iz * nkx + ix;
+                const size_t i = (size_t)ix * nz + iz, o = (size_t)iz * srow + ix;
                 sel[o] = (uint8_t)(i1[i] | (cut[i] ? 0x80 : 0));
                 sel2[o] = (uint8_t)i2[i];
                 used[iz] |= 1u << i1[i];
@@ -521,7 +523,17 @@ static March3DArgs base3(const swem_plan* p)
     a.twx = p->tw.p;
     a.twy = p->twy.p;
     a.spec_s = p->spec_s.p;
+    a.nref = p->g.pspi ? p->g.nref : 1;
+    a.sel = p->sel.p;
+    a.used = p->used.p;
+    a.R = p->R.p;
     return a;
+}
+
+// Bytes of the per-vref x-transformed slices of one 3-D PSPI frequency (both wavefields).
+static size_t pspi3_slice_bytes(const swem_plan* p)
+{
+    return p->g.pspi ? 2 * (size_t)p->g.nref * p->g.nmx * p->s.nky * sizeof(float2) : 0;
 }
 
 static bool stepped_at(const swem_plan* p, int iz, float level)
@@ -552,11 +564,13 @@ static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, con
 {
     const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
     const int F = p->F3;
-    const size_t per_item = 2 * (size_t)p->g.nmx * p->s.nky * sizeof(float2) + nm * sizeof(float) / F + 1;
+    const size_t per_item =
+        2 * (size_t)p->g.nmx * p->s.nky * sizeof(float2) + nm * sizeof(float) / F + 1 + pspi3_slice_bytes(p);
     const int nfc = chunk3(p, per_item, F);
     const size_t gmax = (nfc + F - 1) / F;
     SWEM_TRY(p->W.alloc((size_t)nfc * 2 * p->g.nmx * p->s.nky));
     SWEM_TRY(p->partial.alloc(gmax * nm));
+    if (p->g.pspi) SWEM_TRY(p->R.alloc((size_t)nfc * pspi3_slice_bytes(p) / sizeof(float2)));
     SWEM_TRY(p->spec_r.alloc(nm * nf));
     SWEM_TRY(set_igx(p, sx, sy, nshot, st));
     p->nev = 0;
@@ -577,6 +591,7 @@ static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, con
             a.spec_r = p->spec_r.p;
             a.W = p->W.p;
             a.partial = p->partial.p;
+            a.R = p->R.p;
             a.conj_src = 1;
             a.conj_rec = 0;
             float* out = per_shot ? m_dev + (size_t)s * nm * nz : m_dev;
@@ -589,8 +604,14 @@ static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, con
                 a.next_s = stepped_at(p, iz + 1, p->g.sz);
                 a.next_r = stepped_at(p, iz + 1, p->g.gz);
                 a.wmask = (a.ds ? 1 : 0) | (a.dr ? 2 : 0);
-                if (a.wmask) SWEM_TRY(launch_march3d(1, a, F, 0, st));
-                SWEM_TRY(launch_march3d(2, a, F, 0, st));
+                a.iz_sel = iz;
+                if (p->g.pspi) {
+                    if (a.wmask) SWEM_TRY(launch_march3d(4, a, F, 0, st));
+                    SWEM_TRY(launch_march3d(5, a, F, 0, st));
+                } else {
+                    if (a.wmask) SWEM_TRY(launch_march3d(1, a, F, 0, st));
+                    SWEM_TRY(launch_march3d(2, a, F, 0, st));
+                }
                 // this depth's image, summed over the chunk's frequency groups
                 SWEM_TRY(launch_reduce_depth(p->partial.p, (a.nitem + F - 1) / F, (int)nm, out + iz, (int)nz, acc, st));
             }
@@ -602,14 +623,48 @@ static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, con
     return SWEM_OK;
 }
 
+// 3-D PSPI modelling march of one frequency chunk (compat: select after every step, wem.c:252-281
+// with pspiop).  Source: x-pass then y-pass per depth, the depth's source stored.  Receiver,
+// upward: y-pass (select the previous step's slices, inject m * source, forward y-FFT), x-pass.
+static int model3d_pspi(swem_plan* p, March3DArgs a, int F, hipStream_t st)
+{
+    const int nz = p->g.nz;
+    for (int iz = 0; iz < nz; iz++) {
+        a.iz = iz;
+        a.iz_sel = iz;
+        a.ds = stepped_at(p, iz, p->g.sz);
+        a.next_s = stepped_at(p, iz + 1, p->g.sz);
+        a.wmask = 1;
+        if (a.ds) SWEM_TRY(launch_march3d(4, a, F, 0, st));
+        SWEM_TRY(launch_march3d(5, a, F, 1, st));
+    }
+    a.first = 1;
+    int last = -1;  // depth of the last receiver x-pass
+    for (int iz = nz - 1; iz >= 0; iz--) {
+        if (!stepped_at(p, iz, p->g.gz)) continue;
+        a.iz = iz;
+        a.iz_sel = last < 0 ? iz : last;
+        a.wmask = 2;
+        SWEM_TRY(launch_march3d(5, a, F, 2, st));
+        SWEM_TRY(launch_march3d(4, a, F, 0, st));
+        a.first = 0;
+        last = iz;
+    }
+    a.iz_sel = last < 0 ? 0 : last;
+    a.first = last < 0 ? 1 : 0;
+    return launch_march3d(5, a, F, 3, st);
+}
+
 static int model3d_impl(swem_plan* p, const float* m_dev, const float* sx, const float* sy, int nshot, float* d_dev,
                         hipStream_t st)
 {
     const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
     const int F = p->F3f;
-    const size_t per_item = 2 * (size_t)p->g.nmx * p->s.nky * sizeof(float2) + nz * nm * sizeof(float2);
+    const size_t per_item =
+        2 * (size_t)p->g.nmx * p->s.nky * sizeof(float2) + nz * nm * sizeof(float2) + pspi3_slice_bytes(p);
     const int nfc = chunk3(p, per_item, F);
     SWEM_TRY(p->W.alloc((size_t)nfc * 2 * p->g.nmx * p->s.nky));
+    if (p->g.pspi) SWEM_TRY(p->R.alloc((size_t)nfc * pspi3_slice_bytes(p) / sizeof(float2)));
     SWEM_TRY(p->smig.alloc((size_t)nfc * nz * nm));
     SWEM_TRY(p->dgw.alloc(nm * nf));
     SWEM_TRY(p->mT.alloc(nm * nz));
@@ -634,7 +689,15 @@ static int model3d_impl(swem_plan* p, const float* m_dev, const float* sx, const
             a.dgw = p->dgw.p;
             a.conj_src = 1;
             a.conj_rec = 1;
+            a.R = p->R.p;
             SWEM_TRY(launch_march3d(0, a, F, 0, st));
+            if (p->g.pspi) {
+                SWEM_TRY(model3d_pspi(p, a, F, st));
+                SWEM_TRY(record(p, 1, st));
+                p->last_launches++;
+                p->last_flops += step_flops(p) * (double)a.nitem * nz;
+                continue;
+            }
             // source march, stored per depth (wem.c:254-265)
             for (int iz = 0; iz < (int)nz; iz++) {
                 a.iz = iz;
@@ -763,7 +826,8 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
     p->nkx = p->s.nkx;
     p->three_d = g->nmy > 1;
     if (p->three_d) {
-        if (g->pspi) return set_error(SWEM_EUNSUPPORTED, "3-D PSPI is not available in this build (3-D SSF is)");
+        if (g->pspi && o.pspi_mode != SWEM_PSPI_COMPAT)
+            return set_error(SWEM_EUNSUPPORTED, "3-D PSPI: the linear mode is not available in this build (compat is)");
         if (g->nmx < 2 || !march3d_supported(p->s.nkx, p->s.nky))
             return set_error(SWEM_EUNSUPPORTED, "3-D: nkx=%d, nky=%d: this build handles powers of two 128..1024", p->s.nkx,
                              p->s.nky);
@@ -777,7 +841,7 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
     if (p->three_d) {
         // y-pass workgroups: one wave per frequency (both wavefields in migration)
         if (o.freqs_per_block > 0) p->F3 = p->F3f = o.freqs_per_block;
-        if (p->F3 < 1 || 64 * p->F3 > 1024 || 64 * p->F3f > 1024)
+        if (p->F3 < 1 || 64 * p->F3 > (g->pspi ? 512 : 1024) || 64 * p->F3f > (g->pspi ? 512 : 1024))
             return set_error(SWEM_EINVAL, "3-D: freqs_per_block=%d (1..16)", o.freqs_per_block);
     } else {
         const March2DShape sa = march2d_shape(p->nkx, o.variant, 1), sf = march2d_shape(p->nkx, o.variant, 0);
@@ -816,7 +880,7 @@ void plan_free(swem_plan* p)
     p->taper.release(); p->p2f.release(); p->w2.release();
     p->own_bases.adj.release(); p->own_bases.fwd.release(); p->own_bases.src.release();
     p->p2.release(); p->sel.release(); p->sel2.release();
-    p->used.release(); p->cref.release(); p->spec_r.release(); p->smig.release(); p->dgw.release(); p->partial.release();
+    p->used.release(); p->cref.release(); p->R.release(); p->spec_r.release(); p->smig.release(); p->dgw.release(); p->partial.release();
     p->mT.release(); p->igx.release();
     delete p;
 }
@@ -834,8 +898,10 @@ size_t plan_upload_bytes(const swem_plan* p, int nshot)
     const size_t nz = p->g.nz, nm = p->nm, nmx = p->g.nmx, nref = p->g.pspi ? p->g.nref : 1;
     size_t b = pad256(p->nkx * 8) + pad256(p->s.nky * 8) + pad256(nmx * 4) + pad256(p->g.nmy * 4) + pad256(p->g.nt * 4);
     b += pad256(nz * nref * 8) + pad256(nz * nref * 4) + pad256(nz * 4);
-    if (p->g.pspi)
-        b += 2 * pad256(nz * p->nkx) + pad256(nz * p->nkx * 4);
+    if (p->g.pspi) {
+        const size_t srow = p->three_d ? nm : (size_t)p->nkx;
+        b += 2 * pad256(nz * srow) + pad256(nz * srow * 4) + pad256(nz * kCrefStride * 4) + pad256(nz * 4);
+    }
     else
         b += pad256(nz * nm * 8) + pad256(nz * nm * 4);
     return b + pad256((size_t)nshot * 4);
@@ -848,9 +914,10 @@ size_t plan_migrate_bytes(const swem_plan* p, int nshot)
     if (!p->basis_cache) b += 3 * pad256((size_t)p->g.nt * 2 * nf * 4);
     if (p->three_d) {
         const int F = p->F3;
-        const size_t per_item = 2 * nmx * p->s.nky * sizeof(float2) + nm * sizeof(float) / F + 1;
+        const size_t per_item = 2 * nmx * p->s.nky * sizeof(float2) + nm * sizeof(float) / F + 1 + pspi3_slice_bytes(p);
         const size_t nfc = chunk3(p, per_item, F), gmax = (nfc + F - 1) / F;
         b += pad256(nfc * 2 * nmx * p->s.nky * 8) + pad256(gmax * nm * 4) + pad256(nm * nf * 8);
+        if (p->g.pspi) b += pad256(nfc * pspi3_slice_bytes(p));
         return b;
     }
     const size_t nkh = p->nkx / 2 + 1;

@@ -129,10 +129,18 @@ struct March3DArgs {
     const float* mT;            // [nz][nm] image (modelling input)
     float2* smig;               // [nitem][nz][nm] source wavefields (modelling)
     float2* dgw;                // [nm][nf] modelled spectra of this shot
+    // PSPI (compat): reference velocities, per-point selection iref1 | cut << 7 ([nz][nm]),
+    // vrefs in use per depth, per-vref x-transformed slices [nitem][2][nref][nmx][nky], and the
+    // depth whose step produced the slices a y-pass selects from
+    int nref, iz_sel;
+    const uint8_t* sel;
+    const uint32_t* used;
+    float2* R;
 };
 bool march3d_supported(int nkx, int nky);
 // which: 0 init, 1 x-pass, 2 migration y-pass, 3 modelling y-pass (phase 0 source,
-// 1 receiver, 2 output).  F: frequencies per y-pass workgroup.
+// 1 receiver, 2 output), 4 PSPI x-pass, 5 PSPI y-pass (phase 0 migration, 1 modelling source,
+// 2 modelling receiver, 3 modelling output).  F: frequencies per y-pass workgroup.
 int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_t st);
 
 // C[M x Nc] = A[M x K] * B[K x Nc], row-major with leading dimensions lda, ldb, ldc.

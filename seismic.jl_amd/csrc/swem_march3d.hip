@@ -43,7 +43,7 @@ __device__ __forceinline__ double reduce_rev(double phase)
 }
 
 // kz phase of one (kx, ky) point, +dz orientation, with 1/nk folded in (wem.c:336-344).
-__device__ __forceinline__ float2 kz_phase(const March3DArgs& a, double w, double p2, int ikx, int iky)
+__device__ __forceinline__ float2 kz_phase(const March3DArgs& a, double w, double p2, float p2f, int ikx, int iky)
 {
     const int kxi = ikx <= a.nkx / 2 ? ikx : a.nkx - ikx;
     const int kyi = iky <= a.nky / 2 ? iky : a.nky - iky;
@@ -63,7 +63,7 @@ __device__ __forceinline__ float2 kz_phase(const March3DArgs& a, double w, doubl
     // the reference's float expressions (wem.c:336-343)
     const float wf = (float)w;
     const float kx = a.dkxf * (float)kxi, ky = a.dkyf * (float)kyi;
-    const float s = (wf * wf) * a.p2f[a.iz] - kx * kx - ky * ky;
+    const float s = (wf * wf) * p2f - kx * kx - ky * ky;
     if (s >= 0.f) {
         float sn, cs;
         sincosf(sqrtf(s) * (float)a.dzd, &sn, &cs);
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
     for (int k = 0; k < NWF; ++k) cj[k] = wf_of(k) == 0 ? a.conj_src : a.conj_rec;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-        const float2 ph = a.prec == SWEM_PREC_KZ64 ? kz_phase64(a, c0, t + 64 * q) : kz_phase(a, wd, p2, t + 64 * q, iky);
+        const float2 ph = a.prec == SWEM_PREC_KZ64 ? kz_phase64(a, c0, t + 64 * q) : kz_phase(a, wd, p2, a.p2f[a.iz], t + 64 * q, iky);
 #pragma unroll
         for (int k = 0; k < NWF; ++k) v[k][q] = cmul(v[k][q], make_float2(ph.x, cj[k] ? -ph.y : ph.y));
     }
@@ -242,6 +242,264 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
             const int x = x1 + 64 * j;
             if (x < a.nmx) dst[(size_t)x * a.nky] = src[lds_pad(x)];
         }
+    }
+}
+
+// ------------------------------------------------------------------------ PSPI (compat)
+// pspiop (wem.c:380-471) in the same two passes: the x-pass transforms each ky column once and,
+// for every vref the depth uses, multiplies by its kz phase and transforms back, writing one
+// slice per vref; the y-pass transforms the rows of those slices back and keeps, per point, the
+// slice of its vref iref1 -- its real part where vref2 - vref1 > 10 (linear_interp returns a
+// float, wem.c:450-458, 473-478).  No taper and no split-step phase (wem.c:418, 461-463).
+
+__device__ __forceinline__ float2* rslice(const March3DArgs& a, int item, int wf, int ir)
+{
+    return a.R + (((size_t)item * 2 + wf) * a.nref + ir) * (size_t)a.nmx * a.nky;
+}
+
+// x-pass: as march3d_cols up to the forward x-FFT, then per vref in use: kz phase of
+// vref[ir][iz] (conj for a -dz step), inverse x-FFT, slice tile written through LDS.
+template <int NX, int NWF, int C>
+__global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 512 ? 2 : 4))) void march3d_cols_pspi(
+    March3DArgs a)
+{
+    constexpr int P = NX / 64;
+    constexpr int L = LdsLen<NX>::value;
+    extern __shared__ float2 smem[];
+    const int tiles = a.nky / C;
+    const int nblk = a.nitem * tiles;
+    const int full = nblk & ~7;
+    int b = blockIdx.x;
+    if (b < full) b = (b & 7) * (full >> 3) + (b >> 3);
+    const int item = b / tiles;
+    const int ky0 = (b - item * tiles) * C;
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const int wf1 = (a.wmask & 1) ? 0 : 1;
+    auto wf_of = [&](int k) { return NWF == 2 ? k : wf1; };
+    const int c = threadIdx.x % C, x0 = threadIdx.x / C;
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) {
+        const float2* src = state(a, item, wf_of(k)) + ky0 + c;
+        float2* dst = smem + (c * NWF + k) * L;
+        float2 st[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int x = x0 + 64 * j;
+            st[j] = x < a.nmx ? src[(size_t)x * a.nky] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int x = x0 + 64 * j;
+            if (x < a.nmx) dst[lds_pad(x)] = st[j];
+        }
+    }
+    __syncthreads();
+    float2 v[NWF][P];
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) {
+        float2* buf = smem + (w * NWF + k) * L;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int x = t + 64 * q;
+            v[k][q] = x < a.nmx ? buf[lds_pad(x)] : make_float2(0.f, 0.f);
+        }
+        wave_sync_lds();
+        wfft<NX, -1>(v[k], buf, a.twx, t);
+    }
+    const int iw = a.ifmin + a.f0 + item;
+    const double wd = (double)iw * a.dw;
+    const int iky = ky0 + w;
+    const int kyi = iky <= a.nky / 2 ? iky : a.nky - iky;
+    const double ky = kyi * a.dky;
+    bool cj[NWF];
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) cj[k] = wf_of(k) == 0 ? a.conj_src : a.conj_rec;
+    const uint32_t used = a.used[a.iz];
+    int c1 = c, x1 = x0;
+    asm volatile("" : "+v"(c1), "+v"(x1));
+    for (int ir = 0; ir < a.nref; ++ir) {
+        if (!((used >> ir) & 1u)) continue;
+        const double p2 = a.p2[(size_t)a.iz * a.nref + ir];
+        const float p2f = a.p2f[(size_t)a.iz * a.nref + ir];
+        const double c0 = wd * wd * p2 - ky * ky;
+#pragma unroll
+        for (int k = 0; k < NWF; ++k) {
+            float2 bq[P];
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const float2 ph = a.prec == SWEM_PREC_KZ64 ? kz_phase64(a, c0, t + 64 * q)
+                                                           : kz_phase(a, wd, p2, p2f, t + 64 * q, iky);
+                bq[q] = cmul(v[k][q], make_float2(ph.x, cj[k] ? -ph.y : ph.y));
+            }
+            float2* buf = smem + (w * NWF + k) * L;
+            wave_sync_lds();
+            wfft<NX, +1>(bq, buf, a.twx, t);
+            wave_sync_lds();
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int x = t + 64 * q;
+                if (x < a.nmx) buf[lds_pad(x)] = bq[q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NWF; ++k) {
+            float2* dst = rslice(a, item, wf_of(k), ir) + ky0 + c1;
+            const float2* src = smem + (c1 * NWF + k) * L;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int x = x1 + 64 * j;
+                if (x < a.nmx) dst[(size_t)x * a.nky] = src[lds_pad(x)];
+            }
+        }
+        __syncthreads();  // the tile is rewritten by the next vref
+    }
+}
+
+// u = the pspiop selection of row x of wavefield wf from the slices of depth iz_sel: inverse
+// y-FFT of each slice in use, per point the slice of its iref1, real part where cut.
+template <int NY>
+__device__ __forceinline__ void pspi_select_row(const March3DArgs& a, int item, int wf, int x, float2 (&u)[NY / 64],
+                                                float2* buf, int t)
+{
+    constexpr int P = NY / 64;
+    const uint32_t used = a.used[a.iz_sel];
+    const uint8_t* sr = a.sel + (size_t)a.iz_sel * a.nm + (size_t)x * a.nmy;
+    uint8_t s1[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int y = t + 64 * q;
+        s1[q] = y < a.nmy ? sr[y] : (uint8_t)0x7f;
+        u[q] = make_float2(0.f, 0.f);
+    }
+    for (int ir = 0; ir < a.nref; ++ir) {
+        if (!((used >> ir) & 1u)) continue;
+        const float2* row = rslice(a, item, wf, ir) + (size_t)x * a.nky;
+        float2 bq[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) bq[q] = row[t + 64 * q];
+        wfft<NY, +1>(bq, buf, a.twy, t);
+        wave_sync_lds();
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+            if ((s1[q] & 0x7f) == ir) u[q] = (s1[q] >> 7) ? make_float2(bq[q].x, 0.f) : bq[q];
+    }
+}
+
+// y-pass of PSPI, F waves (frequencies g F + f) of one row x.
+//   phase 0 migration: select the stepped wavefields (the source is re-read from its state when
+//           only the receiver steps), image (summed over the F frequencies in a fixed order),
+//           forward y-FFT of what steps next
+//   phase 1 modelling source: select, store the depth's source in smig (zero where z < sz),
+//           forward y-FFT for the next step
+//   phase 2 modelling receiver: select the previous step's result (zero at the first), inject
+//           m smig of this depth, forward y-FFT
+//   phase 3 modelling output: select the last step's result (zero when no step ran), spectra out
+template <int NY>
+__global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, int phase)
+{
+    constexpr int P = NY / 64;
+    constexpr int L = LdsLen<NY>::value;
+    extern __shared__ float2 smem[];
+    const int ngroups = (a.nitem + F - 1) / F;
+    const int g = blockIdx.x % ngroups, x = blockIdx.x / ngroups;
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const int item = g * F + w;
+    const bool active = item < a.nitem;
+    float2* buf = smem + w * L;
+    if (phase != 0) {
+        if (!active) return;  // no workgroup barriers below
+        const int fl = a.f0 + item;
+        float2 u[P];
+        if (phase == 1) {
+            if (a.ds) pspi_select_row<NY>(a, item, 0, x, u, buf, t);
+            float2* sm = a.smig + ((size_t)item * a.nz + a.iz) * a.nm + (size_t)x * a.nmy;
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int y = t + 64 * q;
+                if (y < a.nmy) sm[y] = a.ds ? u[q] : make_float2(0.f, 0.f);
+            }
+            if (!(a.ds && a.next_s)) return;
+        } else {
+            if (a.first) {
+#pragma unroll
+                for (int q = 0; q < P; ++q) u[q] = make_float2(0.f, 0.f);
+            } else {
+                pspi_select_row<NY>(a, item, 1, x, u, buf, t);
+            }
+            if (phase == 3) {
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const int y = t + 64 * q;
+                    if (y < a.nmy) a.dgw[((size_t)x * a.nmy + y) * a.nf + fl] = u[q];
+                }
+                return;
+            }
+            const float2* sm = a.smig + ((size_t)item * a.nz + a.iz) * a.nm + (size_t)x * a.nmy;
+            const float* mt = a.mT + (size_t)a.iz * a.nm + (size_t)x * a.nmy;
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int y = t + 64 * q;
+                if (y < a.nmy) u[q] = cadd(u[q], cscale(sm[y], mt[y]));
+            }
+        }
+        wave_sync_lds();
+        wfft<NY, -1>(u, buf, a.twy, t);
+        float2* dst = state(a, item, phase == 1 ? 0 : 1) + (size_t)x * a.nky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) dst[t + 64 * q] = u[q];
+        return;
+    }
+    // phase 0: migration
+    const int iw = a.ifmin + a.f0 + (active ? item : 0);
+    float2 vs[P], vr[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) vs[q] = vr[q] = make_float2(0.f, 0.f);
+    if (active && a.ds) {
+        pspi_select_row<NY>(a, item, 0, x, vs, buf, t);
+    } else if (active && a.dr) {
+        // an unstepped source: its state is FFT_y(u), so the inverse transform returns nky u
+        const float2* src = state(a, item, 0) + (size_t)x * a.nky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) vs[q] = src[t + 64 * q];
+        wfft<NY, +1>(vs, buf, a.twy, t);
+        wave_sync_lds();
+        const float rn = 1.f / (float)a.nky;  // exact: nky is a power of two
+#pragma unroll
+        for (int q = 0; q < P; ++q) vs[q] = cscale(vs[q], rn);
+    }
+    if (active && a.dr) pspi_select_row<NY>(a, item, 1, x, vr, buf, t);
+    float cc[P];
+    const float factor = iw == 0 ? 1.f : 2.f;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        if (t + 64 * q >= a.nmy) vs[q] = vr[q] = make_float2(0.f, 0.f);
+        cc[q] = (active && a.dr) ? factor * (vr[q].x * vs[q].x + vr[q].y * vs[q].y) : 0.f;
+    }
+    wave_sync_lds();
+    float* red = reinterpret_cast<float*>(buf);
+#pragma unroll
+    for (int q = 0; q < P; ++q) red[t + 64 * q] = cc[q];
+    __syncthreads();
+    float* part = a.partial + (size_t)g * a.nm + (size_t)x * a.nmy;
+    for (int y = threadIdx.x; y < a.nmy; y += blockDim.x) {
+        float sum = 0.f;
+        for (int ff = 0; ff < F; ++ff) sum += reinterpret_cast<const float*>(smem + ff * L)[y];
+        part[y] = sum;
+    }
+    __syncthreads();
+    if (active && a.ds && a.next_s) {
+        wfft<NY, -1>(vs, buf, a.twy, t);
+        float2* dst = state(a, item, 0) + (size_t)x * a.nky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) dst[t + 64 * q] = vs[q];
+        wave_sync_lds();
+    }
+    if (active && a.dr && a.next_r) {
+        wfft<NY, -1>(vr, buf, a.twy, t);
+        float2* dst = state(a, item, 1) + (size_t)x * a.nky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) dst[t + 64 * q] = vr[q];
     }
 }
 
@@ -281,6 +539,11 @@ __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
     if (need_s) {
         wfft<NY, +1>(vs, buf, a.twy, t);
         wave_sync_lds();
+        if (!a.ds) {  // unstepped source: its state is FFT_y(u), the inverse returns nky u
+            const float rn = 1.f / (float)a.nky;  // exact: nky is a power of two
+#pragma unroll
+            for (int q = 0; q < P; ++q) vs[q] = cscale(vs[q], rn);
+        }
     }
     if (need_r) wfft<NY, +1>(vr, buf, a.twy, t);
     float cc[P];
@@ -435,7 +698,20 @@ static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStrea
         return nwf == 2 ? launch3(march3d_cols<N, 2, C2>, grid, block, lds, st, a)
                         : launch3(march3d_cols<N, 1, C1>, grid, block, lds, st, a);
     }
+    if (which == 4) {  // PSPI x-pass
+        const int nwf = (a.wmask & 1) + ((a.wmask >> 1) & 1);
+        const int C = march3d_cols_tile(nwf, N);
+        const size_t lds = (size_t)C * nwf * L * sizeof(float2);
+        const dim3 grid(a.nitem * (a.nky / C)), block(64 * C);
+        constexpr int C2 = N >= 1024 ? 4 : 8, C1 = 2 * C2;
+        return nwf == 2 ? launch3(march3d_cols_pspi<N, 2, C2>, grid, block, lds, st, a)
+                        : launch3(march3d_cols_pspi<N, 1, C1>, grid, block, lds, st, a);
+    }
     const int ngroups = (a.nitem + F - 1) / F;
+    if (which == 5) {
+        const size_t lds = (size_t)F * L * sizeof(float2);
+        return launch3(march3d_rows_pspi<N>, dim3(ngroups * a.nmx), dim3(64 * F), lds, st, a, F, phase);
+    }
     if (which == 2) {
         const size_t lds = (size_t)F * L * sizeof(float2);
         return launch3(march3d_rows_adj<N>, dim3(ngroups * a.nmx), dim3(64 * F), lds, st, a, F);
@@ -452,11 +728,13 @@ static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStrea
 // (phase 0 source, 1 receiver, 2 output).  F: frequencies per y-pass workgroup.
 int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_t st)
 {
-    const int n = which == 1 ? a.nkx : a.nky;
+    const int n = (which == 1 || which == 4) ? a.nkx : a.nky;
     if (!march3d_supported(a.nkx, a.nky) || a.nmx > a.nkx || a.nmy > a.nky || a.nitem < 1)
         return set_error(SWEM_EINVAL, "3-D march: inconsistent launch (nkx=%d nky=%d)", a.nkx, a.nky);
-    if (((which == 2 || which == 3) && 64 * F > 1024) || F < 1)
+    if (((which == 2 || which == 3) && 64 * F > 1024) || (which == 5 && 64 * F > 512) || F < 1)
         return set_error(SWEM_EINVAL, "3-D march: %d frequencies per workgroup", F);
+    if ((which == 4 || which == 5) && (!a.R || !a.sel || !a.used || a.nref < 1 || a.nref > 31))
+        return set_error(SWEM_EINVAL, "3-D PSPI march: tables missing");
     switch (n) {
     case 128: return launch3_n<128>(which, a, F, phase, st);
     case 256: return launch3_n<256>(which, a, F, phase, st);

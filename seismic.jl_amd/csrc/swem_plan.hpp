@@ -104,7 +104,7 @@ struct swem_plan {
     swem::DevBuf<int> cref;  // [nz][kCrefStride] uncut points of a depth (PSPI compat)
     bool packed_ok = false;  // PSPI compat migration may run the packed kernel
     // work
-    swem::DevBuf<float2> spec_r, smig, dgw, W;
+    swem::DevBuf<float2> spec_r, smig, dgw, W, R;  // R: 3-D PSPI per-vref slices
     swem::DevBuf<float> partial, mT;
     swem::DevBuf<int> igx;
     std::vector<int> igx_host;

@@ -40,6 +40,20 @@ def test_migrate3d_matches_oracle(gpu, nx, ny, nz):
     assert err <= TOL
 
 
+def test_migrate3d_source_below_surface(gpu):
+    """sz = 25 m: the source is not stepped at the first three depths while the receiver is
+    (wem.c:235-241); the image there uses the source as injected."""
+    g, vel, refl, wav, sx, sy = small3d(64, 64, 12)
+    g.sz = 25.0
+    rng = np.random.default_rng(5)
+    d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
+    m_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), d=d, adj=True)
+    m_gpu = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)
+    err = rel_l2(m_gpu, m_ref)
+    print(f"3-D migrate, source below the surface: rel-L2 {err:.3e}", flush=True)
+    assert err <= TOL
+
+
 @pytest.mark.parametrize("nx,ny,nz", [(64, 64, 24), (128, 64, 12)])
 def test_model3d_matches_oracle(gpu, nx, ny, nz):
     g, vel, refl, wav, sx, sy = small3d(nx, ny, nz)
@@ -107,3 +121,44 @@ def test_config3_full_size_dot_and_linearity(gpu):
     err = rel_l2(m12, 0.5 * m1.astype(np.float64) - m2)
     print(f"config3 linearity rel-L2 {err:.3e}", flush=True)
     assert err <= 1e-5
+
+
+def pspi3d(nx, ny, nz, lateral=True, sz=0.0, gz=0.0):
+    g, vel, refl, wav, sx, sy = small3d(nx, ny, nz)
+    g.pspi, g.nref, g.sz, g.gz = True, 4, sz, gz
+    if not lateral:  # v(z): vmax <= vmin + 10 at every depth, pspiop keeps the complex values
+        vel = np.repeat(vel.reshape(nx * ny, nz)[:1], nx * ny, 0)
+    return g, vel, refl, wav, sx, sy
+
+
+@pytest.mark.parametrize("nx,ny,nz,lateral,sz", [(64, 64, 16, True, 0.0), (128, 64, 10, True, 0.0),
+                                                 (64, 64, 12, False, 0.0), (64, 64, 12, True, 25.0)])
+def test_pspi3d_migrate_matches_oracle(gpu, nx, ny, nz, lateral, sz):
+    """3-D PSPI (compat, pspiop wem.c:380-471 with nky > 1): the per-vref x-pass and the selecting
+    y-pass against the FP64 oracle; a v(z) model keeps complex values (no cut), a source below the
+    surface leaves the first depths unstepped for the source."""
+    g, vel, refl, wav, sx, sy = pspi3d(nx, ny, nz, lateral, sz=sz)
+    rng = np.random.default_rng(nx + ny + nz)
+    d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
+    m_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), d=d, adj=True)
+    m_gpu = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)
+    err = rel_l2(m_gpu, m_ref)
+    print(f"3-D PSPI migrate {nx}x{ny}x{nz} lateral={lateral} sz={sz}: rel-L2 {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("nx,ny,nz,gz", [(64, 64, 16, 0.0), (64, 64, 12, 25.0)])
+def test_pspi3d_model_matches_oracle(gpu, nx, ny, nz, gz):
+    g, vel, refl, wav, sx, sy = pspi3d(nx, ny, nz, gz=gz)
+    d_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), m=refl, adj=False)
+    d_gpu = seiswem.ShotProfileWEM(refl, False, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)[0]
+    err = rel_l2(d_gpu, d_ref)
+    print(f"3-D PSPI model {nx}x{ny}x{nz} gz={gz}: rel-L2 {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+def test_pspi3d_linear_mode_is_refused(gpu):
+    g, vel, refl, wav, sx, sy = pspi3d(64, 64, 4)
+    with pytest.raises(seiswem.SwemError) as e:
+        seiswem.Plan(g, vel, wav, pspi_mode=seiswem.PSPI_LINEAR)
+    assert e.value.code == seiswem.SWEM_EUNSUPPORTED
