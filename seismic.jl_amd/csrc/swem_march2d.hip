@@ -11,6 +11,10 @@
 // over frequency groups and shots with compensated sums -- so a shot's image does not depend on
 // what else was launched with it.
 //
+// The default migration kernels keep both wavefields of a frequency in one subgroup instead:
+// PAIR steps them in turn (image register-local), and for PSPI compat PACKED steps both with one
+// transform per vref (their real parts packed in one complex array, see adj_body_packed).
+//
 // Modelling (forward): each subgroup owns one frequency, marches the source down storing it per
 // depth in HBM, then marches the receiver up injecting m * source before each step.
 //
