@@ -790,7 +790,9 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
             n2r += z[q].y * z[q].y + e[q].y * e[q].y;
         }
         // keep the two packed wavefields of comparable size (the rounding of each transform is
-        // relative to their sum): rescale the receiver part when the ratio leaves [1/2, 2]
+        // relative to their sum): every 8 depths, rescale the receiver part when the ratio of
+        // their energies leaves [1/4, 4]
+        if ((iz & 7) != 7) continue;
         n2s = subgroup_sum<T>(n2s, red);
         n2r = subgroup_sum<T>(n2r, red);
         if (n2s > 0.f && n2r > 0.f && (n2s > 4.f * n2r || n2r > 4.f * n2s)) {
