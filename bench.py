@@ -37,7 +37,7 @@ def cpu_baseline(g, vel, wav, sx, m_gpu_fn, cores):
     from seiswem import WemGeometry
 
     bins_per_hz = g.dt * (g.padt * g.nt)  # ifmax - ifmin per Hz
-    nsamp = 10 * cores  # about 10 s of CPU work
+    nsamp = 20 * cores  # about 15 s of CPU work
     fmin = 40.0
     fmax = fmin + nsamp / bins_per_hz
     gs = WemGeometry(**{**g.__dict__, "fmin": fmin, "fmax": fmax})
