@@ -597,7 +597,9 @@ __device__ __forceinline__ void subgroup_sync()
     }
 }
 
-template <int N, int P, bool WAVE>
+// HALF: nmx <= N / 2, so the points q >= P / 2 (x >= N / 2) are zero in the state and never selected:
+// those constants let the first forward pass and the last inverse pass drop their work.
+template <int N, int P, bool WAVE, bool HALF>
 __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
 {
     constexpr int T = N / P;
@@ -644,6 +646,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
         const float2 us = x == igx ? ss : make_float2(0.f, 0.f);
         z[q] = make_float2(us.x, alpha * e[q].x);
         e[q] = make_float2(us.y, -alpha * e[q].y);
+        if (HALF && q >= P / 2) z[q] = e[q] = make_float2(0.f, 0.f);
     }
     float* part = a.partial + ((size_t)shot * a.ngroups + g) * (size_t)a.nz * a.nmx;
     // e before the step: dense (the initial imaginary parts), sparse (nonzero only at the
@@ -708,7 +711,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
             }
             xfft<N, P, +1, WAVE>(w, smem, a.tw, t, pp);
 #pragma unroll
-            for (int q = 0; q < P; ++q)
+            for (int q = 0; q < (HALF ? P / 2 : P); ++q)
                 if ((int)(s1(q) & 0x7fu) == ir) out[q] = w[q];
         }
         if (cnt > 0) {
@@ -727,7 +730,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
                     const int x = t + q * T;
-                    if (!(x < a.nmx && !(s1(q) >> 7))) e[q] = make_float2(0.f, 0.f);
+                    if ((HALF && q >= P / 2) || !(x < a.nmx && !(s1(q) >> 7))) e[q] = make_float2(0.f, 0.f);
                 }
             } else {
                 // e_new(x_j) = sum_k U(k) conj(w^(k x_j)) at the few uncut points: partial sums
@@ -770,7 +773,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
                     if (j >= cnt) break;
                     const int xj = crow[2 + j];
 #pragma unroll
-                    for (int q = 0; q < P; ++q)
+                    for (int q = 0; q < (HALF ? P / 2 : P); ++q)
                         if (t + q * T == xj) e[q] = acc[j];
                 }
             }
@@ -784,7 +787,11 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
 #pragma unroll
         for (int q = 0; q < P; ++q) {
             z[q] = out[q];
-            if (cnt == 0) e[q] = make_float2(0.f, 0.f);
+            if (cnt == 0 || (HALF && q >= P / 2)) e[q] = make_float2(0.f, 0.f);
+            if (HALF && q >= P / 2) {
+                z[q] = make_float2(0.f, 0.f);
+                continue;  // x >= nmx: nothing to image
+            }
             buf_store1(pr, (uint32_t)(t + q * T) * 4u, factor * (z[q].x * z[q].y - e[q].x * e[q].y) * inv_alpha);
             n2s += z[q].x * z[q].x + e[q].x * e[q].x;
             n2r += z[q].y * z[q].y + e[q].y * e[q].y;
@@ -808,16 +815,16 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
     }
 }
 
-template <int N, int P>
+template <int N, int P, bool HALF>
 __global__ __launch_bounds__(N / P) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_packed(March2DArgs a)
 {
-    adj_body_packed<N, P, false>(a);
+    adj_body_packed<N, P, false, HALF>(a);
 }
 
-template <int N>
+template <int N, bool HALF>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_packed_w(March2DArgs a)
 {
-    adj_body_packed<N, N / 64, true>(a);
+    adj_body_packed<N, N / 64, true, HALF>(a);
 }
 
 #define SWEM_SUBGROUP_BOUNDS(P) __launch_bounds__((P) == 16 ? 512 : kMaxBlock)
@@ -927,10 +934,13 @@ static int launch_one(const March2DArgs& a, const March2DShape& sh, int F, int a
     dim3 block(sh.T * sh.per_freq * F);
     if (sh.variant == SWEM_VARIANT_PACKED && MODE == kModePspiCompat && adj) {
         const size_t lds_p = lds + kPackedScratch * sizeof(float2);
+        const bool half = 2 * a.nmx <= N;
         if constexpr (N < 1024)
-            return launch_kernel(march2d_adj_packed_w<N>, a, block, lds_p, st);
+            return half ? launch_kernel(march2d_adj_packed_w<N, true>, a, block, lds_p, st)
+                        : launch_kernel(march2d_adj_packed_w<N, false>, a, block, lds_p, st);
         else
-            return launch_kernel(march2d_adj_packed<N, 16>, a, block, lds_p, st);
+            return half ? launch_kernel(march2d_adj_packed<N, 16, true>, a, block, lds_p, st)
+                        : launch_kernel(march2d_adj_packed<N, 16, false>, a, block, lds_p, st);
     }
     if (sh.variant == SWEM_VARIANT_PAIR || sh.variant == SWEM_VARIANT_PACKED) {
         if constexpr (N < 1024)
