@@ -225,7 +225,8 @@ int swem_shot_main(int argc, char** argv)
     par_bool(argc, argv, "verbose", verbose);
     if (!par_string(argc, argv, "d", d_name) || !par_string(argc, argv, "m", m_name) ||
         !par_string(argc, argv, "vel", vel_name) || !par_string(argc, argv, "wav", wav_name)) {
-        std::fprintf(stderr, " Common shot wave equation migration for 3D acoustic, isotropic data. \n");
+        std::fprintf(stderr, "shotwem: one-shot wave-equation migration (adj=y) or modelling (adj=n) of .seis files;"
+                             " required: d= m= vel= wav=\n");
         return set_error(SWEM_EINVAL, "shotwem: d=, m=, vel= and wav= are required");
     }
     par_float(argc, argv, "sz", sz);
