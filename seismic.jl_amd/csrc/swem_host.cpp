@@ -4,6 +4,7 @@
 #include <atomic>
 #include <mutex>
 #include <cmath>
+#include <complex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -136,18 +137,75 @@ int build_bases(const swem_geometry& g, const Sampling& sm, int nf, Bases& b)
     return SWEM_OK;
 }
 
+// In-place radix-2 transform of a power-of-two length, exp(sign 2 pi i jk / n), FP64.
+static void fft_pow2_f64(std::vector<std::complex<double>>& x, int sign)
+{
+    const size_t n = x.size();
+    for (size_t i = 1, j = 0; i < n; i++) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(x[i], x[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        for (size_t k = 0; k < len / 2; k++) {
+            const double a = sign * 2.0 * kPi * (double)k / (double)len;
+            const std::complex<double> w(std::cos(a), std::sin(a));
+            for (size_t i = 0; i < n; i += len) {
+                const std::complex<double> u = x[i + k], v = x[i + k + len / 2] * w;
+                x[i + k] = u + v;
+                x[i + k + len / 2] = u - v;
+            }
+        }
+    }
+}
+
+// Bluestein tables of a length-n transform through length m (m >= 2n - 1, a power of two):
+// chirp_j = exp(-i pi j^2 / n) (j^2 reduced mod 2n, exact in integers), and for the two signs s
+// of the transform the spectra B_s = FFT_m(b_s) / m of the wrapped even kernel
+// b_s(j) = b_s(m - j) = exp(-s i pi j^2 / n), j < n (zero between).
+static void bluestein_tables(int n, int m, std::vector<float2>& chirp, std::vector<float2>& bchirp)
+{
+    std::vector<std::complex<double>> c(n);
+    for (int j = 0; j < n; j++) {
+        const long long e = ((long long)j * j) % (2LL * n);
+        const double a = -kPi * (double)e / (double)n;
+        c[j] = std::complex<double>(std::cos(a), std::sin(a));
+    }
+    chirp.resize(n);
+    for (int j = 0; j < n; j++) chirp[j] = make_float2((float)c[j].real(), (float)c[j].imag());
+    bchirp.assign((size_t)2 * m, make_float2(0.f, 0.f));
+    for (int h = 0; h < 2; h++) {  // h = 0: sign -1 (kernel conj(chirp)), h = 1: sign +1 (chirp)
+        std::vector<std::complex<double>> b(m, 0.0);
+        for (int j = 0; j < n; j++) {
+            const std::complex<double> v = h == 0 ? std::conj(c[j]) : c[j];
+            b[j] = v;
+            if (j) b[m - j] = v;
+        }
+        fft_pow2_f64(b, -1);
+        for (int k = 0; k < m; k++)
+            bchirp[(size_t)h * m + k] = make_float2((float)(b[k].real() / m), (float)(b[k].imag() / m));
+    }
+}
+
 static int build_tables(swem_plan* p, const float* wav)
 {
     const swem_geometry& g = p->g;
     const float* bsrc = nullptr;
-    const int nmx = g.nmx, nz = g.nz, nm = p->nm, nf = p->nf, nt = g.nt, nkx = p->nkx;
-    // twiddles exp(-2 pi i m / nkx) in FP64
-    std::vector<float2> tw(nkx);
-    for (int m = 0; m < nkx; m++) {
-        double a = -2.0 * kPi * (double)m / (double)nkx;
+    const int nmx = g.nmx, nz = g.nz, nm = p->nm, nf = p->nf, nt = g.nt, nkx = p->nkx, narr = p->narr;
+    // twiddles exp(-2 pi i m / narr) of the transform array length in FP64
+    std::vector<float2> tw(narr);
+    for (int m = 0; m < narr; m++) {
+        double a = -2.0 * kPi * (double)m / (double)narr;
         tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
     SWEM_TRY(p->tw.upload(tw.data(), tw.size()));
+    if (narr != nkx) {
+        std::vector<float2> chirp, bchirp;
+        bluestein_tables(nkx, narr, chirp, bchirp);
+        SWEM_TRY(p->chirp.upload(chirp.data(), chirp.size()));
+        SWEM_TRY(p->bchirp.upload(bchirp.data(), bchirp.size()));
+    }
     // taper (wem.c:546-564)
     std::vector<float> tap(nmx);
     taper_f32(nmx, p->s.lmx, tap.data());
@@ -216,9 +274,9 @@ static int build_tables(swem_plan* p, const float* wav)
                 p2f[(size_t)iz * nref + ir] = 1 / (vf * vf);
             }
         }
-        // 2-D: [nz][nkx], rows padded to the transform length (0x7f: no vref at x >= nmx);
+        // 2-D: [nz][narr], rows padded to the transform array length (0x7f: no vref at x >= nmx);
         // 3-D: [nz][nm], trace ix = imx * nmy + imy
-        const size_t srow = p->three_d ? (size_t)nm : (size_t)nkx;
+        const size_t srow = p->three_d ? (size_t)nm : (size_t)narr;
         std::vector<uint8_t> sel((size_t)nz * srow, 0x7f), sel2((size_t)nz * srow, 0x7f);
         std::vector<float> w2((size_t)nz * srow, 0.f);
         std::vector<uint32_t> used(nz, 0u);
@@ -245,7 +303,7 @@ static int build_tables(swem_plan* p, const float* wav)
         // Per depth: vref of the uncut points (-1: none), their count and the first kUncutSparse
         // positions.
         std::vector<int> cref((size_t)nz * kCrefStride, 0);
-        bool packed = p->opt.pspi_mode == SWEM_PSPI_COMPAT && !p->three_d;
+        bool packed = p->opt.pspi_mode == SWEM_PSPI_COMPAT && !p->three_d && narr == nkx;
         for (int iz = 0; iz < nz; iz++) {
             const float z = g.oz + g.dz * (float)iz;
             if (!(z >= g.sz && z >= g.gz)) packed = false;
@@ -414,7 +472,10 @@ static March2DArgs base_args(const swem_plan* p, int c, int F)
     a.sz = p->g.sz;
     a.gz = p->g.gz;
     a.inv_nk = 1.0f / (float)p->s.nkx;
+    a.nkx = p->nkx;
     a.tw = p->tw.p;
+    a.chirp = p->chirp.p;
+    a.bchirp = p->bchirp.p;
     a.ktab = p->ktab.p;
     a.stab = p->stab.p;
     a.spec_s = p->spec_s.p;
@@ -825,6 +886,8 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
     p->nf = p->s.ifmax - p->s.ifmin;
     p->nkx = p->s.nkx;
     p->three_d = g->nmy > 1;
+    p->narr = p->nkx;
+    if (!p->three_d && march2d_blue_len(p->nkx)) p->narr = march2d_blue_len(p->nkx);
     if (p->three_d) {
         if (g->pspi && o.pspi_mode != SWEM_PSPI_COMPAT)
             return set_error(SWEM_EUNSUPPORTED, "3-D PSPI: the linear mode is not available in this build (compat is)");
@@ -832,7 +895,8 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
             return set_error(SWEM_EUNSUPPORTED, "3-D: nkx=%d, nky=%d: this build handles powers of two 128..1024", p->s.nkx,
                              p->s.nky);
     } else if (!march2d_supported(p->nkx)) {
-        return set_error(SWEM_EUNSUPPORTED, "nkx = padx*nmx = %d: this build handles powers of two 128..4096", p->nkx);
+        return set_error(SWEM_EUNSUPPORTED, "nkx = padx*nmx = %d: this build handles 2..2048 and powers of two to 4096",
+                         p->nkx);
     }
     if (p->nf < 1) return set_error(SWEM_EINVAL, "empty frequency band [ifmin=%d, ifmax=%d)", p->s.ifmin, p->s.ifmax);
     if (g->pspi && (g->nref < 2 || g->nref > 31)) return set_error(SWEM_EINVAL, "PSPI needs 2 <= nref <= 31 (nref=%d)", g->nref);
@@ -876,7 +940,7 @@ void plan_free(swem_plan* p)
     for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
     if (p->igx_pinned) (void)hipHostFree(p->igx_pinned);
     p->twy.release(); p->taper_y.release(); p->W.release(); p->wavd.release();
-    p->tw.release(); p->spec_s.release(); p->ktab.release(); p->stab.release(); p->pd.release(); p->pdf.release();
+    p->tw.release(); p->chirp.release(); p->bchirp.release(); p->spec_s.release(); p->ktab.release(); p->stab.release(); p->pd.release(); p->pdf.release();
     p->taper.release(); p->p2f.release(); p->w2.release();
     p->own_bases.adj.release(); p->own_bases.fwd.release(); p->own_bases.src.release();
     p->p2.release(); p->sel.release(); p->sel2.release();
@@ -896,14 +960,15 @@ static size_t pad256(size_t b) { return (b + 511) & ~(size_t)255; }
 size_t plan_upload_bytes(const swem_plan* p, int nshot)
 {
     const size_t nz = p->g.nz, nm = p->nm, nmx = p->g.nmx, nref = p->g.pspi ? p->g.nref : 1;
-    size_t b = pad256(p->nkx * 8) + pad256(p->s.nky * 8) + pad256(nmx * 4) + pad256(p->g.nmy * 4) + pad256(p->g.nt * 4);
+    size_t b = pad256(p->narr * 8) + pad256(p->s.nky * 8) + pad256(nmx * 4) + pad256(p->g.nmy * 4) + pad256(p->g.nt * 4);
     b += pad256(nz * nref * 8) + pad256(nz * nref * 4) + pad256(nz * 4);
     if (p->g.pspi) {
-        const size_t srow = p->three_d ? nm : (size_t)p->nkx;
+        const size_t srow = p->three_d ? nm : (size_t)p->narr;
         b += 2 * pad256(nz * srow) + pad256(nz * srow * 4) + pad256(nz * kCrefStride * 4) + pad256(nz * 4);
     }
     else
         b += pad256(nz * nm * 8) + pad256(nz * nm * 4);
+    if (p->narr != p->nkx) b += pad256(p->nkx * 8) + pad256(p->narr * 16);
     return b + pad256((size_t)nshot * 4);
 }
 
@@ -1002,7 +1067,7 @@ int swem_plan_bytes(const swem_plan* p, size_t* table_bytes, size_t* work_bytes)
         *table_bytes = p->tw.bytes() + p->spec_s.bytes() + p->ktab.bytes() + p->stab.bytes() + p->pd.bytes() +
                        p->pdf.bytes() + p->taper.bytes() + p->p2f.bytes() +
                        p->w2.bytes() + p->own_bases.adj.bytes() + p->own_bases.fwd.bytes() + p->own_bases.src.bytes() + p->wavd.bytes() + p->p2.bytes() + p->sel.bytes() +
-                       p->sel2.bytes() + p->used.bytes();
+                       p->sel2.bytes() + p->used.bytes() + p->chirp.bytes() + p->bchirp.bytes();
     if (work_bytes)
         *work_bytes = p->spec_r.bytes() + p->smig.bytes() + p->dgw.bytes() + p->partial.bytes() + p->mT.bytes() +
                       p->igx.bytes();

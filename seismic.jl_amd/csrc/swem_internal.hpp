@@ -66,7 +66,10 @@ struct March2DArgs {
     int f0, nfc, nkh;
     float oz, dz, sz, gz;
     float inv_nk;           // 1 / nkx (identity steps)
-    const float2* tw;       // [nkx] exp(-2 pi i m / nkx)
+    int nkx;                // transform length
+    const float2* tw;       // [nkx] exp(-2 pi i m / nkx); Bluestein: [M] of the array length M
+    const float2* chirp;    // Bluestein: [nkx] exp(-i pi j^2 / nkx)
+    const float2* bchirp;   // Bluestein: [2][M] FFT_M of the wrapped conj chirp / M (SIGN -1, +1)
     const float2* ktab;     // [nfc][nz][nref][nkh] kz phase / nk, +dz
     const float2* stab;     // [nfc][nz][nmx] split-step phase, +dz (SSF)
     const float2* spec_s;   // [nf] source spectrum / sqrt(ntfft)
@@ -97,9 +100,13 @@ struct March2DShape {
     int per_freq;   // subgroups per frequency
     int max_block;  // largest workgroup
     int bufs;       // LDS buffers per subgroup
+    int blue;       // Bluestein array length (0: nkx is a supported power of two)
 };
 March2DShape march2d_shape(int nkx, int variant, int adj);
 bool march2d_supported(int nkx);
+// Array length of the Bluestein transform for nkx (0: a power-of-two kernel takes nkx, or
+// nkx is out of range).
+int march2d_blue_len(int nkx);
 size_t march2d_lds_bytes(int nkx, const March2DShape& sh, int F);
 
 // Arguments of the 3-D split-step kernels (swem_march3d.hip).  A launch covers the items

@@ -114,6 +114,75 @@ __device__ __forceinline__ void xfft(float2 (&v)[P], float2* lds, const float2* 
         fft<N, P, SIGN>(v, lds, tw, t, pp);
 }
 
+// Transform policies of the step functions.  XPow2: the transform length is the register array
+// length N (a power of two).  XBlue: a transform of runtime length n = nkx (any length) through
+// two power-of-two transforms of the array length M >= 2 n - 1 (Bluestein): with
+// w_j = exp(SIGN i pi j^2 / n),  X_k = w_k sum_j (x_j w_j) conj(w_(k-j)),  the sum a cyclic
+// convolution of length M taken as IFFT_M(FFT_M(x w) B) with B = FFT_M(conj w, wrapped) / M
+// built in FP64.  Entries j, k >= n of the array are zero on input and on output.  The PSPI
+// selection tables keep the array length as their row stride (0x7f padding beyond nmx) in both.
+template <int N, int P>
+struct XPow2 {
+    template <int SIGN, bool WAVE>
+    __device__ __forceinline__ static void fft(float2 (&v)[P], float2* lds, const March2DArgs& a, int t, int& pp)
+    {
+        xfft<N, P, SIGN, WAVE>(v, lds, a.tw, t, pp);
+    }
+    __device__ __forceinline__ static void kload_row(float2 (&L)[P], const float2* __restrict__ row, const March2DArgs&, int t)
+    {
+        swem::kload_row<N, P>(L, row, t);
+    }
+};
+
+template <int M, int P>
+struct XBlue {
+    static constexpr int T = M / P;
+    template <int SIGN, bool WAVE>
+    __device__ __forceinline__ static void fft(float2 (&v)[P], float2* lds, const March2DArgs& a, int t, int& pp)
+    {
+        static_assert(!WAVE, "Bluestein transforms use the subgroup FFT");
+        chirp_mul<SIGN>(v, a, t);
+        swem::fft<M, P, -1>(v, lds, a.tw, t, pp);
+        const float2* B = a.bchirp + (SIGN < 0 ? 0 : M);
+        {
+            float2 bq[P];
+#pragma unroll
+            for (int q = 0; q < P; ++q) bq[q] = B[t + q * T];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < P; ++q) v[q] = cmul(v[q], bq[q]);
+        }
+        swem::fft<M, P, +1>(v, lds, a.tw, t, pp);
+        chirp_mul<SIGN>(v, a, t);
+    }
+    // v_j <- v_j w_j for j < n, 0 beyond (the chirp is reloaded rather than kept in registers)
+    template <int SIGN>
+    __device__ __forceinline__ static void chirp_mul(float2 (&v)[P], const March2DArgs& a, int t)
+    {
+        const int n = a.nkx;
+        float2 w[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int j = t + q * T;
+            const float2 c = j < n ? a.chirp[j] : make_float2(0.f, 0.f);  // exp(-i pi j^2 / n)
+            w[q] = SIGN < 0 ? c : make_float2(c.x, -c.y);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = cmul(v[q], w[q]);
+    }
+    __device__ __forceinline__ static void kload_row(float2 (&L)[P], const float2* __restrict__ row, const March2DArgs& a, int t)
+    {
+        const int n = a.nkx;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int k = t + q * T;
+            L[q] = row[k < n ? (k <= n / 2 ? k : n - k) : 0];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+};
+
 struct StepCtx {
     int t, iz, fc;  // thread within the subgroup, depth index, frequency index within the chunk
                     // (fc and iz are wave-uniform: table rows are addressed from scalar registers)
@@ -133,22 +202,22 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t ss_row(const March2DArgs& a, c
 // ssop adjoint branch (wem.c:313-320, 334-359) on one wavefield.  ACT = false turns the step
 // into FFT / IFFT with unit phases (a numerical identity) so every subgroup of a workgroup runs
 // the same barriers when only one of the two wavefields is stepped at a depth.
-template <int N, int P, bool WAVE, bool ACT>
+template <int N, int P, bool WAVE, bool ACT, class X = XPow2<N, P>>
 __device__ __forceinline__ void ssf_adj_step(float2 (&u)[P], float2* lds, int& pp, const March2DArgs& a,
                                              const StepCtx& c, bool conj, const float (&tap)[P])
 {
     constexpr int T = N / P;
-    xfft<N, P, -1, WAVE>(u, lds, a.tw, c.t, pp);
+    X::template fft<-1, WAVE>(u, lds, a, c.t, pp);
     if constexpr (ACT) {
         float2 L[P];
-        kload_row<N, P>(L, krow(a, c, 0), c.t);
+        X::kload_row(L, krow(a, c, 0), a, c.t);
 #pragma unroll
         for (int q = 0; q < P; ++q) u[q] = cmul_c(u[q], L[q], conj);
     } else {
 #pragma unroll
         for (int q = 0; q < P; ++q) u[q] = cscale(u[q], a.inv_nk);
     }
-    xfft<N, P, +1, WAVE>(u, lds, a.tw, c.t, pp);
+    X::template fft<+1, WAVE>(u, lds, a, c.t, pp);
     if constexpr (ACT) {
         const auto sr = ss_row(a, c);
         float2 sp[P];
@@ -167,7 +236,7 @@ __device__ __forceinline__ void ssf_adj_step(float2 (&u)[P], float2* lds, int& p
 
 // ssop forward branch with dz < 0 (wem.c:321-332, 334-348, 362-369): taper, split-step
 // phase, FFT, kz phase, IFFT.
-template <int N, int P, bool WAVE>
+template <int N, int P, bool WAVE, class X = XPow2<N, P>>
 __device__ __forceinline__ void ssf_fwd_step(float2 (&u)[P], float2* lds, int& pp, const March2DArgs& a,
                                              const StepCtx& c, const float (&tap)[P])
 {
@@ -181,14 +250,14 @@ __device__ __forceinline__ void ssf_fwd_step(float2 (&u)[P], float2* lds, int& p
 #pragma unroll
         for (int q = 0; q < P; ++q) u[q] = cmulc(cscale(u[q], tap[q]), sp[q]);
     }
-    xfft<N, P, -1, WAVE>(u, lds, a.tw, c.t, pp);
+    X::template fft<-1, WAVE>(u, lds, a, c.t, pp);
     {
         float2 L[P];
-        kload_row<N, P>(L, krow(a, c, 0), c.t);
+        X::kload_row(L, krow(a, c, 0), a, c.t);
 #pragma unroll
         for (int q = 0; q < P; ++q) u[q] = cmulc(u[q], L[q]);
     }
-    xfft<N, P, +1, WAVE>(u, lds, a.tw, c.t, pp);
+    X::template fft<+1, WAVE>(u, lds, a, c.t, pp);
 #pragma unroll
     for (int q = 0; q < P; ++q)
         if (c.t + q * T >= a.nmx) u[q] = make_float2(0.f, 0.f);
@@ -209,12 +278,12 @@ __device__ __forceinline__ float pspi_linear_weight(int ir, uint8_t s1, uint8_t 
 // pspiop with the reference structure (wem.c:409-460): FFT, for each vref phase + IFFT, then
 // per-x selection.  Compat: u = Re(dref[iref1]) when cut else dref[iref1]; linear: weights.
 // The selection rows are padded to N columns (0x7f beyond nmx: no vref selected).
-template <int N, int P, int MODE, bool WAVE, bool ACT>
+template <int N, int P, int MODE, bool WAVE, bool ACT, class X = XPow2<N, P>>
 __device__ __forceinline__ void pspi_select_step(float2 (&u)[P], float2* lds, int& pp, const March2DArgs& a,
                                                  const StepCtx& c, bool conj)
 {
     constexpr int T = N / P;
-    xfft<N, P, -1, WAVE>(u, lds, a.tw, c.t, pp);
+    X::template fft<-1, WAVE>(u, lds, a, c.t, pp);
     float2 U[P];
     uint8_t s1[P], s2[P];
     float w2[P];
@@ -235,14 +304,14 @@ __device__ __forceinline__ void pspi_select_step(float2 (&u)[P], float2* lds, in
         if (!((used >> ir) & 1u)) continue;
         float2 B[P];
         if constexpr (ACT) {
-            kload_row<N, P>(B, krow(a, c, ir), c.t);
+            X::kload_row(B, krow(a, c, ir), a, c.t);
 #pragma unroll
             for (int q = 0; q < P; ++q) B[q] = cmul_c(u[q], B[q], conj);
         } else {
 #pragma unroll
             for (int q = 0; q < P; ++q) B[q] = cscale(u[q], a.inv_nk);
         }
-        xfft<N, P, +1, WAVE>(B, lds, a.tw, c.t, pp);
+        X::template fft<+1, WAVE>(B, lds, a, c.t, pp);
 #pragma unroll
         for (int q = 0; q < P; ++q) {
             if constexpr (!ACT) {
@@ -260,7 +329,7 @@ __device__ __forceinline__ void pspi_select_step(float2 (&u)[P], float2* lds, in
 
 // Exact adjoint of the linear select-after step, for the forward receiver (dz < 0):
 // u <- IFFT( sum_ir conj(L_ir) FFT(W_ir u) ).
-template <int N, int P, bool WAVE>
+template <int N, int P, bool WAVE, class X = XPow2<N, P>>
 __device__ __forceinline__ void pspi_weight_step(float2 (&u)[P], float2* lds, int& pp, const March2DArgs& a,
                                                  const StepCtx& c)
 {
@@ -283,48 +352,48 @@ __device__ __forceinline__ void pspi_weight_step(float2 (&u)[P], float2* lds, in
         float2 B[P];
 #pragma unroll
         for (int q = 0; q < P; ++q) B[q] = cscale(u[q], pspi_linear_weight(ir, s1[q], s2[q], w2[q]));
-        xfft<N, P, -1, WAVE>(B, lds, a.tw, c.t, pp);
+        X::template fft<-1, WAVE>(B, lds, a, c.t, pp);
         const float2* row = krow(a, c, ir);
         float2 L[P];
-        kload_row<N, P>(L, row, c.t);
+        X::kload_row(L, row, a, c.t);
 #pragma unroll
         for (int q = 0; q < P; ++q) acc[q] = cadd(acc[q], cmulc(B[q], L[q]));
     }
-    xfft<N, P, +1, WAVE>(acc, lds, a.tw, c.t, pp);
+    X::template fft<+1, WAVE>(acc, lds, a, c.t, pp);
 #pragma unroll
     for (int q = 0; q < P; ++q) u[q] = (c.t + q * T < a.nmx) ? acc[q] : make_float2(0.f, 0.f);
 }
 
 // Source / adjoint-receiver step of either method (select-after for PSPI).  act is uniform
 // over the subgroup: the branch is taken once per step, not per element.
-template <int N, int P, int MODE, bool WAVE>
+template <int N, int P, int MODE, bool WAVE, class X = XPow2<N, P>>
 __device__ __forceinline__ void adj_step(float2 (&u)[P], float2* lds, int& pp, const March2DArgs& a, const StepCtx& c,
                                          bool act, bool conj, const float (&tap)[P])
 {
     if constexpr (MODE == kModeSSF) {
         if (act)
-            ssf_adj_step<N, P, WAVE, true>(u, lds, pp, a, c, conj, tap);
+            ssf_adj_step<N, P, WAVE, true, X>(u, lds, pp, a, c, conj, tap);
         else
-            ssf_adj_step<N, P, WAVE, false>(u, lds, pp, a, c, conj, tap);
+            ssf_adj_step<N, P, WAVE, false, X>(u, lds, pp, a, c, conj, tap);
     } else {
         if (act)
-            pspi_select_step<N, P, MODE, WAVE, true>(u, lds, pp, a, c, conj);
+            pspi_select_step<N, P, MODE, WAVE, true, X>(u, lds, pp, a, c, conj);
         else
-            pspi_select_step<N, P, MODE, WAVE, false>(u, lds, pp, a, c, conj);
+            pspi_select_step<N, P, MODE, WAVE, false, X>(u, lds, pp, a, c, conj);
     }
 }
 
 // Forward-receiver step (dz < 0).
-template <int N, int P, int MODE, bool WAVE>
+template <int N, int P, int MODE, bool WAVE, class X = XPow2<N, P>>
 __device__ __forceinline__ void fwd_step(float2 (&u)[P], float2* lds, int& pp, const March2DArgs& a, const StepCtx& c,
                                          const float (&tap)[P])
 {
     if constexpr (MODE == kModeSSF)
-        ssf_fwd_step<N, P, WAVE>(u, lds, pp, a, c, tap);
+        ssf_fwd_step<N, P, WAVE, X>(u, lds, pp, a, c, tap);
     else if constexpr (MODE == kModePspiCompat)
-        pspi_select_step<N, P, MODE, WAVE, true>(u, lds, pp, a, c, true);
+        pspi_select_step<N, P, MODE, WAVE, true, X>(u, lds, pp, a, c, true);
     else
-        pspi_weight_step<N, P, WAVE>(u, lds, pp, a, c);
+        pspi_weight_step<N, P, WAVE, X>(u, lds, pp, a, c);
 }
 
 template <int N, int P, int MODE>
@@ -430,7 +499,7 @@ __device__ __forceinline__ void adj_body(const March2DArgs& a)
 }
 
 // Modelling march: NSUB subgroups, one frequency each, no interaction between subgroups.
-template <int N, int P, int MODE, bool WAVE>
+template <int N, int P, int MODE, bool WAVE, class X = XPow2<N, P>>
 __device__ __forceinline__ void fwd_body(const March2DArgs& a)
 {
     constexpr int T = N / P;
@@ -466,7 +535,7 @@ __device__ __forceinline__ void fwd_body(const March2DArgs& a)
         c.iz = iz;
         const float z = a.oz + a.dz * (float)iz;
         const bool ds = z >= a.sz;
-        if (ds) adj_step<N, P, MODE, WAVE>(u, lds, pp, a, c, true, true, tap);
+        if (ds) adj_step<N, P, MODE, WAVE, X>(u, lds, pp, a, c, true, true, tap);
         if (active) {
             const auto sr = buf_rsrc(sm + (size_t)iz * a.nmx, (uint32_t)a.nmx * 8u);
 #pragma unroll
@@ -496,7 +565,7 @@ __device__ __forceinline__ void fwd_body(const March2DArgs& a)
 #pragma unroll
             for (int q = 0; q < P; ++q) u[q] = cadd(u[q], cscale(sv[q], mv[q]));
         }
-        fwd_step<N, P, MODE, WAVE>(u, lds, pp, a, c, tap);
+        fwd_step<N, P, MODE, WAVE, X>(u, lds, pp, a, c, tap);
     }
     if (active) {
         const auto dr = buf_rsrc(a.dgw + (size_t)shot * a.nmx * a.nf, (uint32_t)a.nmx * a.nf * 8u);
@@ -508,7 +577,7 @@ __device__ __forceinline__ void fwd_body(const March2DArgs& a)
 // Migration march with one subgroup per frequency that steps the source, then the receiver,
 // through the same LDS buffers (variant PAIR).  One frequency per workgroup; the imaging
 // condition is register-local.
-template <int N, int P, int MODE, bool WAVE>
+template <int N, int P, int MODE, bool WAVE, class X = XPow2<N, P>>
 __device__ __forceinline__ void adj_body_pair(const March2DArgs& a)
 {
     constexpr int T = N / P;
@@ -541,8 +610,8 @@ __device__ __forceinline__ void adj_body_pair(const March2DArgs& a)
         c.iz = iz;
         const float z = a.oz + a.dz * (float)iz;
         const bool ds = z >= a.sz, dr = z >= a.gz;
-        if (ds) adj_step<N, P, MODE, WAVE>(us, smem, pp, a, c, true, true, tap);
-        if (dr) adj_step<N, P, MODE, WAVE>(ur, smem, pp, a, c, true, false, tap);
+        if (ds) adj_step<N, P, MODE, WAVE, X>(us, smem, pp, a, c, true, true, tap);
+        if (dr) adj_step<N, P, MODE, WAVE, X>(ur, smem, pp, a, c, true, false, tap);
         // imaging condition (wem.c:243-247)
         const auto pr = buf_rsrc(part + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
 #pragma unroll
@@ -861,6 +930,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ma
     adj_body_pair<N, N / 64, MODE, true>(a);
 }
 
+// Transform lengths that are not powers of two: Bluestein through the array length M (one
+// subgroup of M/P threads marches both wavefields of a frequency in migration, as PAIR does;
+// modelling takes F frequencies per workgroup of at most 512 threads).  P = 8 from M = 1024:
+// the chirp products add live registers, and 16 points per thread would spill.
+template <int M, int MODE, int P>
+__global__ __launch_bounds__(M / P) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_blue(March2DArgs a)
+{
+    adj_body_pair<M, P, MODE, false, XBlue<M, P>>(a);
+}
+
+template <int M, int MODE, int P>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void march2d_fwd_blue(March2DArgs a)
+{
+    fwd_body<M, P, MODE, false, XBlue<M, P>>(a);
+}
+
 // ------------------------------------------------------------------------ dispatch
 // Kernel shape of a variant: threads per wavefield subgroup T, subgroups per frequency
 // (2 = source and receiver in separate subgroups, 1 = one subgroup marches both), the largest
@@ -874,9 +959,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ma
 //   PACKED       PAIR's shape; PSPI compat migration with both wavefields in one transform
 //                (the host selects it only where every depth steps both wavefields and the
 //                uncut points of a depth share one vref; other modes run PAIR).
+//   Bluestein    any other transform length up to 2048: the PAIR body (migration) or the
+//                subgroup body (modelling) over the array length M = blue (P = 8 from M = 1024,
+//                one wave below); the variant option does not apply.
+int march2d_blue_len(int nkx)
+{
+    if (nkx < 2 || nkx > 2048 || (nkx >= 128 && !(nkx & (nkx - 1)))) return 0;
+    int m = 128;
+    while (m < 2 * nkx - 1) m <<= 1;
+    return m;
+}
+
 March2DShape march2d_shape(int nkx, int variant, int adj)
 {
     March2DShape sh{};
+    if (const int m = march2d_blue_len(nkx)) {
+        sh.blue = m;
+        sh.variant = adj ? SWEM_VARIANT_PAIR : SWEM_VARIANT_SUBGROUP;
+        sh.T = m >= 1024 ? m / 8 : 64;
+        sh.per_freq = 1;
+        sh.max_block = adj ? sh.T : 512;
+        sh.bufs = 2;
+        return sh;
+    }
     if (nkx < 128 || nkx > 4096 || (nkx & (nkx - 1))) return sh;
     int v = variant;
     if ((v == SWEM_VARIANT_PAIR || v == SWEM_VARIANT_PAIR8 || v == SWEM_VARIANT_PACKED) && !adj) v = SWEM_VARIANT_AUTO;
@@ -911,6 +1016,7 @@ bool march2d_supported(int nkx) { return march2d_shape(nkx, SWEM_VARIANT_AUTO, 1
 
 size_t march2d_lds_bytes(int nkx, const March2DShape& sh, int F)
 {
+    if (sh.blue) nkx = sh.blue;
     const size_t len = (size_t)(((nkx + nkx / 8 + 31) & ~31) + 4);  // LdsLen<nkx>
     return (size_t)F * sh.per_freq * sh.bufs * len * sizeof(float2);
 }
@@ -967,6 +1073,24 @@ static int launch_one(const March2DArgs& a, const March2DShape& sh, int F, int a
                : launch_kernel(march2d_fwd<N, MODE, P>, a, block, lds, st);
 }
 
+template <int M, int MODE>
+static int launch_blue_mode(const March2DArgs& a, const March2DShape& sh, int F, int adj, hipStream_t st)
+{
+    constexpr int P = M >= 1024 ? 8 : M / 64;
+    const size_t lds = march2d_lds_bytes(a.nkx, sh, F);
+    dim3 block(sh.T * sh.per_freq * F);
+    return adj ? launch_kernel(march2d_adj_blue<M, MODE, P>, a, block, lds, st)
+               : launch_kernel(march2d_fwd_blue<M, MODE, P>, a, block, lds, st);
+}
+
+template <int M>
+static int launch_blue(const March2DArgs& a, const March2DShape& sh, int F, int mode, int adj, hipStream_t st)
+{
+    if (mode == kModeSSF) return launch_blue_mode<M, kModeSSF>(a, sh, F, adj, st);
+    if (mode == kModePspiCompat) return launch_blue_mode<M, kModePspiCompat>(a, sh, F, adj, st);
+    return launch_blue_mode<M, kModePspiLinear>(a, sh, F, adj, st);
+}
+
 template <int N>
 static int launch_n(const March2DArgs& a, const March2DShape& sh, int F, int mode, int adj, hipStream_t st)
 {
@@ -979,7 +1103,8 @@ static int launch_n(const March2DArgs& a, const March2DShape& sh, int F, int mod
 int launch_march2d(const March2DArgs& a, int nkx, int F, int mode, int adj, int variant, hipStream_t st)
 {
     const March2DShape sh = march2d_shape(nkx, variant, adj);
-    if (sh.T <= 0) return set_error(SWEM_EUNSUPPORTED, "2-D march: nkx=%d is not one of 128..4096 (powers of two)", nkx);
+    if (sh.T <= 0)
+        return set_error(SWEM_EUNSUPPORTED, "2-D march: nkx=%d is neither a power of two 128..4096 nor 2..2048", nkx);
     if (F < 1 || sh.T * sh.per_freq * F > sh.max_block)
         return set_error(SWEM_EINVAL, "2-D march: %d frequencies of %d subgroups of %d threads exceed %d", F,
                          sh.per_freq, sh.T, sh.max_block);
@@ -988,6 +1113,17 @@ int launch_march2d(const March2DArgs& a, int nkx, int F, int mode, int adj, int 
         return set_error(SWEM_EINVAL, "2-D march: inconsistent launch (nmx=%d nkx=%d nref=%d)", a.nmx, nkx, a.nref);
     if (march2d_lds_bytes(nkx, sh, F) > 160 * 1024)
         return set_error(SWEM_EINVAL, "2-D march: LDS for %d frequencies at nkx=%d exceeds 160 KiB", F, nkx);
+    if (sh.blue) {
+        if (a.nkx != nkx || !a.chirp || !a.bchirp) return set_error(SWEM_EINVAL, "2-D march: Bluestein tables missing");
+        switch (sh.blue) {
+        case 128: return launch_blue<128>(a, sh, F, mode, adj, st);
+        case 256: return launch_blue<256>(a, sh, F, mode, adj, st);
+        case 512: return launch_blue<512>(a, sh, F, mode, adj, st);
+        case 1024: return launch_blue<1024>(a, sh, F, mode, adj, st);
+        case 2048: return launch_blue<2048>(a, sh, F, mode, adj, st);
+        case 4096: return launch_blue<4096>(a, sh, F, mode, adj, st);
+        }
+    }
     switch (nkx) {
     case 128: return launch_n<128>(a, sh, F, mode, adj, st);
     case 256: return launch_n<256>(a, sh, F, mode, adj, st);
