@@ -271,6 +271,41 @@ def bench_lsm(args):
     print(json.dumps(out))
 
 
+def plain_single_shot(args, g, vel, wav, sx, d, plan, batched):
+    """The single-shot paths beside the batched launch: (a) the plain drop-in, one shot per
+    wem() call (host arrays; the plan, its phase tables, the uploads, the march, the download
+    and the frees are all inside each call, as one shotwem run per shot does), and (b) one shot
+    per launch on a resident plan with the record already on the device."""
+    import seiswem
+
+    k = min(args.plain_shots, int(sx.size))
+    d_host = d.download()[:k]
+    seiswem.ShotProfileWEM(d_host[:1], True, vel=vel, wav=wav, geom=g, sx=sx[:1])  # warm-up
+    seiswem.synchronize()
+    t0 = time.perf_counter()
+    for i in range(k):
+        seiswem.ShotProfileWEM(d_host[i:i + 1], True, vel=vel, wav=wav, geom=g, sx=sx[i:i + 1])
+    plain = k / (time.perf_counter() - t0)
+    recs = [seiswem.DeviceArray((1, g.nm, g.nt), d_host[i:i + 1]) for i in range(k)]
+    img = seiswem.DeviceArray((g.nm, g.nz))
+    plan.migrate(recs[0], sx[:1], m=img)
+    seiswem.synchronize()
+    t0 = time.perf_counter()
+    for i in range(k):
+        plan.migrate(recs[i], sx[i:i + 1], m=img)
+    seiswem.synchronize()
+    resident = k / (time.perf_counter() - t0)
+    return {
+        "plain_single_shot": {"shots_per_s": plain, "sample_shots": k,
+                              "path": "ShotProfileWEM per shot on host arrays: plan, phase tables, upload, "
+                                      "march, download, free in every call"},
+        "single_shot_resident_plan": {"shots_per_s": resident, "sample_shots": k,
+                                      "path": "one shot per launch, plan and record resident in HBM"},
+        "batched_vs_plain_single_shot": batched / plain,
+        "batched_vs_single_shot_resident": batched / resident,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,6 +315,8 @@ def main():
     ap.add_argument("--freqs-per-block", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0, help="2-D kernel variant (SWEM_VARIANT_*, 0 = automatic)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-shots", type=int, default=4,
+                    help="pspi2d: shots timed one per call on the plain and resident single-shot paths (0: skip)")
     ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm"), default="pspi2d")
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrival rate (requests/s)")
@@ -349,6 +386,8 @@ def main():
         "roofline": roofline,
         "setup_model_s": t_model,
     }
+    if args.plain_shots > 0:
+        out.update(plain_single_shot(args, g, vel, wav, sx, d, plan, value))
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
         os.environ.setdefault("OMP_NUM_THREADS", str(cores))

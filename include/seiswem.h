@@ -227,11 +227,14 @@ int swem_synchronize(void);
 /* Device vector operations of the least-squares solver, for vectors resident in device memory
  * (models [nm][nz], records [nshot][nm][nt]):
  *   swem_vec_cgstep  m1 = a*m1 + b*m2                       (CGStep, src/Solvers/CGStep.jl:1-4)
+ *   swem_vec_mul     y = x .* w, y may be x                  (WeightingOp, src/Operators/WeightingOp.jl:1-5:
+ *                    the data weights and mute of ShotProfileLSWEM, src/Imaging/ShotProfileLSWEM.jl:12-15)
  *   swem_vec_dot     *out = sum x*y accumulated in double in an order that depends only on n
  *                    (InnerProduct, src/Tools/InnerProduct.jl:1-5, whose in-memory form rounds to
  *                    Float32); waits for the stream. */
 int swem_vec_cgstep(float a, float* m1, float b, const float* m2, size_t n, void* stream);
 int swem_vec_dot(const float* x, const float* y, size_t n, double* out, void* stream);
+int swem_vec_mul(const float* x, const float* w, float* y, size_t n, void* stream);
 
 /* ------------------------------------------- discrete decisions (host only, no GPU) */
 

@@ -1180,6 +1180,12 @@ int swem_vec_cgstep(float a, float* m1, float b, const float* m2, size_t n, void
     return launch_vec_cgstep(a, m1, b, m2, n, (hipStream_t)stream);
 }
 
+int swem_vec_mul(const float* x, const float* w, float* y, size_t n, void* stream)
+{
+    if ((!x || !w || !y) && n) return set_error(SWEM_EINVAL, "vec_mul: null vector");
+    return launch_vec_mul(x, w, y, n, (hipStream_t)stream);
+}
+
 int swem_vec_dot(const float* x, const float* y, size_t n, double* out, void* stream)
 {
     if (!out || ((!x || !y) && n)) return set_error(SWEM_EINVAL, "vec_dot: null argument");

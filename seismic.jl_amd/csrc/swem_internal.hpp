@@ -161,6 +161,7 @@ int launch_reduce_image(const float* partial, int nshot, int ngroups, int nz, in
 // Solver vector operations: m1 = a m1 + b m2; <x, y> in double into part[kDotBlocks] (part has
 // vec_dot_scratch_doubles() entries).
 int launch_vec_cgstep(float a, float* m1, float b, const float* m2, size_t n, hipStream_t st);
+int launch_vec_mul(const float* x, const float* w, float* y, size_t n, hipStream_t st);
 int launch_vec_dot(const float* x, const float* y, size_t n, double* part, hipStream_t st);
 int vec_dot_scratch_doubles();
 // One depth of an image: out[x * ldo] (+)= sum_g partial[g][x] (compensated, fixed order).

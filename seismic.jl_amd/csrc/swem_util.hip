@@ -119,6 +119,22 @@ __global__ __launch_bounds__(256) void vec_cgstep_k(float a, float* __restrict__
         m1[i] = a * m1[i] + b * m2[i];
 }
 
+// y = x w elementwise (a diagonal operator: WeightingOp.jl:1-5; y may be x)
+__global__ __launch_bounds__(256) void vec_mul_k(const float* x, const float* __restrict__ w, float* y, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = x[i] * w[i];
+}
+
+int launch_vec_mul(const float* x, const float* w, float* y, size_t n, hipStream_t st)
+{
+    if (!n) return SWEM_OK;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(vec_mul_k, dim3(blocks), dim3(256), 0, st, x, w, y, n);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
 int launch_vec_cgstep(float a, float* m1, float b, const float* m2, size_t n, hipStream_t st)
 {
     if (!n) return SWEM_OK;

@@ -28,7 +28,7 @@ EXPORTS = (
     "swem_sampling", "swem_source_index", "swem_pspi_tables", "swem_taper",
     "swem_seis_extent_init", "swem_seis_read_header", "swem_seis_read", "swem_seis_write",
     "swem_server_config_init", "swem_server_create", "swem_server_destroy", "swem_batch_submit",
-    "swem_batch_poll", "swem_batch_wait", "swem_server_get_stats", "swem_vec_cgstep", "swem_vec_dot",
+    "swem_batch_poll", "swem_batch_wait", "swem_server_get_stats", "swem_vec_cgstep", "swem_vec_dot", "swem_vec_mul",
 )
 REQ_PENDING, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(4)
 
@@ -175,6 +175,7 @@ def load():
         "swem_server_get_stats": ([vp, P(ServerStats)], ctypes.c_int),
         "swem_vec_cgstep": ([ctypes.c_float, vp, ctypes.c_float, vp, ctypes.c_size_t, vp], ctypes.c_int),
         "swem_vec_dot": ([vp, vp, ctypes.c_size_t, P(ctypes.c_double), vp], ctypes.c_int),
+        "swem_vec_mul": ([vp, vp, vp, ctypes.c_size_t, vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -57,3 +57,44 @@ def test_wem_operator_dot_test(gpu):
     d = rng.standard_normal((sx.size, g.nm, g.nt)).astype(np.float32)
     a, b = solvers.DotTest(m, d, [op], [{}], inner=solvers.InnerProduct64)
     assert abs(a - b) / abs(b) <= 1e-5
+
+
+def test_vec_mul(gpu):
+    rng = np.random.default_rng(6)
+    for n in (1, 777, 1 << 20):
+        x = rng.standard_normal(n).astype(np.float32)
+        w = rng.standard_normal(n).astype(np.float32)
+        dx, dw, dy = seiswem.DeviceArray(x.shape, x), seiswem.DeviceArray(w.shape, w), seiswem.DeviceArray(x.shape)
+        solvers.vec_mul(dx, dw, dy)
+        assert np.array_equal(dy.download(), x * w)
+        solvers.vec_mul(dx, dw)  # in place
+        assert np.array_equal(dx.download(), x * w)
+
+
+def test_lswem_matches_host_weighted_chain(gpu):
+    """ShotProfileLSWEM (data weights from CalculateSampling with dead traces, a mute, the GPU WEM
+    pair, all on the device) against the host CG over the chain [WeightingOp, WEMOperator] with
+    the same weight vector, and the weighted chain's dot test."""
+    g, vel, refl, wav, sx = small_lsm()
+    op = solvers.WEMOperator(g, vel, wav, sx)
+    d_obs = op(refl, False)
+    d_obs[0, 5:9] = 0.0    # dead traces
+    d_obs[2, 40] = 0.0
+    kw = dict(tmute=0.05, vmute=1500.0, taper=0.04)
+    plan = seiswem.Plan(g, vel, wav)
+    m_dev, cost_dev = solvers.ShotProfileLSWEM(d_obs, plan, sx, Niter=5, **kw)
+    wd, _ = solvers.CalculateSampling(d_obs, h=True)
+    assert wd[0, 5:9].sum() == 0 and wd[1].min() == 1
+    gx = g.omx + g.dmx * np.arange(g.nm)
+    w = np.stack([wd[i] * solvers.SeisMute(np.ones((g.nm, g.nt), np.float32), offset=np.abs(gx - sx[i]),
+                                           dt=g.dt, **kw) for i in range(sx.size)])
+    ops, pars = [solvers.WeightingOp, op], [dict(w=w), {}]
+    m_host, cost_host = solvers.ConjugateGradients(d_obs, ops, pars, Niter=5, inner=solvers.InnerProduct64)
+    err = rel_l2(m_dev, m_host)
+    print(f"LSWEM device vs host chain: rel-L2 {err:.3e}")
+    assert err <= 1e-5
+    assert np.allclose(cost_dev, cost_host, rtol=1e-4)
+    rng = np.random.default_rng(8)
+    a, b = solvers.DotTest(rng.standard_normal((g.nm, g.nz)).astype(np.float32),
+                           rng.standard_normal(d_obs.shape).astype(np.float32), ops, pars, inner=solvers.InnerProduct64)
+    assert abs(a - b) / abs(b) <= 1e-5

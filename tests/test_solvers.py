@@ -57,3 +57,58 @@ def test_inner_product_is_float32_of_real_part():
     assert isinstance(v, np.float32)
     assert v == np.float32(np.real(np.sum(np.conj(a) * b)))
     assert solvers.CGStep(np.ones(3), np.full(3, 2.0), a=0.5, b=-1.0).tolist() == [-1.5, -1.5, -1.5]
+
+
+def test_seis_mute_closed_form():
+    """SeisMute.jl:1-19: t = sqrt(tmute^2 + (offset/vmute)^2); samples at it*dt <= t - taper are
+    zeroed, those in (t - taper, t] scaled by 1 - (t - it*dt)/taper, later ones untouched."""
+    dt, nt = 0.01, 60
+    x = np.ones((3, nt), np.float32)
+    off = np.array([0.0, 300.0, 800.0])
+    out = solvers.SeisMute(x, offset=off, tmute=0.2, vmute=1000.0, taper=0.1, dt=dt)
+    for ix, o in enumerate(off):
+        t = np.hypot(0.2, o / 1000.0)
+        for it in range(nt):
+            ts = it * dt
+            want = (0.0 if ts <= t - 0.1 else 1.0 - (t - ts) / 0.1) if ts <= t else 1.0
+            assert abs(out[ix, it] - want) <= 1e-6, (ix, it)
+    # offset 0, tmute 0.2: samples 0..10 zeroed (ts <= 0.1), ramp on 11..20, 1 from 21 on
+    assert out[0, 10] == 0.0 and out[0, 21] == 1.0 and abs(out[0, 15] - 0.5) < 1e-6
+
+
+def test_calculate_sampling_live_and_dead_traces():
+    d = np.zeros((2, 4, 8), np.float32)
+    d[0, 1, 3] = 1.0
+    d[1, 2, 0] = 1e-6          # sum of squares 1e-12 <= 1e-10, sqrt 1e-6 > 1e-10
+    wd = solvers.CalculateSampling(d)
+    assert wd.shape == d.shape and wd.dtype == np.float32
+    assert np.array_equal(wd[:, :, 0], np.array([[0, 1, 0, 0], [0, 0, 0, 0]], np.float32))
+    wd2, h = solvers.CalculateSampling(d, h=True)
+    assert h is True
+    assert np.array_equal(wd2[:, :, 5], np.array([[0, 1, 0, 0], [0, 0, 1, 0]], np.float32))
+
+
+def test_weighted_chain_cg_equals_cg_on_weighted_matrix():
+    """[WeightingOp, SeisMute-as-operator, L] through LinearOperator is diag(w) diag(mute) L: CG on
+    the chain reproduces CG on the explicitly weighted matrix, and the chain passes the dot test."""
+    rng = np.random.default_rng(4)
+    ntr, nt, nm = 6, 20, 40
+    L = rng.standard_normal((ntr * nt, nm))
+    w = (rng.random((ntr, nt)) > 0.2).astype(np.float64)
+    off = np.linspace(0, 500, ntr)
+
+    def mute_op(x, adj, **kw):
+        return solvers.SeisMute(x.reshape(ntr, nt), **kw).reshape(x.shape)
+
+    mute_kw = dict(offset=off, tmute=0.05, vmute=2000.0, taper=0.05, dt=0.01)
+    mvec = solvers.SeisMute(np.ones((ntr, nt)), **mute_kw).ravel()
+    ops = [solvers.WeightingOp, mute_op, matrix_op]
+    pars = [dict(w=w.ravel()), mute_kw, dict(matrix=L)]
+    d = L @ rng.standard_normal(nm)
+    m1, c1 = solvers.ConjugateGradients(d, ops, pars, Niter=15, inner=solvers.InnerProduct64)
+    Lw = (w.ravel() * mvec)[:, None] * L
+    m2, c2 = solvers.ConjugateGradients(d, [matrix_op], [dict(matrix=Lw)], Niter=15, inner=solvers.InnerProduct64)
+    assert np.allclose(m1, m2, rtol=1e-10, atol=1e-12) and np.allclose(c1, c2, rtol=1e-10)
+    a, b = solvers.DotTest(rng.standard_normal(nm), rng.standard_normal(ntr * nt), ops, pars,
+                           inner=solvers.InnerProduct64)
+    assert abs(a - b) <= 1e-10 * abs(b)
