@@ -116,6 +116,19 @@ int swem_wem(float* d, float* m, const float* wav,
              int padt, int padx,
              int adj, int pspi, int verbose);
 
+/* Replaces compute_angles() (wem.c:566-574, the routine behind compute_angles.c:12-118):
+ * source-side incidence angles in degrees, angx and angy [nm][nz] (host arrays, overwritten),
+ * of one shot at (sx, sy), from a split-step source march with ntfft = nt and nkx = nmx and the
+ * spatial derivatives of the source wavefield per frequency (wem.c:694-981).  2-D only in this
+ * build (nmy > 1 returns SWEM_EUNSUPPORTED); angy is zero in 2-D, as in the reference. */
+int swem_compute_angles(float* angx, float* angy, const float* wav,
+                        int nt, float ot, float dt,
+                        int nmx, float omx, float dmx,
+                        int nmy, float omy, float dmy,
+                        float sx, float sy,
+                        int nz, float oz, float dz, float sz,
+                        const float* vel, float fmin, float fmax, int verbose);
+
 /* wem() with explicit options (pspi_mode, precision, device...). */
 int swem_wem_ex(float* d, float* m, const float* wav, const swem_geometry* g,
                 float sx, float sy, const float* vel, int adj, const swem_options* opt);

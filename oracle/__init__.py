@@ -80,6 +80,7 @@ def lib():
             getattr(L, pre + "step").argtypes = [rp, f32p, gp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int]
             getattr(L, pre + "wem").argtypes = [rp, rp, f32p, f32p, gp, ctypes.c_float, ctypes.c_float, ctypes.c_int]
             getattr(L, pre + "shots").argtypes = [rp, rp, f32p, f32p, gp, f32p, f32p, ctypes.c_int, ctypes.c_int]
+            getattr(L, pre + "angles").argtypes = [rp, rp, f32p, f32p, gp, ctypes.c_float, ctypes.c_float]
         _lib = L
     return _lib
 
@@ -251,3 +252,18 @@ def shots(p: WemParams, vel: np.ndarray, wav: np.ndarray, sx, sy=None, *, d=None
     _check(fn(_ptr(d, ct), _ptr(m, ct), _ptr(wav, ctypes.c_float), _ptr(vel, ctypes.c_float), ctypes.byref(g),
               _ptr(sx, ctypes.c_float), _ptr(sy, ctypes.c_float), ns, int(adj)), "shots")
     return m if adj else d
+
+
+def angles(p: WemParams, vel: np.ndarray, wav: np.ndarray, sx: float, sy: float = 0.0, f64: bool = True):
+    """compute_angles (wem.c:566-992) of one shot: (angx, angy) [nm, nz] in degrees.  padt, padx
+    and pspi of `p` are ignored (the routine uses ntfft = nt, nkx = nmx and split-step)."""
+    rt, ct = (np.float64, ctypes.c_double) if f64 else (np.float32, ctypes.c_float)
+    fn = lib().orc64_angles if f64 else lib().orc32_angles
+    vel = np.ascontiguousarray(vel, dtype=np.float32)
+    wav = np.ascontiguousarray(wav, dtype=np.float32)
+    ax = np.zeros((p.nm, p.nz), rt)
+    ay = np.zeros((p.nm, p.nz), rt)
+    g = p.geom()
+    _check(fn(_ptr(ax, ct), _ptr(ay, ct), _ptr(wav, ctypes.c_float), _ptr(vel, ctypes.c_float), ctypes.byref(g),
+              sx, sy), "angles")
+    return ax, ay

@@ -96,6 +96,10 @@ int orc64_shots(double* d, double* m, const float* wav, const float* vel, const 
                 const float* sx, const float* sy, int nshot, int adj);
 int orc32_shots(float* d, float* m, const float* wav, const float* vel, const orc_geom* g,
                 const float* sx, const float* sy, int nshot, int adj);
+/* compute_angles (wem.c:566-992): source incidence angles (degrees) angx, angy [nm][nz] of one
+ * shot; the geometry's padt / padx / pspi are ignored (ntfft = nt, nkx = nmx, split-step). */
+int orc64_angles(double* angx, double* angy, const float* wav, const float* vel, const orc_geom* g, float sx, float sy);
+int orc32_angles(float* angx, float* angy, const float* wav, const float* vel, const orc_geom* g, float sx, float sy);
 
 #ifdef __cplusplus
 }

@@ -602,4 +602,187 @@ int SFX(shots)(RT* d, RT* m, const float* wav, const float* vel, const orc_geom*
     return ORC_OK;
 }
 
+
+/* ------------------------------------------------------------------ compute_angles() */
+
+/* 3-D transform of a[ix*nky*nkz + iy*nkz + iz] (fftwf_plan_dft(3, {nkx,nky,nkz}), wem.c:631-633):
+ * 1-D transforms along z, y and x.  tw* hold exp(-2 pi i j / n) of each length. */
+static void SFX(fft3)(CT* a, int nkx, int nky, int nkz, const CT* twx, const CT* twy, const CT* twz, int sign,
+                      CT* line, CT* sc)
+{
+    for (size_t l = 0; l < (size_t)nkx * nky; l++) SFX(fft1)(a + l * nkz, nkz, twz, sign, sc);
+    if (nky > 1)
+        for (int ix = 0; ix < nkx; ix++)
+            for (int iz = 0; iz < nkz; iz++) {
+                CT* base = a + (size_t)ix * nky * nkz + iz;
+                for (int iy = 0; iy < nky; iy++) line[iy] = base[(size_t)iy * nkz];
+                SFX(fft1)(line, nky, twy, sign, sc);
+                for (int iy = 0; iy < nky; iy++) base[(size_t)iy * nkz] = line[iy];
+            }
+    if (nkx > 1)
+        for (size_t l = 0; l < (size_t)nky * nkz; l++) {
+            CT* base = a + l;
+            for (int ix = 0; ix < nkx; ix++) line[ix] = base[(size_t)ix * nky * nkz];
+            SFX(fft1)(line, nkx, twx, sign, sc);
+            for (int ix = 0; ix < nkx; ix++) base[(size_t)ix * nky * nkz] = line[ix];
+        }
+}
+
+/* Signed wavenumber of bin i of n with spacing dk (wem.c:799-804, 862-863, 886-887, 907-908). */
+static RT SFX(kbin)(int i, int n, RT dk)
+{
+#if ORC_IS_F64
+    return i <= n / 2 ? dk * i : -(dk * n - dk * i);
+#else
+    return i <= (int)n / 2 ? (float)dk * i : -((float)dk * n - dk * i);
+#endif
+}
+
+/* compute_angles (wem.c:566-692), extrapolate_source (wem.c:694-776), ssop_source
+ * (wem.c:778-826) and calculate_derivatives (wem.c:828-981): source incidence angles in
+ * degrees, angx / angy [nm][nz].  The source march is ssop with adj = false and -dz
+ * (ssop_source: taper, split-step phase, FFT, kz phase, IFFT; ntfft = nt, nkx = nmx, so the
+ * geometry is taken with padt = padx = 1); per frequency the derivatives of the stored source
+ * volume come from one forward 3-D transform (nkx, nky, nkz = nz) and inverse transforms of
+ * i kx, i ky, i kz times it, normalised by 1/(nkx nky nkz), and the left / right signs compare
+ * the inverse transforms of the negative-only and positive-only kx (ky) parts.  Frequencies are
+ * summed in order (the reference sums them from OpenMP threads without synchronisation,
+ * wem.c:657-665), then angx *= sign(angx_sign) / (ifmax - ifmin + 1) (wem.c:671-672). */
+int SFX(angles)(RT* angx, RT* angy, const float* wav, const float* vel, const orc_geom* g0, float sx, float sy)
+{
+    orc_geom ga = *g0;
+    ga.padt = 1;
+    ga.padx = 1;
+    ga.pspi = 0;
+    const orc_geom* g = &ga;
+    SFX(ctx) c;
+    int rc = SFX(ctx_init)(&c, g, vel);
+    if (rc) return rc;
+    int igx, igy;
+    rc = orc_source_index(g, sx, sy, &igx, &igy);
+    if (rc) { SFX(ctx_free)(&c); return rc; }
+    const orc_sampling* s = &c.s;
+    const int nmy = g->nmy, nz = g->nz, nt = g->nt, nw = s->nw, nm = c.nm;
+    const int nkx = s->nkx, nky = s->nky, nkz = nz;
+    const size_t nvol = (size_t)nkx * nky * nkz;
+    const int ifmin = s->ifmin, ifmax = s->ifmax;
+    memset(angx, 0, sizeof(RT) * (size_t)nm * nz);
+    memset(angy, 0, sizeof(RT) * (size_t)nm * nz);
+    RT* sgx = (RT*)calloc((size_t)nm * nz, sizeof(RT));
+    RT* sgy = (RT*)calloc((size_t)nm * nz, sizeof(RT));
+    CT* ws = (CT*)malloc(sizeof(CT) * nw);
+    RT* tr = (RT*)malloc(sizeof(RT) * nt);
+    CT* u = (CT*)malloc(sizeof(CT) * nm);
+    CT* us = (CT*)malloc(sizeof(CT) * (size_t)nm * nz);
+    CT* a = (CT*)malloc(sizeof(CT) * nvol);
+    CT* b = (CT*)malloc(sizeof(CT) * nvol);
+    CT* bl = (CT*)malloc(sizeof(CT) * nvol);
+    CT* br = (CT*)malloc(sizeof(CT) * nvol);
+    RT* dxa = (RT*)malloc(sizeof(RT) * (size_t)nm * nz);
+    RT* dya = (RT*)malloc(sizeof(RT) * (size_t)nm * nz);
+    int nl = nkx > nky ? nkx : nky;
+    if (nkz > nl) nl = nkz;
+    CT* line = (CT*)malloc(sizeof(CT) * nl);
+    CT* sc = (CT*)malloc(sizeof(CT) * nl);
+    CT* twz = (CT*)malloc(sizeof(CT) * nkz);
+    SFX(work) wk;
+    memset(&wk, 0, sizeof(wk));
+    int ok = sgx && sgy && ws && tr && u && us && a && b && bl && br && dxa && dya && line && sc && twz &&
+             SFX(work_init)(&wk, &c) == ORC_OK;
+    if (ok && ifmax > ifmin) {
+        SFX(twiddles)(twz, nkz);
+        for (int it = 0; it < nt; it++) tr[it] = wav[it];
+        SFX(fop)((RT*)ws, tr, nw, nt, 1);
+#if ORC_IS_F64
+        const RT rnt = 1.0 / sqrt((double)s->ntfft), rvol = (double)nvol;
+        const RT dkx = 2.0 * M_PI / ((double)nkx * g->dmx), dky = 2.0 * M_PI / ((double)nky * g->dmy);
+        const RT dkz = 2.0 * M_PI / ((double)nkz * g->dz), deg = 180.0 / M_PI, eps = 1e-10;
+#else
+        const RT rnt = 1.0f / sqrtf((float)s->ntfft), rvol = (float)nkx * nky * nkz;
+        const RT dkx = s->dkx, dky = s->dky;
+        const RT dkz = (float)(2 * 3.141592653589793 / ((double)(float)nkz) / (double)g->dz);
+        const RT deg = (float)(180 / 3.141592653589793), eps = 1e-10f;
+#endif
+        for (int iw = ifmin; iw < ifmax; iw++) {
+            RT w = SFX(omega)(&c, iw);
+            for (int x = 0; x < nm; x++) u[x] = 0;
+            u[(size_t)igx * nmy + igy] = ws[iw] * rnt;
+            for (int iz = 0; iz < nz; iz++) {
+                float z = g->oz + g->dz * iz;
+                if (z >= g->sz) {
+                    SFX(ssop)(&c, u, w, iz, (RT)(-g->dz), 0, &wk);
+                    for (int x = 0; x < nm; x++) us[(size_t)x * nz + iz] = u[x];
+                } else
+                    for (int x = 0; x < nm; x++) us[(size_t)x * nz + iz] = 0;
+            }
+            /* calculate_derivatives (wem.c:828-981); nkx = nmx, nky = nmy, nkz = nz */
+            memcpy(a, us, sizeof(CT) * nvol);
+            SFX(fft3)(a, nkx, nky, nkz, c.twx, c.twy, twz, -1, line, sc);
+            for (int pass = 0; pass < 3; pass++) {  /* x, y, z derivative magnitudes */
+                for (int ix = 0; ix < nkx; ix++)
+                    for (int iy = 0; iy < nky; iy++)
+                        for (int iz = 0; iz < nkz; iz++) {
+                            size_t k = ((size_t)ix * nky + iy) * nkz + iz;
+                            RT kk = pass == 0 ? SFX(kbin)(ix, nkx, dkx)
+                                  : pass == 1 ? SFX(kbin)(iy, nky, dky) : SFX(kbin)(iz, nkz, dkz);
+                            b[k] = (RT _Complex)_Complex_I * kk * a[k];
+                        }
+                SFX(fft3)(b, nkx, nky, nkz, c.twx, c.twy, twz, +1, line, sc);
+                for (size_t k = 0; k < nvol; k++) {
+                    CT v = b[k] / rvol;
+#if ORC_IS_F64
+                    RT mag = cabs(v);
+#else
+                    RT mag = cabsf(v);
+#endif
+                    if (pass == 0) dxa[k] = mag;
+                    else if (pass == 1) dya[k] = mag;
+                    else {
+#if ORC_IS_F64
+                        angx[k] += deg * atan(dxa[k] / (mag + eps));
+                        angy[k] += deg * atan(dya[k] / (mag + eps));
+#else
+                        angx[k] += deg * atanf(dxa[k] / (mag + eps));
+                        angy[k] += deg * atanf(dya[k] / (mag + eps));
+#endif
+                    }
+                }
+            }
+            for (int axis = 0; axis < 2; axis++) {  /* left / right signs along x, then y */
+                for (int ix = 0; ix < nkx; ix++)
+                    for (int iy = 0; iy < nky; iy++)
+                        for (int iz = 0; iz < nkz; iz++) {
+                            size_t k = ((size_t)ix * nky + iy) * nkz + iz;
+                            int i = axis == 0 ? ix : iy, n = axis == 0 ? nkx : nky;
+                            RT kk = SFX(kbin)(i, n, axis == 0 ? dkx : dky);
+                            RT kl = i <= n / 2 ? 0 : kk, kr = i <= n / 2 ? kk : 0;
+                            bl[k] = (RT _Complex)_Complex_I * kl * a[k];
+                            br[k] = (RT _Complex)_Complex_I * kr * a[k];
+                        }
+                SFX(fft3)(bl, nkx, nky, nkz, c.twx, c.twy, twz, +1, line, sc);
+                SFX(fft3)(br, nkx, nky, nkz, c.twx, c.twy, twz, +1, line, sc);
+                RT* sg = axis == 0 ? sgx : sgy;
+                for (size_t k = 0; k < nvol; k++) {
+#if ORC_IS_F64
+                    RT mu = cabs(us[k]), ml = cabs(bl[k]), mr = cabs(br[k]);
+#else
+                    RT mu = cabsf(us[k]), ml = cabsf(bl[k]), mr = cabsf(br[k]);
+#endif
+                    sg[k] += (ml >= mr ? mu : -mu) / (RT)nw;
+                }
+            }
+        }
+        const RT nrm = (RT)1 / (RT)(ifmax - ifmin + 1);
+        for (size_t k = 0; k < (size_t)nm * nz; k++) {
+            angx[k] *= (sgx[k] >= 0 ? (RT)1 : (RT)-1) * nrm;
+            angy[k] *= (sgy[k] >= 0 ? (RT)1 : (RT)-1) * nrm;
+        }
+    }
+    SFX(work_free)(&wk);
+    free(sgx); free(sgy); free(ws); free(tr); free(u); free(us); free(a); free(b); free(bl); free(br);
+    free(dxa); free(dya); free(line); free(sc); free(twz);
+    SFX(ctx_free)(&c);
+    return ok ? ORC_OK : ORC_ENOMEM;
+}
+
 #undef CT

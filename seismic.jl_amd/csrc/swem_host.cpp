@@ -1143,6 +1143,103 @@ int swem_wem(float* d, float* m, const float* wav, int nt, float ot, float dt, i
     return swem_wem_ex(d, m, wav, &g, sx, sy, vel, adj, &o);
 }
 
+// compute_angles (wem.c:566-692) on the device, 2-D: the plan is built with padt = padx = 1 and
+// split-step (ntfft = nt, nkx = nmx, wem.c:588-594); per frequency chunk the source march stores
+// the volume, the column pass its z derivative and the row pass sums the angles and signs.
+static int angles_impl(swem_plan* p, float sx, float sy, float* angx, float* angy)
+{
+    hipStream_t st = nullptr;
+    const int nmx = p->g.nmx, nz = p->g.nz;
+    const size_t nmz = (size_t)nmx * nz;
+    for (size_t i = 0; i < nmz; i++) angx[i] = angy[i] = 0.f;
+    if (p->nf < 1) return SWEM_OK;
+    // z transforms: twiddles of the array length, Bluestein tables for other lengths
+    const int mz = march2d_blue_len(nz), nza = mz ? mz : nz;
+    DevBuf<float2> twz, chz, bchz, vol;
+    DevBuf<float> dzmag, acc, sgn;
+    std::vector<float2> tw(nza);
+    for (int m = 0; m < nza; m++) {
+        const double a = -2.0 * kPi * (double)m / (double)nza;
+        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    SWEM_TRY(twz.upload(tw.data(), tw.size()));
+    if (mz) {
+        std::vector<float2> chirp, bchirp;
+        bluestein_tables(nz, mz, chirp, bchirp);
+        SWEM_TRY(chz.upload(chirp.data(), chirp.size()));
+        SWEM_TRY(bchz.upload(bchirp.data(), bchirp.size()));
+    }
+    SWEM_TRY(vol.alloc((size_t)p->nfc * nmz));
+    SWEM_TRY(dzmag.alloc((size_t)p->nfc * nmz));
+    SWEM_TRY(acc.alloc(nmz));
+    SWEM_TRY(sgn.alloc(nmz));
+    SWEM_HIP(hipMemsetAsync(acc.p, 0, nmz * sizeof(float), st));
+    SWEM_HIP(hipMemsetAsync(sgn.p, 0, nmz * sizeof(float), st));
+    SWEM_TRY(set_igx(p, &sx, &sy, 1, st));
+    const int T = p->narr >= 1024 ? p->narr / 8 : 64;
+    const size_t lds = ((size_t)(p->narr + p->narr / 8 + 31) / 32 * 32 + 4) * 2 * sizeof(float2);
+    int F = std::max(1, 512 / T);
+    while (F > 1 && F * lds > 80 * 1024) F--;
+    March2DArgs az{};
+    az.nkx = nz;
+    az.tw = twz.p;
+    az.chirp = chz.p;
+    az.bchirp = bchz.p;
+    AngleArgs b{};
+    b.nmx = nmx;
+    b.nz = nz;
+    b.nw = p->s.nw;
+    b.dkx = p->s.dkx;
+    b.dkz = (float)(2.0 * kPi / (double)(float)nz / (double)p->g.dz);
+    b.vol = vol.p;
+    b.dzmag = dzmag.p;
+    b.acc = acc.p;
+    b.sgn = sgn.p;
+    for (int c = 0; c < p->nchunks; c++) {
+        SWEM_TRY(ensure_chunk(p, c, st));
+        March2DArgs ax = base_args(p, c, F);
+        ax.nshot = 1;
+        ax.igx = p->igx.p;
+        ax.smig = vol.p;
+        b.nfc = ax.nfc;
+        SWEM_TRY(launch_angles(0, ax, az, b, F, st));
+        SWEM_TRY(launch_angles(1, ax, az, b, F, st));
+        SWEM_TRY(launch_angles(2, ax, az, b, F, st));
+    }
+    std::vector<float> ha(nmz), hs(nmz);
+    SWEM_HIP(hipMemcpyAsync(ha.data(), acc.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
+    SWEM_HIP(hipMemcpyAsync(hs.data(), sgn.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
+    SWEM_HIP(hipStreamSynchronize(st));
+    // angx *= signf1(angx_sign) / (ifmax - ifmin + 1) (wem.c:671); angy stays 0 in 2-D (ky = 0)
+    for (int ix = 0; ix < nmx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            const size_t i = (size_t)iz * nmx + ix;
+            angx[(size_t)ix * nz + iz] = ha[i] * ((hs[i] >= 0.f ? 1.f : -1.f) / (float)(p->nf + 1));
+        }
+    return SWEM_OK;
+}
+
+int swem_compute_angles(float* angx, float* angy, const float* wav, int nt, float ot, float dt, int nmx, float omx,
+                        float dmx, int nmy, float omy, float dmy, float sx, float sy, int nz, float oz, float dz,
+                        float sz, const float* vel, float fmin, float fmax, int verbose)
+{
+    if (!angx || !angy || !wav || !vel) return set_error(SWEM_EINVAL, "compute_angles: null argument");
+    if (nmy > 1) return set_error(SWEM_EUNSUPPORTED, "compute_angles: 3-D (nmy > 1) is not available in this build");
+    if (nmx < 2 || !angles_supported(nmx) || !angles_supported(nz))
+        return set_error(SWEM_EUNSUPPORTED, "compute_angles: nmx = %d, nz = %d: this build handles 2..2048 and powers of "
+                         "two to 4096 (nmx >= 2)", nmx, nz);
+    swem_geometry g;
+    geom_from_args(&g, nt, ot, dt, nmx, omx, dmx, nmy, omy, dmy, nz, oz, dz, 0.f, sz, 1, fmin, fmax, 1, 1, 0);
+    swem_options o;
+    swem_options_init(&o);
+    o.verbose = verbose;
+    swem_plan* p = nullptr;
+    SWEM_TRY(swem_plan_create(&g, vel, wav, &o, &p));
+    const int rc = angles_impl(p, sx, sy, angx, angy);
+    swem_plan_destroy(p);
+    return rc;
+}
+
 int swem_malloc(void** ptr, size_t bytes)
 {
     if (!ptr) return set_error(SWEM_EINVAL, "null pointer");

@@ -88,6 +88,22 @@ struct March2DArgs {
     float2* dgw;            // [nshot][nmx][nf] modelled spectra (modelling output)
 };
 
+// Source incidence angles (compute_angles, wem.c:566-992), 2-D.  The march writes every depth of
+// the source wavefield to vol; the column pass writes |u_z| (z derivative) to dzmag; the row pass
+// adds the x derivative angles and the left / right signs of every frequency, in order, to acc /
+// sgn.  ax: x transforms (nkx = nmx, tables of the plan); az: z transforms (length nz).
+struct AngleArgs {
+    int nmx, nz, nfc, nw;
+    float dkx, dkz;         // wavenumber spacings (float, as the reference)
+    const float2* vol;      // [nfc][nz][nmx]
+    float* dzmag;           // [nfc][nz][nmx]
+    float* acc;             // [nz][nmx] sum of angles (degrees)
+    float* sgn;             // [nz][nmx] sum of signed |u| / nw
+};
+// which: 0 = march (F frequencies per workgroup), 1 = column pass, 2 = row pass.
+int launch_angles(int which, const March2DArgs& ax, const March2DArgs& az, const AngleArgs& b, int F, hipStream_t st);
+int angles_supported(int n);
+
 // mode: 0 = SSF, 1 = PSPI compat, 2 = PSPI linear.  variant: SWEM_VARIANT_*.  F: frequencies
 // per workgroup.
 int launch_march2d(const March2DArgs& a, int nkx, int F, int mode, int adj, int variant, hipStream_t st);

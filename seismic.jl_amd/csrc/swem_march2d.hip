@@ -1135,4 +1135,215 @@ int launch_march2d(const March2DArgs& a, int nkx, int F, int mode, int adj, int 
     return set_error(SWEM_EUNSUPPORTED, "2-D march: nkx=%d", nkx);
 }
 
+
+// ------------------------------------------------------------------ source incidence angles
+// compute_angles (wem.c:566-776) for 2-D.  The source march is the modelling receiver's step
+// (ssop with adj = false and -dz, which is ssop_source, wem.c:778-826) stored at every depth.
+// The derivatives of calculate_derivatives (wem.c:828-981) are separable: the inverse 3-D
+// transform of i kx FFT3(u), divided by nkx nky nkz, equals IFFT_x(i kx FFT_x(u)) / nkx, so the
+// x parts run along rows and the z part along columns of the stored volume.
+template <int N, int P, class X>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void angle_march(March2DArgs a)
+{
+    constexpr int T = N / P;
+    constexpr int L = LdsLen<N>::value;
+    extern __shared__ float2 smem[];
+    int g, shot;
+    decode_block(a.ngroups, a.nshot, g, shot);
+    if (g < 0) return;
+    const int NSUB = blockDim.x / T;
+    const int tid = threadIdx.x, s = tid / T, t = tid - s * T;
+    const int fc = g * NSUB + s;
+    const bool active = fc < a.nfc;
+    const int fl = a.f0 + fc;
+    StepCtx c;
+    c.t = t;
+    c.fc = __builtin_amdgcn_readfirstlane(active ? fc : 0);
+    float2* lds = smem + (size_t)s * 2 * L;
+    int pp = 0;
+    float tap[P];
+    load_taper<N, P, kModeSSF>(tap, a, t);
+    const int igx = a.igx[shot];
+    float2* vol = a.smig + (size_t)(active ? fc : 0) * a.nz * a.nmx;
+    float2 u[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int x = t + q * T;
+        u[q] = (active && x == igx) ? a.spec_s[fl] : make_float2(0.f, 0.f);
+    }
+    for (int iz = 0; iz < a.nz; ++iz) {  // extrapolate_source, wem.c:712-719
+        c.iz = iz;
+        const float z = a.oz + a.dz * (float)iz;
+        const bool ds = z >= a.sz;
+        if (ds) ssf_fwd_step<N, P, false, X>(u, lds, pp, a, c, tap);
+        if (active) {
+            const auto sr = buf_rsrc(vol + (size_t)iz * a.nmx, (uint32_t)a.nmx * 8u);
+#pragma unroll
+            for (int q = 0; q < P; ++q) buf_store2(sr, (uint32_t)(t + q * T) * 8u, ds ? u[q] : make_float2(0.f, 0.f));
+        }
+    }
+}
+
+__device__ __forceinline__ float kbin(int k, int n, float dk)
+{
+    return k <= n / 2 ? dk * (float)k : -(dk * (float)n - dk * (float)k);
+}
+
+__device__ __forceinline__ float2 times_i(float2 v, float k) { return make_float2(-v.y * k, v.x * k); }
+
+// Column pass (wem.c:888-899): |IFFT_z(i kz FFT_z(u))| / nz for one (frequency, x) column.
+template <int N, int P, class X>
+__global__ __launch_bounds__(N / P) void angle_cols(March2DArgs az, AngleArgs b)
+{
+    constexpr int T = N / P;
+    extern __shared__ float2 smem[];
+    const int f = blockIdx.x / b.nmx, x = blockIdx.x - f * b.nmx;
+    const int t = threadIdx.x;
+    int pp = 0;
+    const float2* col = b.vol + (size_t)f * b.nz * b.nmx + x;
+    float2 v[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int iz = t + q * T;
+        v[q] = iz < b.nz ? col[(size_t)iz * b.nmx] : make_float2(0.f, 0.f);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    X::template fft<-1, false>(v, smem, az, t, pp);
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int k = t + q * T;
+        v[q] = times_i(v[q], k < b.nz ? kbin(k, b.nz, b.dkz) : 0.f);
+    }
+    X::template fft<+1, false>(v, smem, az, t, pp);
+    float* out = b.dzmag + (size_t)f * b.nz * b.nmx + x;
+    const float inv = 1.f / (float)b.nz;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int iz = t + q * T;
+        if (iz < b.nz) out[(size_t)iz * b.nmx] = sqrtf(v[q].x * v[q].x + v[q].y * v[q].y) * inv;
+    }
+}
+
+// Row pass for depth iz: per frequency in order, the x derivative (wem.c:858-875) against the
+// column pass's z derivative (wem.c:747-748) and the left / right split (wem.c:921-945,
+// wem.c:739-740), added to the sums of earlier chunks.
+template <int N, int P, class X>
+__global__ __launch_bounds__(N / P) void angle_rows(March2DArgs ax, AngleArgs b)
+{
+    constexpr int T = N / P;
+    extern __shared__ float2 smem[];
+    const int iz = blockIdx.x;
+    const int t = threadIdx.x;
+    int pp = 0;
+    const int n = ax.nkx;
+    const float deg = (float)(180.0 / 3.141592653589793);
+    const float inv = 1.f / (float)n;
+    const float inv_nw = 1.f / (float)b.nw;
+    float acc[P], sgn[P], kx[P];
+    const size_t row = (size_t)iz * b.nmx;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int x = t + q * T;
+        acc[q] = x < b.nmx ? b.acc[row + x] : 0.f;
+        sgn[q] = x < b.nmx ? b.sgn[row + x] : 0.f;
+        kx[q] = x < n ? kbin(x, n, b.dkx) : 0.f;
+    }
+    for (int f = 0; f < b.nfc; ++f) {
+        const float2* ur = b.vol + ((size_t)f * b.nz + iz) * b.nmx;
+        const float* zr = b.dzmag + ((size_t)f * b.nz + iz) * b.nmx;
+        float2 A[P];
+        float mu[P], mz[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int x = t + q * T;
+            A[q] = x < b.nmx ? ur[x] : make_float2(0.f, 0.f);
+            mz[q] = x < b.nmx ? zr[x] : 0.f;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < P; ++q) mu[q] = sqrtf(A[q].x * A[q].x + A[q].y * A[q].y);
+        X::template fft<-1, false>(A, smem, ax, t, pp);
+        float mx[P], ml[P];
+        float2 B[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) B[q] = times_i(A[q], kx[q]);
+        X::template fft<+1, false>(B, smem, ax, t, pp);
+#pragma unroll
+        for (int q = 0; q < P; ++q) mx[q] = sqrtf(B[q].x * B[q].x + B[q].y * B[q].y) * inv;
+        // negative wavenumbers only (k > n/2), then positive only (k <= n/2)
+#pragma unroll
+        for (int q = 0; q < P; ++q) B[q] = times_i(A[q], (t + q * T) <= n / 2 ? 0.f : kx[q]);
+        X::template fft<+1, false>(B, smem, ax, t, pp);
+#pragma unroll
+        for (int q = 0; q < P; ++q) ml[q] = sqrtf(B[q].x * B[q].x + B[q].y * B[q].y);
+#pragma unroll
+        for (int q = 0; q < P; ++q) B[q] = times_i(A[q], (t + q * T) <= n / 2 ? kx[q] : 0.f);
+        X::template fft<+1, false>(B, smem, ax, t, pp);
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const float mr = sqrtf(B[q].x * B[q].x + B[q].y * B[q].y);
+            sgn[q] += (ml[q] >= mr ? mu[q] : -mu[q]) * inv_nw;
+            acc[q] += deg * atanf(mx[q] / (mz[q] + 1e-10f));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int x = t + q * T;
+        if (x < b.nmx) {
+            b.acc[row + x] = acc[q];
+            b.sgn[row + x] = sgn[q];
+        }
+    }
+}
+
+template <int N, class X>
+static int launch_angles_n(int which, const March2DArgs& ax, const March2DArgs& az, const AngleArgs& b, int F,
+                           hipStream_t st)
+{
+    constexpr int P = N >= 1024 ? 8 : N / 64, T = N / P;
+    const size_t len = (size_t)LdsLen<N>::value * sizeof(float2);
+    if (which == 0) {
+        if (F < 1 || F * T > 512 || F * 2 * len > 160 * 1024) return set_error(SWEM_EINVAL, "angles: %d frequencies per block", F);
+        return launch_kernel(angle_march<N, P, X>, ax, dim3(T * F), F * 2 * len, st);
+    }
+    auto k = which == 1 ? angle_cols<N, P, X> : angle_rows<N, P, X>;
+    const March2DArgs& a = which == 1 ? az : ax;
+    const unsigned grid = which == 1 ? (unsigned)b.nfc * (unsigned)b.nmx : (unsigned)b.nz;
+    SWEM_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * len)));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), 2 * len, st, a, b);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
+template <int N>
+static int launch_angles_len(int which, int blue, const March2DArgs& ax, const March2DArgs& az, const AngleArgs& b,
+                             int F, hipStream_t st)
+{
+    constexpr int P = N >= 1024 ? 8 : N / 64;
+    return blue ? launch_angles_n<N, XBlue<N, P>>(which, ax, az, b, F, st)
+                : launch_angles_n<N, XPow2<N, P>>(which, ax, az, b, F, st);
+}
+
+int angles_supported(int n) { return march2d_blue_len(n) || (n >= 128 && n <= 4096 && !(n & (n - 1))); }
+
+int launch_angles(int which, const March2DArgs& ax, const March2DArgs& az, const AngleArgs& b, int F, hipStream_t st)
+{
+    const int n = which == 1 ? az.nkx : ax.nkx;
+    const int m = march2d_blue_len(n);
+    const int len = m ? m : n;
+    if (!angles_supported(n) || (m && !(which == 1 ? az : ax).chirp))
+        return set_error(SWEM_EUNSUPPORTED, "angles: transform length %d", n);
+    if (b.nmx < 1 || b.nz < 1 || b.nfc < 1 || (which == 2 && b.nmx > n) || (which == 1 && b.nz > n))
+        return set_error(SWEM_EINVAL, "angles: inconsistent launch (nmx=%d nz=%d n=%d)", b.nmx, b.nz, n);
+    switch (len) {
+    case 128: return launch_angles_len<128>(which, m != 0, ax, az, b, F, st);
+    case 256: return launch_angles_len<256>(which, m != 0, ax, az, b, F, st);
+    case 512: return launch_angles_len<512>(which, m != 0, ax, az, b, F, st);
+    case 1024: return launch_angles_len<1024>(which, m != 0, ax, az, b, F, st);
+    case 2048: return launch_angles_len<2048>(which, m != 0, ax, az, b, F, st);
+    case 4096: return launch_angles_len<4096>(which, m != 0, ax, az, b, F, st);
+    }
+    return set_error(SWEM_EUNSUPPORTED, "angles: transform length %d", n);
+}
+
 }  // namespace swem

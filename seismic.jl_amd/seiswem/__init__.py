@@ -8,4 +8,5 @@ from .wem import DeviceArray, Plan, ShotProfileWEM, WemGeometry, copy_device, sy
 from .serve import Server  # noqa: F401
 from .files import ShotProfileWEMFiles  # noqa: F401
 from . import seis, solvers  # noqa: F401
+from .angles import ShotProfileWEMAngles, compute_angles  # noqa: F401
 from . import synth  # noqa: F401
