@@ -104,6 +104,13 @@ int swem_device_count(int* count);
  * Julia ccall of ShotProfileWEM.jl:347 binds to it unchanged. */
 int swem_shot_main(int argc, char** argv);
 
+/* Replaces compute_angles.so's `int main(int argc, char** argv)` (compute_angles.c:12-118): keys
+ * angx, angy, vel, wav (required), dipx, dipy (required with dip_flag=y), dip_flag, sx, sy, sz,
+ * fmin, fmax, verbose; writes the angle files angx and angy.  compute_angles.so built beside
+ * libseiswem.so exports it as `main` for ComputeAngles.jl's ccall.  Returns SWEM_EINVAL instead
+ * of calling exit(1) when a required name is missing. */
+int swem_angles_main(int argc, char** argv);
+
 /* Replaces wem() (wem.c:3-11) with host arrays: adj=1 migrates d [nm][nt] into m [nm][nz]
  * (overwritten), adj=0 models m into d (overwritten).  pspi selects PSPI (compat mode) or SSF. */
 int swem_wem(float* d, float* m, const float* wav,

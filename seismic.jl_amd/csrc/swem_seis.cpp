@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "swem_internal.hpp"
@@ -330,3 +331,85 @@ int swem_shot_main(int argc, char** argv)
 }
 
 }  // extern "C"
+
+// Drop-in for compute_angles.so's main (compute_angles.c:12-118), the library ComputeAngles.jl
+// calls through ccall((:main, "compute_angles")): the angles of one shot from the velocity and
+// wavelet files, written as angx / angy with the velocity's headers and sx set.  Geometry comes
+// from the trace headers as the reference reads it (nt = n1 of the wavelet file, ot / dt from its
+// first trace header; nz = n1 and oz / dz from the first velocity trace header, nx / ny from the
+// imx / imy range, ox = mx of the first trace, dx from the first two, dy as compute_angles.c:63).
+// dipx / dipy are read only with dip_flag=y (the reference requires the names either way).
+int swem_angles_main(int argc, char** argv)
+{
+    bool verbose = false, dip_flag = false;
+    std::string angx_name, angy_name, vel_name, wav_name, dipx_name, dipy_name;
+    float sz = 0, sx = 0, sy = 0, fmin = 0, fmax = 80;
+    par_bool(argc, argv, "verbose", verbose);
+    par_bool(argc, argv, "dip_flag", dip_flag);
+    if (!par_string(argc, argv, "angx", angx_name) || !par_string(argc, argv, "angy", angy_name) ||
+        !par_string(argc, argv, "vel", vel_name) || !par_string(argc, argv, "wav", wav_name) ||
+        (dip_flag && (!par_string(argc, argv, "dipx", dipx_name) || !par_string(argc, argv, "dipy", dipy_name)))) {
+        std::fprintf(stderr, "compute_angles: source incidence angles of one shot from .seis files;"
+                             " required: angx= angy= vel= wav= (dipx= dipy= with dip_flag=y)\n");
+        return set_error(SWEM_EINVAL, "compute_angles: angx=, angy=, vel= and wav= are required");
+    }
+    par_float(argc, argv, "sz", sz);
+    par_float(argc, argv, "sx", sx);
+    par_float(argc, argv, "sy", sy);
+    par_float(argc, argv, "fmin", fmin);
+    par_float(argc, argv, "fmax", fmax);
+
+    swem_seis_extent fh;
+    swem_seis_extent_init(&fh);
+    SWEM_TRY(swem_seis_read_header(wav_name.c_str(), &fh));
+    const int nt = fh.n1;
+    const size_t nwtr = std::max<size_t>((size_t)fh.n2 * fh.n3 * fh.n4 * fh.n5, 1);
+    std::vector<float> wav((size_t)nt * nwtr);
+    std::vector<SeisHeader> hw(nwtr);
+    SWEM_TRY(swem_seis_read(wav_name.c_str(), wav.data(), hw.data(), &fh));
+    const float ot = hw[0].o1, dt = hw[0].d1;
+
+    swem_seis_extent_init(&fh);
+    SWEM_TRY(swem_seis_read_header(vel_name.c_str(), &fh));
+    const int nz = fh.n1;
+    const size_t ntraces = (size_t)fh.n2 * fh.n3 * fh.n4 * fh.n5;
+    if (ntraces < 2) return set_error(SWEM_EINVAL, "compute_angles: velocity %s has %zu traces", vel_name.c_str(), ntraces);
+    std::vector<float> vel((size_t)nz * ntraces);
+    std::vector<SeisHeader> hv(ntraces);
+    SWEM_TRY(swem_seis_read(vel_name.c_str(), vel.data(), hv.data(), &fh));
+    const int nx = hv[ntraces - 1].imx - hv[0].imx + 1;
+    const int ny = hv[ntraces - 1].imy - hv[0].imy + 1;
+    if (nx < 1 || ny < 1 || (size_t)nx * ny != ntraces)
+        return set_error(SWEM_EINVAL, "compute_angles: velocity headers give nx=%d ny=%d for %zu traces", nx, ny, ntraces);
+    const float oz = hv[0].o1, dz = hv[0].d1;
+    const float ox = hv[0].mx, dx = hv[1].mx - hv[0].mx;
+    const float oy = hv[0].my;
+    const float dy = ntraces > (size_t)nx ? hv[nx].my - hv[nx - 1].my : dx;
+    const size_t nm = (size_t)nx * ny;
+    std::vector<float> angx(nm * nz), angy(nm * nz);
+    SWEM_TRY(swem_compute_angles(angx.data(), angy.data(), wav.data(), nt, ot, dt, nx, ox, dx, ny, oy, dy, sx, sy, nz,
+                                 oz, dz, sz, vel.data(), fmin, fmax, verbose ? 1 : 0));
+    if (dip_flag) {  // compute_angles.c:84-100: angles relative to the reflector normal
+        for (int k = 0; k < 2; k++) {
+            const std::string& name = k == 0 ? dipx_name : dipy_name;
+            std::vector<float>& ang = k == 0 ? angx : angy;
+            swem_seis_extent_init(&fh);
+            SWEM_TRY(swem_seis_read_header(name.c_str(), &fh));
+            if (fh.n1 != nz || (size_t)fh.n2 * fh.n3 * fh.n4 * fh.n5 != ntraces)
+                return set_error(SWEM_EINVAL, "compute_angles: %s does not match the velocity grid", name.c_str());
+            std::vector<float> dip((size_t)nz * ntraces);
+            std::vector<SeisHeader> hd(ntraces);
+            SWEM_TRY(swem_seis_read(name.c_str(), dip.data(), hd.data(), &fh));
+            for (size_t i = 0; i < nm * nz; i++) ang[i] += dip[i];
+        }
+    }
+    std::vector<SeisHeader> ha(hv.begin(), hv.begin() + nm);
+    for (auto& h : ha) h.sx = sx;
+    swem_seis_extent_init(&fh);
+    fh.n1 = nz; fh.o1 = oz; fh.d1 = dz;
+    fh.n2 = nx; fh.o2 = ox; fh.d2 = dx;
+    fh.n3 = ny; fh.o3 = oy; fh.d3 = dy;
+    SWEM_TRY(swem_seis_write(angx_name.c_str(), angx.data(), ha.data(), &fh));
+    SWEM_TRY(swem_seis_write(angy_name.c_str(), angy.data(), ha.data(), &fh));
+    return SWEM_OK;
+}

@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libseiswem.so")
 SHOTWEM_PATH = os.path.join(LIB_DIR, "shotwem.so")
+ANGLES_PATH = os.path.join(LIB_DIR, "compute_angles.so")
 
 SWEM_OK, SWEM_EINVAL, SWEM_ENOMEM, SWEM_EDEVICE, SWEM_EIO, SWEM_EUNSUPPORTED, SWEM_EBUSY = range(7)
 PSPI_COMPAT, PSPI_LINEAR = 0, 1
@@ -21,7 +22,7 @@ VARIANT_AUTO, VARIANT_SUBGROUP, VARIANT_WAVE, VARIANT_SUBGROUP16, VARIANT_PAIR, 
 # Every symbol declared in include/seiswem.h.
 EXPORTS = (
     "swem_version", "swem_last_error", "swem_options_init", "swem_device_count",
-    "swem_shot_main", "swem_wem", "swem_wem_ex", "swem_shots", "swem_compute_angles",
+    "swem_shot_main", "swem_angles_main", "swem_wem", "swem_wem_ex", "swem_shots", "swem_compute_angles",
     "swem_plan_create", "swem_plan_destroy", "swem_plan_migrate", "swem_plan_migrate_each",
     "swem_plan_model", "swem_plan_timing", "swem_plan_bytes",
     "swem_malloc", "swem_free", "swem_memcpy_htod", "swem_memcpy_dtoh", "swem_memcpy_dtod", "swem_memset", "swem_synchronize",
@@ -137,6 +138,7 @@ def load():
         "swem_options_init": ([op_], None),
         "swem_device_count": ([i32p], ctypes.c_int),
         "swem_shot_main": ([ctypes.c_int, P(ctypes.c_char_p)], ctypes.c_int),
+        "swem_angles_main": ([ctypes.c_int, P(ctypes.c_char_p)], ctypes.c_int),
         "swem_wem": ([f32p, f32p, f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                       ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                       ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,

@@ -40,6 +40,22 @@ def test_shotwem_alias_exports_main():
     assert re.search(r"\bT main$", out, re.M)
 
 
+def test_compute_angles_alias_exports_main():
+    out = subprocess.run(["nm", "-D", "--defined-only", seiswem.ANGLES_PATH], capture_output=True, text=True).stdout
+    assert re.search(r"\bT main$", out, re.M)
+
+
+def test_compute_angles_main_missing_argument_returns_error():
+    """compute_angles.c:29-34 calls exit(1) in the caller's process; the replacement returns EINVAL."""
+    so = ctypes.CDLL(seiswem.ANGLES_PATH)
+    so.main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    so.main.restype = ctypes.c_int
+    for args in ([b"angx=a", b"angy=b", b"vel=v"], [b"angx=a", b"angy=b", b"vel=v", b"wav=w", b"dip_flag=y"]):
+        argv = [b"0"] + args
+        assert so.main(len(argv), (ctypes.c_char_p * len(argv))(*argv)) == _lib.SWEM_EINVAL
+    assert "required" in seiswem.load().swem_last_error().decode()
+
+
 def geoms(rng, n=200):
     for _ in range(n):
         yield seiswem.WemGeometry(

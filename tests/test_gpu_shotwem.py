@@ -126,3 +126,43 @@ def test_shot_profile_wem_files_stacks_and_concatenates_like_the_reference(gpu, 
         assert np.array_equal(hk["hx"], mx.astype(np.float32) - sx[k]) and np.all(hk["n1"] == nt)
         d_ref = O.wem(p, vel, wav, float(sx[k]), m=m, adj=False)
         assert rel_l2(dm[k], d_ref) <= 1e-4
+
+
+def test_compute_angles_main_writes_the_angle_files(gpu, tmp_path, monkeypatch):
+    """compute_angles.so's main (compute_angles.c:12-118) from .seis files: the angles equal the
+    in-memory swem_compute_angles of the same grid, the headers are the velocity's with sx set,
+    and dip_flag=y adds the dip files."""
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("DATAPATH", raising=False)
+    lib = seiswem.load()
+    nx, nz, nt, dx, dz, dt = 128, 60, 256, 10.0, 10.0, 0.004
+    rng = np.random.default_rng(5)
+    vel = synth.blob_velocity(nx, nz, dx, dz, rng, nblobs=2, v0=2000.0)
+    wav = synth.wavelet_trace(nt, dt, 20.0)
+    mx = dx * np.arange(nx)
+    hv = header_array(nx, tracenum=np.arange(1, nx + 1), n1=nz, o1=0.0, d1=dz, mx=mx, imx=np.arange(nx))
+    write_seis(lib, "vel.seis", vel, hv, nz, nx, o1=0.0, d1=dz, d2=dx)
+    write_seis(lib, "wav.seis", wav[None], header_array(1, n1=nt, o1=0.0, d1=dt), nt, 1, o1=0.0, d1=dt)
+    dip = np.full((nx, nz), 2.5, np.float32)
+    write_seis(lib, "dipx.seis", dip, hv, nz, nx, o1=0.0, d1=dz, d2=dx)
+    write_seis(lib, "dipy.seis", dip, hv, nz, nx, o1=0.0, d1=dz, d2=dx)
+    so = ctypes.CDLL(seiswem.ANGLES_PATH)
+    so.main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    so.main.restype = ctypes.c_int
+
+    def run(args):
+        argv = [b"0"] + [a.encode() for a in args]
+        return so.main(len(argv), (ctypes.c_char_p * len(argv))(*argv))
+
+    common = ["vel=vel.seis", "wav=wav.seis", "sx=640", "fmin=5", "fmax=40", "dipx=dipx.seis", "dipy=dipy.seis"]
+    assert run(["angx=ax.seis", "angy=ay.seis"] + common) == 0
+    ax, hx, fh = read_seis("ax.seis", nz, nx)
+    ay, _, _ = read_seis("ay.seis", nz, nx)
+    g = seiswem.WemGeometry(nt=nt, dt=dt, nmx=nx, dmx=dx, nz=nz, dz=dz, fmin=5.0, fmax=40.0, pspi=False)
+    rx, ry = seiswem.compute_angles(vel, wav, g, 640.0)
+    assert np.array_equal(ax, rx) and np.array_equal(ay, ry)
+    assert np.all(hx["sx"] == 640.0) and np.array_equal(hx["imx"], np.arange(nx)) and (fh.n1, fh.n2) == (nz, nx)
+    assert run(["angx=bx.seis", "angy=by.seis", "dip_flag=y"] + common) == 0
+    bx, _, _ = read_seis("bx.seis", nz, nx)
+    by, _, _ = read_seis("by.seis", nz, nx)
+    assert np.array_equal(bx, rx + np.float32(2.5)) and np.array_equal(by, ry + np.float32(2.5))
