@@ -166,3 +166,51 @@ def test_compute_angles_main_writes_the_angle_files(gpu, tmp_path, monkeypatch):
     bx, _, _ = read_seis("bx.seis", nz, nx)
     by, _, _ = read_seis("by.seis", nz, nx)
     assert np.array_equal(bx, rx + np.float32(2.5)) and np.array_equal(by, ry + np.float32(2.5))
+
+
+def test_shot_profile_wem_files_angle_gathers(gpu, tmp_path, monkeypatch):
+    """ShotProfileWEM with nangx > 1 on files (ShotProfileWEM.jl:93-201, 240-275): angle files
+    from compute_angles.so for two shots concatenated as ComputeAngles.jl writes them; the gathers
+    and their headers, and the modelled shots, against ShotProfileWEMAngles on arrays."""
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("DATAPATH", raising=False)
+    lib = seiswem.load()
+    nx, nz, nt, dx, dz, dt = 64, 40, 256, 10.0, 10.0, 0.004
+    rng = np.random.default_rng(8)
+    vel, wav, mx = _grid_files(lib, nx, nz, nt, dx, dz, dt, rng)
+    sx = np.array([200.0, 440.0], np.float32)
+    so = ctypes.CDLL(seiswem.ANGLES_PATH)
+    so.main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    so.main.restype = ctypes.c_int
+    parts = {"x": [], "y": []}
+    for k, s in enumerate(sx):
+        argv = [b"0"] + [a.encode() for a in (f"angx=ax{k}.seis", f"angy=ay{k}.seis", "vel=vel.seis", "wav=wav.seis",
+                                              f"sx={s}", "fmax=40")]
+        assert so.main(len(argv), (ctypes.c_char_p * len(argv))(*argv)) == 0
+        for a in "xy":
+            parts[a].append(seiswem.seis.read(f"a{a}{k}.seis"))
+    for a in "xy":  # ComputeAngles.jl:89-99: shots appended to one file
+        data = np.concatenate([p[0] for p in parts[a]])
+        hh = np.concatenate([p[1] for p in parts[a]])
+        hh["tracenum"] = np.arange(1, hh.size + 1)
+        seiswem.seis.write(f"ang{a}.seis", data, hh, parts[a][0][2])
+    d = rng.standard_normal((2, nx, nt)).astype(np.float32)
+    h = np.concatenate([header_array(nx, n1=nt, d1=dt, sx=sx[k], gx=mx) for k in range(2)])
+    write_seis(lib, "d.seis", d.reshape(-1, nt), h, nt, 2 * nx, o1=0.0, d1=dt)
+    akw = dict(nangx=6, oangx=-45.0, dangx=15.0)
+    common = dict(vel="vel.seis", wav="wav.seis", angx="angx.seis", angy="angy.seis", fmax=40.0, pspi=False,
+                  padt=2, padx=2, **akw)
+    gath = seiswem.ShotProfileWEMFiles("m.seis", "d.seis", True, sx=sx, sy=np.zeros(2, np.float32), **common)
+    g = seiswem.WemGeometry(nt=nt, dt=dt, nmx=nx, dmx=dx, nz=nz, dz=dz, fmax=40.0, pspi=False)
+    angx = np.stack([parts["x"][k][0] for k in range(2)])
+    angy = np.stack([parts["y"][k][0] for k in range(2)])
+    ref = seiswem.ShotProfileWEMAngles(d, True, vel=vel, wav=wav, geom=g, sx=sx, angx=angx, angy=angy, **akw)
+    assert rel_l2(gath, ref) <= 1e-6
+    mr, hm, fh = seiswem.seis.read("m.seis")
+    assert (fh.n1, fh.n2, fh.n3, fh.n4, fh.n5) == (nz, nx, 1, 6, 1) and abs(fh.o4 + 45.0) < 1e-4
+    assert np.array_equal(mr, gath.reshape(nx * 6, nz))
+    assert np.array_equal(hm["iang"], np.tile(np.arange(6), nx)) and np.array_equal(hm["imx"], np.repeat(np.arange(nx), 6))
+    assert np.allclose(hm["ang"], np.tile(-45.0 + 15.0 * np.arange(6), nx))
+    dm = seiswem.ShotProfileWEMFiles("m.seis", "dm.seis", False, sx=sx, sy=np.zeros(2, np.float32), **common)
+    dref = seiswem.ShotProfileWEMAngles(gath, False, vel=vel, wav=wav, geom=g, sx=sx, angx=angx, angy=angy, **akw)
+    assert rel_l2(dm, dref) <= 1e-6
