@@ -306,6 +306,48 @@ def plain_single_shot(args, g, vel, wav, sx, d, plan, batched):
     }
 
 
+def bench_angles(args):
+    """Source incidence angles (compute_angles, wem.c:566-992) on the config-2 grid: nx=1024,
+    nz=500, nt=2000 (ntfft = nt: 321 frequencies to 80 Hz), one shot per call of the drop-in
+    swem_compute_angles (host arrays in and out, plan and volume buffers inside the call), shots
+    spread over the line.  CPU baseline: the FP32 oracle on one shot of a reduced band."""
+    import seiswem
+    from seiswem import synth
+
+    g, vel, _, wav, sx = synth.config2(nshot=args.shots)
+    geo = seiswem.WemGeometry(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, fmin=0.0, fmax=80.0,
+                              pspi=False)
+    for s in sx[: args.warmup]:
+        seiswem.compute_angles(vel, wav, geo, float(s))
+    k = min(args.steps, sx.size)
+    t0 = time.perf_counter()
+    for s in sx[:k]:
+        angx, _ = seiswem.compute_angles(vel, wav, geo, float(s))
+    el = time.perf_counter() - t0
+    out = {
+        "metric": "source incidence angles: shots/sec (compute_angles, 2-D)", "value": k / el, "unit": "shots/s",
+        "n_gpus": 1, "steps": k, "warmup": args.warmup, "ms_per_step": el / k * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (config-2 velocity generator)",
+        "config": {"workload": "compute_angles, nx=1024 nz=500 nt=2000 dt=2 ms, 0-80 Hz (321 bins), one shot per call"},
+    }
+    if not args.no_cpu:
+        import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+        p = O.WemParams(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, fmin=76.0, fmax=80.0, pspi=False)
+        t0 = time.perf_counter()
+        O.angles(p, vel, wav, float(sx[0]), f64=False)
+        tc = time.perf_counter() - t0
+        sp = seiswem.WemGeometry(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, fmin=76.0, fmax=80.0,
+                                 pspi=False, padt=1, padx=1).sampling()
+        nb = sp["ifmax"] - sp["ifmin"]  # 17 bins
+        out["cpu_baseline"] = {"value": nb / tc / 321, "unit": "shots/s", "cores": 1, "kind": "port",
+                               "sample": f"1 shot, {nb} of 321 frequency bins (76-80 Hz) at full nx/nz, {tc:.2f} s, "
+                                         "scaled by bins (the FP32 oracle is serial over frequencies)"}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,7 +359,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="pspi2d: shots timed one per call on the plain and resident single-shot paths (0: skip)")
-    ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm"), default="pspi2d")
+    ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm", "angles"), default="pspi2d")
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--plain-sample", type=int, default=24, help="serving: requests timed on the plain path")
@@ -336,6 +378,8 @@ def main():
         return bench_serving(args)
     if args.workload == "lsm":
         return bench_lsm(args)
+    if args.workload == "angles":
+        return bench_angles(args)
 
     import seiswem
     from seiswem import synth
