@@ -1146,29 +1146,21 @@ int swem_wem(float* d, float* m, const float* wav, int nt, float ot, float dt, i
 // compute_angles (wem.c:566-692) on the device, 2-D: the plan is built with padt = padx = 1 and
 // split-step (ntfft = nt, nkx = nmx, wem.c:588-594); per frequency chunk the source march stores
 // the volume, the column pass its z derivative and the row pass sums the angles and signs.
+static int angles3d_impl(swem_plan* p, float sx, float sy, float* angx, float* angy);
+static int angle_z_tables(int nz, DevBuf<float2>& twz, DevBuf<float2>& chz, DevBuf<float2>& bchz, March2DArgs& az);
+
 static int angles_impl(swem_plan* p, float sx, float sy, float* angx, float* angy)
 {
+    if (p->three_d) return angles3d_impl(p, sx, sy, angx, angy);
     hipStream_t st = nullptr;
     const int nmx = p->g.nmx, nz = p->g.nz;
     const size_t nmz = (size_t)nmx * nz;
     for (size_t i = 0; i < nmz; i++) angx[i] = angy[i] = 0.f;
     if (p->nf < 1) return SWEM_OK;
-    // z transforms: twiddles of the array length, Bluestein tables for other lengths
-    const int mz = march2d_blue_len(nz), nza = mz ? mz : nz;
     DevBuf<float2> twz, chz, bchz, vol;
     DevBuf<float> dzmag, acc, sgn;
-    std::vector<float2> tw(nza);
-    for (int m = 0; m < nza; m++) {
-        const double a = -2.0 * kPi * (double)m / (double)nza;
-        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    SWEM_TRY(twz.upload(tw.data(), tw.size()));
-    if (mz) {
-        std::vector<float2> chirp, bchirp;
-        bluestein_tables(nz, mz, chirp, bchirp);
-        SWEM_TRY(chz.upload(chirp.data(), chirp.size()));
-        SWEM_TRY(bchz.upload(bchirp.data(), bchirp.size()));
-    }
+    March2DArgs az;
+    SWEM_TRY(angle_z_tables(nz, twz, chz, bchz, az));
     SWEM_TRY(vol.alloc((size_t)p->nfc * nmz));
     SWEM_TRY(dzmag.alloc((size_t)p->nfc * nmz));
     SWEM_TRY(acc.alloc(nmz));
@@ -1180,17 +1172,13 @@ static int angles_impl(swem_plan* p, float sx, float sy, float* angx, float* ang
     const size_t lds = ((size_t)(p->narr + p->narr / 8 + 31) / 32 * 32 + 4) * 2 * sizeof(float2);
     int F = std::max(1, 512 / T);
     while (F > 1 && F * lds > 80 * 1024) F--;
-    March2DArgs az{};
-    az.nkx = nz;
-    az.tw = twz.p;
-    az.chirp = chz.p;
-    az.bchirp = bchz.p;
     AngleArgs b{};
-    b.nmx = nmx;
     b.nz = nz;
     b.nw = p->s.nw;
-    b.dkx = p->s.dkx;
     b.dkz = (float)(2.0 * kPi / (double)(float)nz / (double)p->g.dz);
+    b.item = (long long)nmz;
+    b.len = nmx;
+    b.dk = p->s.dkx;
     b.vol = vol.p;
     b.dzmag = dzmag.p;
     b.acc = acc.p;
@@ -1203,7 +1191,11 @@ static int angles_impl(swem_plan* p, float sx, float sy, float* angx, float* ang
         ax.smig = vol.p;
         b.nfc = ax.nfc;
         SWEM_TRY(launch_angles(0, ax, az, b, F, st));
+        // z pass: the z line of every (frequency, x)
+        b.inner = nmx, b.lines = ax.nfc * nmx, b.outer = (long long)nmz, b.inner_stride = 1, b.elem = nmx;
         SWEM_TRY(launch_angles(1, ax, az, b, F, st));
+        // x pass: the x line of every depth
+        b.inner = 1, b.lines = nz, b.outer = nmx, b.inner_stride = 0, b.elem = 1;
         SWEM_TRY(launch_angles(2, ax, az, b, F, st));
     }
     std::vector<float> ha(nmz), hs(nmz);
@@ -1219,12 +1211,138 @@ static int angles_impl(swem_plan* p, float sx, float sy, float* angx, float* ang
     return SWEM_OK;
 }
 
+// z tables of the angle passes: twiddles of the array length, Bluestein tables for other nz.
+static int angle_z_tables(int nz, DevBuf<float2>& twz, DevBuf<float2>& chz, DevBuf<float2>& bchz, March2DArgs& az)
+{
+    const int mz = march2d_blue_len(nz), nza = mz ? mz : nz;
+    std::vector<float2> tw(nza);
+    for (int m = 0; m < nza; m++) {
+        const double a = -2.0 * kPi * (double)m / (double)nza;
+        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    SWEM_TRY(twz.upload(tw.data(), tw.size()));
+    if (mz) {
+        std::vector<float2> chirp, bchirp;
+        bluestein_tables(nz, mz, chirp, bchirp);
+        SWEM_TRY(chz.upload(chirp.data(), chirp.size()));
+        SWEM_TRY(bchz.upload(bchirp.data(), bchirp.size()));
+    }
+    az = March2DArgs{};
+    az.nkx = nz;
+    az.tw = twz.p;
+    az.chirp = chz.p;
+    az.bchirp = bchz.p;
+    return SWEM_OK;
+}
+
+// compute_angles in 3-D (nkx = nmx, nky = nmy; wem.c:588-594): per chunk of frequencies the
+// source march of the 3-D modelling receiver (x-pass with conj(L), y-pass phase 3) stores every
+// depth slice, then the z pass and the x and y passes add the angles of both axes.
+static int angles3d_impl(swem_plan* p, float sx, float sy, float* angx, float* angy)
+{
+    hipStream_t st = nullptr;
+    const int nmx = p->g.nmx, nmy = p->g.nmy, nz = p->g.nz;
+    const size_t nm = p->nm, nmz = nm * nz;
+    for (size_t i = 0; i < nmz; i++) angx[i] = angy[i] = 0.f;
+    if (p->nf < 1) return SWEM_OK;
+    DevBuf<float2> twz, chz, bchz, vol, W;
+    DevBuf<float> dzmag, accx, sgx, accy, sgy;
+    March2DArgs az;
+    SWEM_TRY(angle_z_tables(nz, twz, chz, bchz, az));
+    const size_t per_item = nmz * (sizeof(float2) + sizeof(float)) + 2 * (size_t)nmx * p->s.nky * sizeof(float2);
+    const size_t cap = std::min<size_t>(p->budget / 2, (size_t)16 << 30);
+    const int nfc = (int)std::max<size_t>(1, std::min<size_t>(p->nf, cap / per_item));
+    SWEM_TRY(vol.alloc((size_t)nfc * nmz));
+    SWEM_TRY(dzmag.alloc((size_t)nfc * nmz));
+    SWEM_TRY(W.alloc((size_t)nfc * 2 * nmx * p->s.nky));
+    for (DevBuf<float>* b : {&accx, &sgx, &accy, &sgy}) {
+        SWEM_TRY(b->alloc(nmz));
+        SWEM_HIP(hipMemsetAsync(b->p, 0, nmz * sizeof(float), st));
+    }
+    SWEM_TRY(set_igx(p, &sx, &sy, 1, st));
+    const int F = p->F3f;
+    March2DArgs ax{}, ay{};
+    ax.nkx = nmx;
+    ax.tw = p->tw.p;
+    ay.nkx = nmy;
+    ay.tw = p->twy.p;
+    AngleArgs b{};
+    b.nz = nz;
+    b.nw = p->s.nw;
+    b.dkz = (float)(2.0 * kPi / (double)(float)nz / (double)p->g.dz);
+    b.item = (long long)nmz;
+    b.vol = vol.p;
+    b.dzmag = dzmag.p;
+    for (int f0 = 0; f0 < p->nf; f0 += nfc) {
+        March3DArgs a = base3(p);
+        a.f0 = f0;
+        a.nitem = std::min(nfc, p->nf - f0);
+        a.igx = p->igx_host[0];
+        a.adj = 0;
+        a.W = W.p;
+        a.smig = vol.p;
+        a.conj_src = 1;
+        a.conj_rec = 1;
+        a.wmask = 2;
+        SWEM_HIP(hipMemsetAsync(vol.p, 0, (size_t)a.nitem * nmz * sizeof(float2), st));
+        // extrapolate_source (wem.c:712-719): the depths z >= sz step; the others stay zero
+        int prev = -1;
+        for (int iz = 0; iz < nz; iz++) {
+            if (!stepped_at(p, iz, p->g.sz)) continue;
+            a.iz = iz;
+            a.iz_sel = prev;
+            a.first = prev < 0;
+            a.ds = 1;
+            SWEM_TRY(launch_march3d(3, a, F, 3, st));
+            SWEM_TRY(launch_march3d(1, a, F, 0, st));
+            prev = iz;
+        }
+        if (prev >= 0) {
+            a.iz_sel = prev;
+            a.first = 0;
+            a.ds = 0;
+            SWEM_TRY(launch_march3d(3, a, F, 3, st));
+        }
+        b.nfc = a.nitem;
+        // z pass over every (frequency, trace)
+        b.len = nmx, b.dk = p->s.dkx;
+        b.inner = (int)nm, b.lines = a.nitem * (int)nm, b.outer = (long long)nmz, b.inner_stride = 1, b.elem = (long long)nm;
+        SWEM_TRY(launch_angles(1, ax, az, b, 1, st));
+        // x lines: (iz, y), points nmy apart
+        b.acc = accx.p, b.sgn = sgx.p;
+        b.len = nmx, b.dk = p->s.dkx;
+        b.inner = nmy, b.lines = nz * nmy, b.outer = (long long)nm, b.inner_stride = 1, b.elem = nmy;
+        SWEM_TRY(launch_angles(2, ax, az, b, 1, st));
+        // y lines: (iz, x), contiguous
+        b.acc = accy.p, b.sgn = sgy.p;
+        b.len = nmy, b.dk = p->s.dky;
+        b.inner = nmx, b.lines = nz * nmx, b.outer = (long long)nm, b.inner_stride = nmy, b.elem = 1;
+        SWEM_TRY(launch_angles(2, ay, az, b, 1, st));
+    }
+    std::vector<float> hax(nmz), hsx(nmz), hay(nmz), hsy(nmz);
+    SWEM_HIP(hipMemcpyAsync(hax.data(), accx.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
+    SWEM_HIP(hipMemcpyAsync(hsx.data(), sgx.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
+    SWEM_HIP(hipMemcpyAsync(hay.data(), accy.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
+    SWEM_HIP(hipMemcpyAsync(hsy.data(), sgy.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
+    SWEM_HIP(hipStreamSynchronize(st));
+    const float inv = 1.f / (float)(p->nf + 1);
+    for (size_t i = 0; i < nm; i++)
+        for (int iz = 0; iz < nz; iz++) {
+            const size_t k = (size_t)iz * nm + i;
+            angx[i * nz + iz] = hax[k] * ((hsx[k] >= 0.f ? 1.f : -1.f) * inv);
+            angy[i * nz + iz] = hay[k] * ((hsy[k] >= 0.f ? 1.f : -1.f) * inv);
+        }
+    return SWEM_OK;
+}
+
 int swem_compute_angles(float* angx, float* angy, const float* wav, int nt, float ot, float dt, int nmx, float omx,
                         float dmx, int nmy, float omy, float dmy, float sx, float sy, int nz, float oz, float dz,
                         float sz, const float* vel, float fmin, float fmax, int verbose)
 {
     if (!angx || !angy || !wav || !vel) return set_error(SWEM_EINVAL, "compute_angles: null argument");
-    if (nmy > 1) return set_error(SWEM_EUNSUPPORTED, "compute_angles: 3-D (nmy > 1) is not available in this build");
+    if (nmy > 1 && !march3d_supported(nmx, nmy))
+        return set_error(SWEM_EUNSUPPORTED, "compute_angles: 3-D needs nmx and nmy powers of two 128..1024 (%d x %d)",
+                         nmx, nmy);
     if (nmx < 2 || !angles_supported(nmx) || !angles_supported(nz))
         return set_error(SWEM_EUNSUPPORTED, "compute_angles: nmx = %d, nz = %d: this build handles 2..2048 and powers of "
                          "two to 4096 (nmx >= 2)", nmx, nz);

@@ -88,17 +88,21 @@ struct March2DArgs {
     float2* dgw;            // [nshot][nmx][nf] modelled spectra (modelling output)
 };
 
-// Source incidence angles (compute_angles, wem.c:566-992), 2-D.  The march writes every depth of
-// the source wavefield to vol; the column pass writes |u_z| (z derivative) to dzmag; the row pass
-// adds the x derivative angles and the left / right signs of every frequency, in order, to acc /
-// sgn.  ax: x transforms (nkx = nmx, tables of the plan); az: z transforms (length nz).
+// Source incidence angles (compute_angles, wem.c:566-992).  The march writes every depth of the
+// source wavefield to vol [nfc][nz][nm] (trace ix * nmy + iy); the z pass writes |u_z| to dzmag;
+// the lateral pass, along x or y lines, adds that axis's angles and left / right signs of every
+// frequency, in order, to acc / sgn [nz][nm].  A line is `len` points `elem` apart starting at
+// (l / inner) * outer + (l % inner) * inner_stride for line l (z pass: l over nm points of one
+// frequency; lateral pass: l over the nz * (nm / len) lines of a depth).  ax: lateral transforms
+// (length len, tables of the plan or of y); az: z transforms (length nz).
 struct AngleArgs {
-    int nmx, nz, nfc, nw;
-    float dkx, dkz;         // wavenumber spacings (float, as the reference)
-    const float2* vol;      // [nfc][nz][nmx]
-    float* dzmag;           // [nfc][nz][nmx]
-    float* acc;             // [nz][nmx] sum of angles (degrees)
-    float* sgn;             // [nz][nmx] sum of signed |u| / nw
+    int len, inner, lines, nfc, nw, nz;  // lines: workgroups of the pass
+    long long outer, inner_stride, elem, item;  // item: distance between frequencies in vol / dzmag
+    float dk, dkz;          // wavenumber spacing of the line's axis and of z (float, as the reference)
+    const float2* vol;      // [nfc][nz][nm]
+    float* dzmag;           // [nfc][nz][nm]
+    float* acc;             // [nz][nm] sum of angles (degrees)
+    float* sgn;             // [nz][nm] sum of signed |u| / nw
 };
 // which: 0 = march (F frequencies per workgroup), 1 = column pass, 2 = row pass.
 int launch_angles(int which, const March2DArgs& ax, const March2DArgs& az, const AngleArgs& b, int F, hipStream_t st);

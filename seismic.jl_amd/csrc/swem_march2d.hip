@@ -1191,21 +1191,22 @@ __device__ __forceinline__ float kbin(int k, int n, float dk)
 
 __device__ __forceinline__ float2 times_i(float2 v, float k) { return make_float2(-v.y * k, v.x * k); }
 
-// Column pass (wem.c:888-899): |IFFT_z(i kz FFT_z(u))| / nz for one (frequency, x) column.
+// z pass (wem.c:888-899): |IFFT_z(i kz FFT_z(u))| / nz along the z line of one (frequency, point).
 template <int N, int P, class X>
 __global__ __launch_bounds__(N / P) void angle_cols(March2DArgs az, AngleArgs b)
 {
     constexpr int T = N / P;
     extern __shared__ float2 smem[];
-    const int f = blockIdx.x / b.nmx, x = blockIdx.x - f * b.nmx;
+    const int l = blockIdx.x;
+    const size_t base = (size_t)(l / b.inner) * b.outer + (size_t)(l % b.inner) * b.inner_stride;
     const int t = threadIdx.x;
     int pp = 0;
-    const float2* col = b.vol + (size_t)f * b.nz * b.nmx + x;
+    const float2* col = b.vol + base;
     float2 v[P];
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const int iz = t + q * T;
-        v[q] = iz < b.nz ? col[(size_t)iz * b.nmx] : make_float2(0.f, 0.f);
+        v[q] = iz < b.nz ? col[(size_t)iz * b.elem] : make_float2(0.f, 0.f);
     }
     __builtin_amdgcn_sched_barrier(0);
     X::template fft<-1, false>(v, smem, az, t, pp);
@@ -1215,24 +1216,25 @@ __global__ __launch_bounds__(N / P) void angle_cols(March2DArgs az, AngleArgs b)
         v[q] = times_i(v[q], k < b.nz ? kbin(k, b.nz, b.dkz) : 0.f);
     }
     X::template fft<+1, false>(v, smem, az, t, pp);
-    float* out = b.dzmag + (size_t)f * b.nz * b.nmx + x;
+    float* out = b.dzmag + base;
     const float inv = 1.f / (float)b.nz;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const int iz = t + q * T;
-        if (iz < b.nz) out[(size_t)iz * b.nmx] = sqrtf(v[q].x * v[q].x + v[q].y * v[q].y) * inv;
+        if (iz < b.nz) out[(size_t)iz * b.elem] = sqrtf(v[q].x * v[q].x + v[q].y * v[q].y) * inv;
     }
 }
 
-// Row pass for depth iz: per frequency in order, the x derivative (wem.c:858-875) against the
-// column pass's z derivative (wem.c:747-748) and the left / right split (wem.c:921-945,
-// wem.c:739-740), added to the sums of earlier chunks.
+// Lateral pass along one x (or y) line of one depth: per frequency in order, the derivative along
+// the line (wem.c:858-887) against the z pass's |u_z| (wem.c:747-748) and the left / right split
+// (wem.c:901-970, wem.c:739-740), added to the sums of earlier chunks.
 template <int N, int P, class X>
 __global__ __launch_bounds__(N / P) void angle_rows(March2DArgs ax, AngleArgs b)
 {
     constexpr int T = N / P;
     extern __shared__ float2 smem[];
-    const int iz = blockIdx.x;
+    const int l = blockIdx.x;
+    const size_t base = (size_t)(l / b.inner) * b.outer + (size_t)(l % b.inner) * b.inner_stride;
     const int t = threadIdx.x;
     int pp = 0;
     const int n = ax.nkx;
@@ -1240,24 +1242,24 @@ __global__ __launch_bounds__(N / P) void angle_rows(March2DArgs ax, AngleArgs b)
     const float inv = 1.f / (float)n;
     const float inv_nw = 1.f / (float)b.nw;
     float acc[P], sgn[P], kx[P];
-    const size_t row = (size_t)iz * b.nmx;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const int x = t + q * T;
-        acc[q] = x < b.nmx ? b.acc[row + x] : 0.f;
-        sgn[q] = x < b.nmx ? b.sgn[row + x] : 0.f;
-        kx[q] = x < n ? kbin(x, n, b.dkx) : 0.f;
+        const size_t i = base + (size_t)x * b.elem;
+        acc[q] = x < b.len ? b.acc[i] : 0.f;
+        sgn[q] = x < b.len ? b.sgn[i] : 0.f;
+        kx[q] = x < n ? kbin(x, n, b.dk) : 0.f;
     }
     for (int f = 0; f < b.nfc; ++f) {
-        const float2* ur = b.vol + ((size_t)f * b.nz + iz) * b.nmx;
-        const float* zr = b.dzmag + ((size_t)f * b.nz + iz) * b.nmx;
+        const float2* ur = b.vol + (size_t)f * b.item + base;
+        const float* zr = b.dzmag + (size_t)f * b.item + base;
         float2 A[P];
         float mu[P], mz[P];
 #pragma unroll
         for (int q = 0; q < P; ++q) {
             const int x = t + q * T;
-            A[q] = x < b.nmx ? ur[x] : make_float2(0.f, 0.f);
-            mz[q] = x < b.nmx ? zr[x] : 0.f;
+            A[q] = x < b.len ? ur[(size_t)x * b.elem] : make_float2(0.f, 0.f);
+            mz[q] = x < b.len ? zr[(size_t)x * b.elem] : 0.f;
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1289,9 +1291,10 @@ __global__ __launch_bounds__(N / P) void angle_rows(March2DArgs ax, AngleArgs b)
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const int x = t + q * T;
-        if (x < b.nmx) {
-            b.acc[row + x] = acc[q];
-            b.sgn[row + x] = sgn[q];
+        if (x < b.len) {
+            const size_t i = base + (size_t)x * b.elem;
+            b.acc[i] = acc[q];
+            b.sgn[i] = sgn[q];
         }
     }
 }
@@ -1308,7 +1311,7 @@ static int launch_angles_n(int which, const March2DArgs& ax, const March2DArgs& 
     }
     auto k = which == 1 ? angle_cols<N, P, X> : angle_rows<N, P, X>;
     const March2DArgs& a = which == 1 ? az : ax;
-    const unsigned grid = which == 1 ? (unsigned)b.nfc * (unsigned)b.nmx : (unsigned)b.nz;
+    const unsigned grid = (unsigned)b.lines;
     SWEM_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * len)));
     hipLaunchKernelGGL(k, dim3(grid), dim3(T), 2 * len, st, a, b);
     SWEM_HIP(hipGetLastError());
@@ -1333,8 +1336,9 @@ int launch_angles(int which, const March2DArgs& ax, const March2DArgs& az, const
     const int len = m ? m : n;
     if (!angles_supported(n) || (m && !(which == 1 ? az : ax).chirp))
         return set_error(SWEM_EUNSUPPORTED, "angles: transform length %d", n);
-    if (b.nmx < 1 || b.nz < 1 || b.nfc < 1 || (which == 2 && b.nmx > n) || (which == 1 && b.nz > n))
-        return set_error(SWEM_EINVAL, "angles: inconsistent launch (nmx=%d nz=%d n=%d)", b.nmx, b.nz, n);
+    if (b.len < 1 || b.nz < 1 || b.nfc < 1 || (which > 0 && (b.inner < 1 || b.lines < 1)) ||
+        (which == 2 && b.len != n) || (which == 1 && b.nz != n))
+        return set_error(SWEM_EINVAL, "angles: inconsistent launch (len=%d nz=%d n=%d)", b.len, b.nz, n);
     switch (len) {
     case 128: return launch_angles_len<128>(which, m != 0, ax, az, b, F, st);
     case 256: return launch_angles_len<256>(which, m != 0, ax, az, b, F, st);

@@ -599,6 +599,10 @@ __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
 //   phase 1 (receiver march, upward): IFFTy of the previous step's result (skipped at the first
 //           step, whose state is zero), inject m * smig, taper, conj(SS), FFTy.
 //   phase 2 (output): IFFTy of the last step's result, write the spectra of the record.
+//   phase 3 (incidence angles, compute_angles wem.c:712-719): the source marched with the
+//           receiver's step; at the first call the state is the source trace, later IFFTy of
+//           the previous step's result is stored in smig at depth iz_sel; with ds the next step
+//           starts (taper, conj(SS), FFTy).
 template <int NY>
 __global__ __launch_bounds__(1024) void march3d_rows_fwd(March3DArgs a, int F, int phase)
 {
@@ -616,6 +620,38 @@ __global__ __launch_bounds__(1024) void march3d_rows_fwd(March3DArgs a, int F, i
     const int wf = phase == 0 ? 0 : 1;
     float2* row = state(a, item, wf) + (size_t)x * a.nky;
     float2 v[P];
+    if (phase == 3) {
+        const float tx = a.taper_x[x];
+        if (a.first) {
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int y = t + 64 * q;
+                v[q] = (y < a.nmy && x * a.nmy + y == a.igx) ? a.spec_s[fl] : make_float2(0.f, 0.f);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < P; ++q) v[q] = row[t + 64 * q];
+            wfft<NY, +1>(v, buf, a.twy, t);
+            float2* sm = a.smig + ((size_t)item * a.nz + a.iz_sel) * a.nm + (size_t)x * a.nmy;
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int y = t + 64 * q;
+                if (y < a.nmy) sm[y] = v[q];
+            }
+        }
+        if (!a.ds) return;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int y = t + 64 * q;
+            v[q] = y < a.nmy ? cmulc(cscale(v[q], tx * a.taper_y[y]), ss_phase(a, wd, (size_t)x * a.nmy + y))
+                             : make_float2(0.f, 0.f);
+        }
+        wave_sync_lds();
+        wfft<NY, -1>(v, buf, a.twy, t);
+#pragma unroll
+        for (int q = 0; q < P; ++q) row[t + 64 * q] = v[q];
+        return;
+    }
     if (phase == 1 && a.first) {
 #pragma unroll
         for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);

@@ -1,7 +1,8 @@
 """Angle-domain shot-profile imaging.
 
   compute_angles         wem.c:566-992 (compute_angles.c:12-118 without the file I/O): source
-                         incidence angles of one shot on the device, 2-D
+                         incidence angles of one shot on the device (2-D, and 3-D on
+                         power-of-two grids 128..1024)
   angle_bin / angle_unbin   the bilinear angle binning of ShotProfileWEM.jl:162-188 (migration:
                          a shot image spread into angle gathers) and its transpose
                          ShotProfileWEM.jl:246-275 (modelling: angle gathers collapsed to a shot
@@ -22,8 +23,8 @@ from .wem import DeviceArray, Plan, WemGeometry, _fp
 
 def compute_angles(vel, wav, geom: WemGeometry, sx: float, sy: float = 0.0, verbose: bool = False):
     """Source incidence angles (degrees) angx, angy [nm, nz] of the shot at (sx, sy): the
-    reference's compute_angles with its own sampling (ntfft = nt, nkx = nmx; padt, padx, pspi and
-    nref of `geom` are not used).  2-D only; angy is zero there."""
+    reference's compute_angles with its own sampling (ntfft = nt, nkx = nmx, nky = nmy; padt,
+    padx, pspi and nref of `geom` are not used).  angy is zero in 2-D."""
     g = geom
     nm = g.nmx * g.nmy
     angx = np.zeros((nm, g.nz), np.float32)
