@@ -81,6 +81,11 @@ def lib():
             getattr(L, pre + "wem").argtypes = [rp, rp, f32p, f32p, gp, ctypes.c_float, ctypes.c_float, ctypes.c_int]
             getattr(L, pre + "shots").argtypes = [rp, rp, f32p, f32p, gp, f32p, f32p, ctypes.c_int, ctypes.c_int]
             getattr(L, pre + "angles").argtypes = [rp, rp, f32p, f32p, gp, ctypes.c_float, ctypes.c_float]
+            getattr(L, pre + "ewem").argtypes = [rp, rp, rp, rp, f32p, f32p, f32p, gp, ctypes.c_float, ctypes.c_float,
+                                                 ctypes.c_int]
+            st = ctypes.c_double if pre == "orc64_" else ctypes.c_float
+            getattr(L, pre + "esep_k").argtypes = [st, st, st, st, ctypes.c_int, ctypes.c_int, rp, rp, rp, rp]
+            getattr(L, pre + "esep_k").restype = None
         _lib = L
     return _lib
 
@@ -267,3 +272,38 @@ def angles(p: WemParams, vel: np.ndarray, wav: np.ndarray, sx: float, sy: float 
     _check(fn(_ptr(ax, ct), _ptr(ay, ct), _ptr(wav, ctypes.c_float), _ptr(vel, ctypes.c_float), ctypes.byref(g),
               sx, sy), "angles")
     return ax, ay
+
+
+def ewem(p: WemParams, vp: np.ndarray, vs: np.ndarray, wav: np.ndarray, sx: float, sy: float = 0.0, *, ux=None,
+         uz=None, mpp=None, mps2=None, adj: bool = True, f64: bool = True):
+    """ewem (ewem.c:3-366), 2-D.  adj: components ux, uz [nm, nt] -> images (mpp, mps2) [nm, nz]
+    (mps1 of the reference is always zero); else (mpp, mps2) -> (ux, uz)."""
+    rt, ct = (np.float64, ctypes.c_double) if f64 else (np.float32, ctypes.c_float)
+    fn = lib().orc64_ewem if f64 else lib().orc32_ewem
+    vp = np.ascontiguousarray(vp, dtype=np.float32)
+    vs = np.ascontiguousarray(vs, dtype=np.float32)
+    wav = np.ascontiguousarray(wav, dtype=np.float32)
+    shp_d, shp_m = (p.nm, p.nt), (p.nm, p.nz)
+    if adj:
+        ux = np.ascontiguousarray(ux, dtype=rt)
+        uz = np.ascontiguousarray(uz, dtype=rt)
+        mpp, mps2 = np.zeros(shp_m, rt), np.zeros(shp_m, rt)
+    else:
+        mpp = np.ascontiguousarray(mpp, dtype=rt)
+        mps2 = np.ascontiguousarray(mps2, dtype=rt)
+        ux, uz = np.zeros(shp_d, rt), np.zeros(shp_d, rt)
+    g = p.geom()
+    _check(fn(_ptr(ux, ct), _ptr(uz, ct), _ptr(mpp, ct), _ptr(mps2, ct), _ptr(wav, ctypes.c_float),
+              _ptr(vp, ctypes.c_float), _ptr(vs, ctypes.c_float), ctypes.byref(g), sx, sy, int(adj)), "ewem")
+    return (mpp, mps2) if adj else (ux, uz)
+
+
+def esep_k(kx: float, w: float, vp: float, vs: float, sep: bool, adj: bool, a1: complex, a2: complex):
+    """elastic_separate_3d's 2 x 2 map at one wavenumber (ewem.c:600-630, 680-710), FP64."""
+    i1 = np.array([a1.real, a1.imag])
+    i2 = np.array([a2.real, a2.imag])
+    o1, o2 = np.zeros(2), np.zeros(2)
+    P = ctypes.POINTER(ctypes.c_double)
+    lib().orc64_esep_k(kx, w, vp, vs, int(sep), int(adj), i1.ctypes.data_as(P), i2.ctypes.data_as(P),
+                       o1.ctypes.data_as(P), o2.ctypes.data_as(P))
+    return complex(o1[0], o1[1]), complex(o2[0], o2[1])

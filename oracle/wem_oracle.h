@@ -100,6 +100,17 @@ int orc32_shots(float* d, float* m, const float* wav, const float* vel, const or
  * shot; the geometry's padt / padx / pspi are ignored (ntfft = nt, nkx = nmx, split-step). */
 int orc64_angles(double* angx, double* angy, const float* wav, const float* vel, const orc_geom* g, float sx, float sy);
 int orc32_angles(float* angx, float* angy, const float* wav, const float* vel, const orc_geom* g, float sx, float sy);
+/* ewem (ewem.c:3-366), 2-D: adj: ux, uz [nm][nt] -> mpp, mps2 [nm][nz]; else the reverse. */
+int orc64_ewem(double* ux, double* uz, double* mpp, double* mps2, const float* wav, const float* vp, const float* vs,
+               const orc_geom* g, float sx, float sy, int adj);
+int orc32_ewem(float* ux, float* uz, float* mpp, float* mps2, const float* wav, const float* vp, const float* vs,
+               const orc_geom* g, float sx, float sy, int adj);
+/* elastic_separate_3d's matrix at one wavenumber (ewem.c:600-630, 680-710): complex in / out as
+ * (re, im) pairs. */
+void orc64_esep_k(double kx, double w, double vp, double vs, int sep, int adj, const double* in1, const double* in2,
+                  double* o1, double* o2);
+void orc32_esep_k(float kx, float w, float vp, float vs, int sep, int adj, const float* in1, const float* in2,
+                  float* o1, float* o2);
 
 #ifdef __cplusplus
 }

@@ -785,4 +785,221 @@ int SFX(angles)(RT* angx, RT* angy, const float* wav, const float* vel, const or
     return ok ? ORC_OK : ORC_ENOMEM;
 }
 
+
+/* ------------------------------------------------------------------------- ewem() (elastic) */
+
+/* elastic_separate_3d (ewem.c:553-741) at one wavenumber: sep maps (ux, uz) to the potentials
+ * (up, us2), !sep maps (up, us2) back; adj selects the transposed matrices. */
+void SFX(esep_k)(RT kx, RT w, RT vp, RT vs, int sep, int adj, const RT* in1, const RT* in2, RT* o1, RT* o2)
+{
+    CT a1 = in1[0] + in1[1] * (RT _Complex)_Complex_I, a2 = in2[0] + in2[1] * (RT _Complex)_Complex_I, r1, r2;
+    RT sp = w * w / (vp * vp) - kx * kx, ss = w * w / (vs * vs) - kx * kx;
+    RT kzp = sp > 0 ? MSQRT(sp) : 0, kzs = ss > 0 ? MSQRT(ss) : 0;
+    RT np = MSQRT(kx * kx + kzp * kzp), ns = MSQRT(kx * kx + kzs * kzs);
+    RT kxp = kx / np, kxs = kx / ns;
+    kzp = kzp / np;
+    kzs = kzs / ns;
+    RT norm = kxp * kxs + kzp * kzs;
+    if (norm >= (RT)0.1 && w >= (RT)20) {
+        if (sep) {
+            if (!adj) { r1 = (kxs * a1 + kzs * a2) / norm; r2 = (-kzp * a1 + kxp * a2) / norm; }
+            else { r1 = kxp * a1 + kzp * a2; r2 = -kzs * a1 + kxs * a2; }
+        } else {
+            if (!adj) { r1 = kxp * a1 - kzs * a2; r2 = kzp * a1 + kxs * a2; }
+            else { r1 = (kxs * a1 - kzp * a2) / norm; r2 = (kzs * a1 + kxp * a2) / norm; }
+        }
+    } else {  /* up = uz, us2 = ux (sep); ux = us2, uz = up (!sep) */
+        r1 = a2;
+        r2 = a1;
+    }
+    o1[0] = MCREAL(r1); o1[1] = MCREAL(-r1 * (RT _Complex)_Complex_I);
+    o2[0] = MCREAL(r2); o2[1] = MCREAL(-r2 * (RT _Complex)_Complex_I);
+}
+
+/* Separation / combination of a wavefield pair on the x grid (2-D): transform both (unitary),
+ * apply esep_k per kx (kx = dkx ikx for ikx < nkx/2, else -(dkx nkx - dkx ikx), ewem.c:593), and
+ * transform back; entries at x >= nmx are not written. */
+static void SFX(esep)(const SFX(ctx)* c, CT* i1, CT* i2, CT* o1, CT* o2, RT w, RT vp, RT vs, int sep, int adj,
+                      SFX(work)* wk, CT* b1, CT* b2)
+{
+    int nmx = c->g->nmx, nkx = c->s.nkx;
+#if ORC_IS_F64
+    RT rnk = 1.0 / sqrt((double)nkx), dkx = 2.0 * M_PI / ((double)nkx * c->g->dmx);
+#else
+    RT rnk = 1.0f / sqrtf((float)nkx), dkx = c->s.dkx;
+#endif
+    for (int ix = 0; ix < nkx; ix++) {
+        b1[ix] = ix < nmx ? i1[ix] : 0;
+        b2[ix] = ix < nmx ? i2[ix] : 0;
+    }
+    SFX(fft1)(b1, nkx, c->twx, -1, wk->col);
+    SFX(fft1)(b2, nkx, c->twx, -1, wk->col);
+    for (int ik = 0; ik < nkx; ik++) {
+        RT kx = ik < nkx / 2.0 ? dkx * ik : -(dkx * nkx - dkx * ik);
+        CT a1 = b1[ik] * rnk, a2 = b2[ik] * rnk;
+        RT r1[2], r2[2];
+        SFX(esep_k)(kx, w, vp, vs, sep, adj, (RT*)&a1, (RT*)&a2, r1, r2);
+        b1[ik] = r1[0] + r1[1] * (RT _Complex)_Complex_I;
+        b2[ik] = r2[0] + r2[1] * (RT _Complex)_Complex_I;
+    }
+    SFX(fft1)(b1, nkx, c->twx, +1, wk->col);
+    SFX(fft1)(b2, nkx, c->twx, +1, wk->col);
+    for (int ix = 0; ix < nmx; ix++) {
+        o1[ix] = b1[ix] * rnk;
+        o2[ix] = b2[ix] * rnk;
+    }
+}
+
+/* The sampling of ewem (ewem.c:46-57): ntfft = 2 trunc(padt nt / 2), ifmax = trunc(fmax dt ntfft)
+ * + 1, ifmin = trunc(fmin dt ntfft) (no integer cast of fmax / fmin as in wem.c). */
+static void SFX(ewem_sampling)(const orc_geom* g, orc_sampling* s)
+{
+    float h = (float)g->padt * (float)g->nt / 2.f;
+    s->ntfft = (int)(2.f * truncf(h));
+    s->nw = s->ntfft / 2 + 1;
+    s->fop_n = (s->nw - 1) * 2;
+    s->dw = (float)(2 * 3.141592653589793 / ((double)(float)s->ntfft) / (double)g->dt);
+    float t = g->fmax * g->dt * (float)s->ntfft;
+    s->ifmax = t + 1 < (float)s->nw ? (int)trunc(t) + 1 : s->nw;
+    t = g->fmin * g->dt * (float)s->ntfft;
+    s->ifmin = t + 1 < (float)s->ifmax ? (int)trunc(t) : 0;
+}
+
+/* ewem (ewem.c:3-253) and elastic_extrap1f (ewem.c:255-366), 2-D: adj maps the components
+ * ux, uz [nm][nt] to the images mpp, mps2 [nm][nz] (mps1 stays zero: the reference never forms
+ * the first S potential); !adj maps mpp, mps2 to ux, uz (uy stays zero).  The source (vp) is
+ * normalised by max|wav| / sqrt(ntfft) (ewem.c:146); the receiver spectra carry no w^2. */
+int SFX(ewem)(RT* ux, RT* uz, RT* mpp, RT* mps2, const float* wav, const float* vp, const float* vs,
+              const orc_geom* g0, float sx, float sy, int adj)
+{
+    orc_geom ga = *g0;
+    ga.pspi = 0;
+    const orc_geom* g = &ga;
+    if (g->nmy != 1) return ORC_EINVAL;
+    SFX(ctx) cp, cs;
+    int rc = SFX(ctx_init)(&cp, g, vp);
+    if (rc) return rc;
+    rc = SFX(ctx_init)(&cs, g, vs);
+    if (rc) { SFX(ctx_free)(&cp); return rc; }
+    SFX(ewem_sampling)(g, &cp.s);
+    SFX(ewem_sampling)(g, &cs.s);
+    const orc_sampling* s = &cp.s;
+    const int nmx = g->nmx, nz = g->nz, nt = g->nt, nw = s->nw, nkx = s->nkx;
+    const int ifmin = s->ifmin, ifmax = s->ifmax;
+    const int igx = (int)truncf((sx - g->omx) / g->dmx);
+    if (igx < 0 || igx >= nmx || s->fop_n < nt) { SFX(ctx_free)(&cp); SFX(ctx_free)(&cs); return ORC_EINVAL; }
+    if (adj) { memset(mpp, 0, sizeof(RT) * (size_t)nmx * nz); memset(mps2, 0, sizeof(RT) * (size_t)nmx * nz); }
+    else { memset(ux, 0, sizeof(RT) * (size_t)nmx * nt); memset(uz, 0, sizeof(RT) * (size_t)nmx * nt); }
+#if ORC_IS_F64
+    const RT rnt = 1.0 / sqrt((double)s->ntfft);
+#else
+    const RT rnt = 1.0f / sqrtf((float)s->ntfft);
+#endif
+    RT maxs = 0;
+    for (int it = 0; it < nt; it++) {
+        RT v = MFABS((RT)wav[it]) * rnt;
+        if (v > maxs) maxs = v;
+    }
+    const int nf = ifmax > ifmin ? ifmax - ifmin : 0;
+    CT* ws = (CT*)malloc(sizeof(CT) * nw);
+    CT* dw = (CT*)malloc(sizeof(CT) * nw);
+    RT* tr = (RT*)malloc(sizeof(RT) * nt);
+    CT* gx = (CT*)calloc((size_t)nmx * (nf ? nf : 1), sizeof(CT));
+    CT* gz = (CT*)calloc((size_t)nmx * (nf ? nf : 1), sizeof(CT));
+    CT* us = (CT*)malloc(sizeof(CT) * nmx);
+    CT* rx = (CT*)malloc(sizeof(CT) * nmx);
+    CT* rz = (CT*)malloc(sizeof(CT) * nmx);
+    CT* up = (CT*)malloc(sizeof(CT) * nmx);
+    CT* u2 = (CT*)malloc(sizeof(CT) * nmx);
+    CT* smig = (CT*)malloc(sizeof(CT) * (size_t)nmx * nz);
+    CT* b1 = (CT*)malloc(sizeof(CT) * nkx);
+    CT* b2 = (CT*)malloc(sizeof(CT) * nkx);
+    RT* vp0 = (RT*)malloc(sizeof(RT) * nz);
+    RT* vs0 = (RT*)malloc(sizeof(RT) * nz);
+    SFX(work) wk;
+    memset(&wk, 0, sizeof(wk));
+    int ok = ws && dw && tr && gx && gz && us && rx && rz && up && u2 && smig && b1 && b2 && vp0 && vs0 &&
+             SFX(work_init)(&wk, &cp) == ORC_OK;
+    if (ok && nf > 0) {
+        for (int iz = 0; iz < nz; iz++) { vp0[iz] = (RT)1 / cp.po[iz]; vs0[iz] = (RT)1 / cs.po[iz]; }
+        for (int it = 0; it < nt; it++) tr[it] = wav[it];
+        SFX(fop)((RT*)ws, tr, nw, nt, 1);
+        if (adj)
+            for (int ix = 0; ix < nmx; ix++) {
+                for (int it = 0; it < nt; it++) tr[it] = ux[(size_t)ix * nt + it];
+                SFX(fop)((RT*)dw, tr, nw, nt, 1);
+                for (int iw = ifmin; iw < ifmax; iw++) gx[(size_t)ix * nf + iw - ifmin] = dw[iw];
+                for (int it = 0; it < nt; it++) tr[it] = uz[(size_t)ix * nt + it];
+                SFX(fop)((RT*)dw, tr, nw, nt, 1);
+                for (int iw = ifmin; iw < ifmax; iw++) gz[(size_t)ix * nf + iw - ifmin] = dw[iw];
+            }
+        for (int iw = ifmin; iw < ifmax; iw++) {
+            const RT w = SFX(omega)(&cp, iw), factor = iw == 0 ? 1 : 2;
+            for (int x = 0; x < nmx; x++) us[x] = 0;
+            us[igx] = ws[iw] * rnt;
+            for (int iz = 0; iz < nz; iz++) {  /* source, ewem.c:302-310 */
+                float z = g->oz + g->dz * iz;
+                if (z >= g->sz) {
+                    SFX(ssop)(&cp, us, w, iz, (RT)(-g->dz), 1, &wk);
+                    for (int x = 0; x < nmx; x++) smig[(size_t)x * nz + iz] = us[x] / maxs;
+                } else
+                    for (int x = 0; x < nmx; x++) smig[(size_t)x * nz + iz] = 0;
+            }
+            if (adj) {  /* ewem.c:311-327 */
+                for (int x = 0; x < nmx; x++) {
+                    rx[x] = gx[(size_t)x * nf + iw - ifmin] * rnt;
+                    rz[x] = gz[(size_t)x * nf + iw - ifmin] * rnt;
+                }
+                for (int iz = 0; iz < nz; iz++) {
+                    float z = g->oz + g->dz * iz;
+                    if (!(z >= g->gz)) continue;
+                    SFX(esep)(&cp, rx, rz, up, u2, w, vp0[iz], vs0[iz], 1, 1, &wk, b1, b2);
+                    SFX(ssop)(&cp, up, w, iz, (RT)g->dz, 1, &wk);
+                    SFX(ssop)(&cs, u2, w, iz, (RT)g->dz, 1, &wk);
+                    SFX(esep)(&cp, up, u2, rx, rz, w, vp0[iz], vs0[iz], 0, 1, &wk, b1, b2);
+                    for (int x = 0; x < nmx; x++) {
+                        CT sm = MCONJ(smig[(size_t)x * nz + iz]);
+                        mpp[(size_t)x * nz + iz] += factor * MCREAL(sm * up[x]);
+                        mps2[(size_t)x * nz + iz] += factor * MCREAL(sm * u2[x]);
+                    }
+                }
+            } else {  /* ewem.c:328-346 */
+                for (int x = 0; x < nmx; x++) rx[x] = rz[x] = 0;
+                for (int iz = nz - 1; iz >= 0; iz--) {
+                    float z = g->oz + g->dz * iz;
+                    if (!(z >= g->gz)) continue;
+                    SFX(esep)(&cp, rx, rz, up, u2, w, vp0[iz], vs0[iz], 1, 0, &wk, b1, b2);
+                    for (int x = 0; x < nmx; x++) {
+                        up[x] += smig[(size_t)x * nz + iz] * mpp[(size_t)x * nz + iz];
+                        u2[x] += smig[(size_t)x * nz + iz] * mps2[(size_t)x * nz + iz];
+                    }
+                    SFX(ssop)(&cp, up, w, iz, (RT)(-g->dz), 0, &wk);
+                    SFX(ssop)(&cs, u2, w, iz, (RT)(-g->dz), 0, &wk);
+                    SFX(esep)(&cp, up, u2, rx, rz, w, vp0[iz], vs0[iz], 0, 0, &wk, b1, b2);
+                }
+                for (int x = 0; x < nmx; x++) {
+                    gx[(size_t)x * nf + iw - ifmin] = rx[x] * rnt;
+                    gz[(size_t)x * nf + iw - ifmin] = rz[x] * rnt;
+                }
+            }
+        }
+        if (!adj)  /* ewem.c:219-241: c2r of the band, no w^2 */
+            for (int ix = 0; ix < nmx; ix++)
+                for (int k = 0; k < 2; k++) {
+                    CT* src = k == 0 ? gx : gz;
+                    RT* out = k == 0 ? ux : uz;
+                    for (int iw = 0; iw < nw; iw++) dw[iw] = 0;
+                    for (int iw = ifmin; iw < ifmax; iw++) dw[iw] = src[(size_t)ix * nf + iw - ifmin];
+                    SFX(fop)((RT*)dw, tr, nw, nt, 0);
+                    for (int it = 0; it < nt; it++) out[(size_t)ix * nt + it] = tr[it];
+                }
+    }
+    SFX(work_free)(&wk);
+    free(ws); free(dw); free(tr); free(gx); free(gz); free(us); free(rx); free(rz); free(up); free(u2);
+    free(smig); free(b1); free(b2); free(vp0); free(vs0);
+    SFX(ctx_free)(&cp);
+    SFX(ctx_free)(&cs);
+    return ok ? ORC_OK : ORC_ENOMEM;
+}
+
 #undef CT
