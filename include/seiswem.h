@@ -137,6 +137,22 @@ int swem_compute_angles(float* angx, float* angy, const float* wav,
                         int nz, float oz, float dz, float sz,
                         const float* vel, float fmin, float fmax, int verbose);
 
+/* Replaces ewem() (ewem.c:3-15), elastic shot-profile migration (adj=1: components ux, uy, uz
+ * [nm][nt] -> images mpp, mps1, mps2 [nm][nz]) and modelling (adj=0: the reverse), with the P
+ * source, the P / S separation of the receiver components at every depth and split-step
+ * propagation with vp and vs (ewem.c:255-741).  2-D (nmy = 1); as in the reference, uy is not
+ * used (migration) and returned zero (modelling), and mps1 is returned zero and not used.
+ * Returns SWEM_EUNSUPPORTED when ewem.c's frequency band (its fmax / fmin truncation,
+ * ewem.c:46-57) differs from wem.c's for the geometry. */
+int swem_ewem(float* ux, float* uy, float* uz, float* mpp, float* mps1, float* mps2, const float* wav,
+              int nt, float ot, float dt,
+              int nmx, float omx, float dmx,
+              int nmy, float omy, float dmy,
+              float sx, float sy,
+              int nz, float oz, float dz, float gz, float sz,
+              const float* vp, const float* vs, float fmin, float fmax,
+              int padt, int padx, int adj, int verbose);
+
 /* wem() with explicit options (pspi_mode, precision, device...). */
 int swem_wem_ex(float* d, float* m, const float* wav, const swem_geometry* g,
                 float sx, float sy, const float* vel, int adj, const swem_options* opt);

@@ -1358,6 +1358,173 @@ int swem_compute_angles(float* angx, float* angy, const float* wav, int nt, floa
     return rc;
 }
 
+// ------------------------------------------------------------------------ elastic WEM (2-D)
+// The sampling of ewem (ewem.c:46-57), which truncates fmax dt ntfft where wem.c casts fmax.
+static void ewem_sampling(const swem_geometry& g, int& ntfft, int& ifmin, int& ifmax)
+{
+    ntfft = (int)(2.f * std::trunc((float)g.padt * (float)g.nt / 2.f));
+    const int nw = ntfft / 2 + 1;
+    float t = g.fmax * g.dt * (float)ntfft;
+    ifmax = t + 1 < (float)nw ? (int)std::trunc(t) + 1 : nw;
+    t = g.fmin * g.dt * (float)ntfft;
+    ifmin = t + 1 < (float)ifmax ? (int)std::trunc(t) : 0;
+}
+
+static int ewem_impl(swem_plan* pp, swem_plan* ps, int igx, float* ux, float* uy, float* uz, float* mpp, float* mps1,
+                     float* mps2, const float* wav, int adj)
+{
+    hipStream_t st = nullptr;
+    const int nmx = pp->g.nmx, nz = pp->g.nz, nt = pp->g.nt, nf = pp->nf;
+    const size_t nmz = (size_t)nmx * nz, nmt = (size_t)nmx * nt;
+    if (adj) {
+        std::fill(mpp, mpp + nmz, 0.f);
+        std::fill(mps1, mps1 + nmz, 0.f);
+        std::fill(mps2, mps2 + nmz, 0.f);
+    } else {
+        std::fill(ux, ux + nmt, 0.f);
+        std::fill(uy, uy + nmt, 0.f);
+        std::fill(uz, uz + nmt, 0.f);
+    }
+    if (nf < 1) return SWEM_OK;
+    if (pp->nchunks != ps->nchunks || pp->nfc != ps->nfc)
+        return set_error(SWEM_EINVAL, "ewem: the P and S plans chunk the band differently");
+    // bases without w^2 (ewem.c:116-140, 219-241): R2C / sqrt(ntfft) and the c2r of the band
+    const int n = pp->s.fop_n;
+    const double rnt = 1.0 / std::sqrt((double)pp->s.ntfft);
+    std::vector<float> badj((size_t)nt * 2 * nf), bfwd((size_t)2 * nf * nt);
+    for (int f = 0; f < nf; f++) {
+        const int iw = pp->s.ifmin + f;
+        const double cf = (iw == 0 || 2 * iw == n) ? 1.0 : 2.0;
+        for (int t = 0; t < nt; t++) {
+            const long long e = ((long long)iw * t) % n;
+            const double ang = 2.0 * kPi * (double)e / (double)n, c = std::cos(ang), sn = std::sin(ang);
+            badj[(size_t)t * 2 * nf + 2 * f] = (float)(rnt * c);
+            badj[(size_t)t * 2 * nf + 2 * f + 1] = (float)(-rnt * sn);
+            bfwd[(size_t)(2 * f) * nt + t] = (float)(cf * rnt * c);
+            bfwd[(size_t)(2 * f + 1) * nt + t] = (float)(-cf * rnt * sn);
+        }
+    }
+    // layer velocities of the separation, in float as ewem.c:64-81 forms them
+    std::vector<float> vp0(nz), vs0(nz);
+    for (int k = 0; k < 2; k++) {
+        const std::vector<float>& v = k == 0 ? pp->vel : ps->vel;
+        for (int iz = 0; iz < nz; iz++) {
+            float sum = 0.f;
+            for (int ix = 0; ix < nmx; ix++) sum += v[(size_t)ix * nz + iz];
+            const float po = 1.f / (sum / (float)nmx);
+            (k == 0 ? vp0 : vs0)[iz] = (float)(1.0 / (double)po);
+        }
+    }
+    float maxs = 0.f;
+    const float rntf = std::sqrt((float)pp->s.ntfft);
+    for (int t = 0; t < nt; t++) maxs = std::max(maxs, std::fabs(wav[t]) / rntf);
+    if (!(maxs > 0.f)) return set_error(SWEM_EINVAL, "ewem: the wavelet is zero");
+    DevBuf<float> dbadj, dbfwd, dvp0, dvs0, drx, drz, ppart, spart, dmpp, dmps, mppT, mpsT;
+    DevBuf<float2> spx, spz, smig, dgx, dgz;
+    DevBuf<int> dig;
+    SWEM_TRY(dvp0.upload(vp0.data(), nz));
+    SWEM_TRY(dvs0.upload(vs0.data(), nz));
+    SWEM_TRY(dig.upload(&igx, 1));
+    const size_t nfc = (size_t)pp->nfc;
+    if (adj) {
+        SWEM_TRY(dbadj.upload(badj.data(), badj.size()));
+        SWEM_TRY(drx.upload(ux, nmt));
+        SWEM_TRY(drz.upload(uz, nmt));
+        SWEM_TRY(spx.alloc((size_t)nmx * nf));
+        SWEM_TRY(spz.alloc((size_t)nmx * nf));
+        SWEM_TRY(launch_sgemm(nmx, 2 * nf, nt, drx.p, nt, dbadj.p, 2 * nf, reinterpret_cast<float*>(spx.p), 2 * nf, st));
+        SWEM_TRY(launch_sgemm(nmx, 2 * nf, nt, drz.p, nt, dbadj.p, 2 * nf, reinterpret_cast<float*>(spz.p), 2 * nf, st));
+        SWEM_TRY(ppart.alloc(nfc * nmz));
+        SWEM_TRY(spart.alloc(nfc * nmz));
+        SWEM_TRY(dmpp.alloc(nmz));
+        SWEM_TRY(dmps.alloc(nmz));
+    } else {
+        SWEM_TRY(dbfwd.upload(bfwd.data(), bfwd.size()));
+        std::vector<float> t1(nmz), t2(nmz);
+        for (int ix = 0; ix < nmx; ix++)
+            for (int iz = 0; iz < nz; iz++) {
+                t1[(size_t)iz * nmx + ix] = mpp[(size_t)ix * nz + iz];
+                t2[(size_t)iz * nmx + ix] = mps2[(size_t)ix * nz + iz];
+            }
+        SWEM_TRY(mppT.upload(t1.data(), nmz));
+        SWEM_TRY(mpsT.upload(t2.data(), nmz));
+        SWEM_TRY(smig.alloc(nfc * nmz));
+        SWEM_TRY(dgx.alloc((size_t)nmx * nf));
+        SWEM_TRY(dgz.alloc((size_t)nmx * nf));
+        SWEM_TRY(drx.alloc(nmt));
+        SWEM_TRY(drz.alloc(nmt));
+    }
+    for (int c = 0; c < pp->nchunks; c++) {
+        SWEM_TRY(ensure_chunk(pp, c, st));
+        SWEM_TRY(ensure_chunk(ps, c, st));
+        March2DArgs ap = base_args(pp, c, 1), as = base_args(ps, c, 1);
+        ap.nshot = as.nshot = 1;
+        ap.igx = as.igx = dig.p;
+        EwemArgs e{};
+        e.nf = nf;
+        e.dwf = pp->s.dw;
+        e.dkx = pp->s.dkx;
+        e.inv_maxs = 1.f / maxs;
+        e.vp0 = dvp0.p;
+        e.vs0 = dvs0.p;
+        e.spx = spx.p;
+        e.spz = spz.p;
+        e.ppart = ppart.p;
+        e.spart = spart.p;
+        e.smig = smig.p;
+        e.mppT = mppT.p;
+        e.mpsT = mpsT.p;
+        e.dgx = dgx.p;
+        e.dgz = dgz.p;
+        SWEM_TRY(launch_ewem(ap, as, e, adj, st));
+        if (adj) {
+            SWEM_TRY(launch_reduce_image(ppart.p, 1, ap.ngroups, nz, nmx, dmpp.p, c > 0, 0, st));
+            SWEM_TRY(launch_reduce_image(spart.p, 1, ap.ngroups, nz, nmx, dmps.p, c > 0, 0, st));
+        }
+    }
+    if (adj) {
+        SWEM_HIP(hipMemcpyAsync(mpp, dmpp.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
+        SWEM_HIP(hipMemcpyAsync(mps2, dmps.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
+    } else {
+        SWEM_TRY(launch_sgemm(nmx, nt, 2 * nf, reinterpret_cast<const float*>(dgx.p), 2 * nf, dbfwd.p, nt, drx.p, nt, st));
+        SWEM_TRY(launch_sgemm(nmx, nt, 2 * nf, reinterpret_cast<const float*>(dgz.p), 2 * nf, dbfwd.p, nt, drz.p, nt, st));
+        SWEM_HIP(hipMemcpyAsync(ux, drx.p, nmt * sizeof(float), hipMemcpyDeviceToHost, st));
+        SWEM_HIP(hipMemcpyAsync(uz, drz.p, nmt * sizeof(float), hipMemcpyDeviceToHost, st));
+    }
+    SWEM_HIP(hipStreamSynchronize(st));
+    return SWEM_OK;
+}
+
+int swem_ewem(float* ux, float* uy, float* uz, float* mpp, float* mps1, float* mps2, const float* wav, int nt,
+              float ot, float dt, int nmx, float omx, float dmx, int nmy, float omy, float dmy, float sx, float sy,
+              int nz, float oz, float dz, float gz, float sz, const float* vp, const float* vs, float fmin, float fmax,
+              int padt, int padx, int adj, int verbose)
+{
+    if (!ux || !uy || !uz || !mpp || !mps1 || !mps2 || !wav || !vp || !vs)
+        return set_error(SWEM_EINVAL, "ewem: null argument");
+    if (nmy > 1) return set_error(SWEM_EUNSUPPORTED, "ewem: 3-D (nmy > 1) is not available in this build");
+    swem_geometry g;
+    geom_from_args(&g, nt, ot, dt, nmx, omx, dmx, nmy, omy, dmy, nz, oz, dz, gz, sz, 1, fmin, fmax, padt, padx, 0);
+    int ntfft, ifmin, ifmax;
+    ewem_sampling(g, ntfft, ifmin, ifmax);
+    const int igx = (int)std::trunc((sx - omx) / dmx);
+    if (igx < 0 || igx >= nmx) return set_error(SWEM_EINVAL, "ewem: source x %g outside the grid", (double)sx);
+    swem_options o;
+    swem_options_init(&o);
+    o.verbose = verbose;
+    swem_plan *pp = nullptr, *ps = nullptr;
+    SWEM_TRY(swem_plan_create(&g, vp, wav, &o, &pp));
+    int rc = swem_plan_create(&g, vs, wav, &o, &ps);
+    if (!rc && (pp->s.ntfft != ntfft || pp->s.ifmin != ifmin || pp->s.ifmax != ifmax))
+        rc = set_error(SWEM_EUNSUPPORTED, "ewem: this geometry's band differs between ewem.c (ntfft %d, [%d, %d)) and "
+                       "wem.c (ntfft %d, [%d, %d)), whose sampling the plans use", ntfft, ifmin, ifmax, pp->s.ntfft,
+                       pp->s.ifmin, pp->s.ifmax);
+    if (!rc) rc = ewem_impl(pp, ps, igx, ux, uy, uz, mpp, mps1, mps2, wav, adj);
+    if (ps) swem_plan_destroy(ps);
+    swem_plan_destroy(pp);
+    return rc;
+}
+
 int swem_malloc(void** ptr, size_t bytes)
 {
     if (!ptr) return set_error(SWEM_EINVAL, "null pointer");

@@ -1350,4 +1350,224 @@ int launch_angles(int which, const March2DArgs& ax, const March2DArgs& az, const
     return set_error(SWEM_EUNSUPPORTED, "angles: transform length %d", n);
 }
 
+
+// ------------------------------------------------------------------------ elastic WEM (2-D)
+// ewem (ewem.c:3-366).  Per frequency: the P source (vp, -dz, adjoint order) normalised by the
+// largest source sample, and the receiver components (ux, uz).  Per depth the components are
+// separated into the P and second S potentials (elastic_separate_3d, ewem.c:553-741; the first
+// S potential is never formed), each propagated with its own velocity's split-step step, and
+// recombined; the P and S images correlate the potentials with the source.  The separation
+// matrix of wavenumber kx (kx = dkx ikx for ikx < nkx/2, else -(dkx nkx - dkx ikx)) uses the
+// depth's layer velocities vp0 = 1/po_p, vs0 = 1/po_s and is computed in place.
+__device__ __forceinline__ void esep_k(float kx, float w, float vp, float vs, bool sep, bool adj, float2 a1,
+                                       float2 a2, float2& o1, float2& o2)
+{
+    const float sp = w * w / (vp * vp) - kx * kx, ss = w * w / (vs * vs) - kx * kx;
+    float kzp = sp > 0.f ? sqrtf(sp) : 0.f, kzs = ss > 0.f ? sqrtf(ss) : 0.f;
+    const float np = sqrtf(kx * kx + kzp * kzp), ns = sqrtf(kx * kx + kzs * kzs);
+    const float kxp = kx / np, kxs = kx / ns;
+    kzp = kzp / np;
+    kzs = kzs / ns;
+    const float norm = kxp * kxs + kzp * kzs;
+    if (norm >= 0.1f && w >= 20.f) {
+        float m11, m12, m21, m22;
+        if (sep) {
+            if (!adj) m11 = kxs / norm, m12 = kzs / norm, m21 = -kzp / norm, m22 = kxp / norm;
+            else m11 = kxp, m12 = kzp, m21 = -kzs, m22 = kxs;
+        } else {
+            if (!adj) m11 = kxp, m12 = -kzs, m21 = kzp, m22 = kxs;
+            else m11 = kxs / norm, m12 = -kzp / norm, m21 = kzs / norm, m22 = kxp / norm;
+        }
+        o1 = cadd(cscale(a1, m11), cscale(a2, m12));
+        o2 = cadd(cscale(a1, m21), cscale(a2, m22));
+    } else {
+        o1 = a2;
+        o2 = a1;
+    }
+}
+
+// (i1, i2) -> (o1, o2) through the wavenumber domain; o1, o2 are zero at x >= nmx.
+template <int N, int P, class X>
+__device__ __forceinline__ void esep(const float2 (&i1)[P], const float2 (&i2)[P], float2 (&o1)[P], float2 (&o2)[P],
+                                     float2* lds, int& pp, const March2DArgs& a, const EwemArgs& e, int t, float w,
+                                     float vp, float vs, bool sep, bool adj)
+{
+    constexpr int T = N / P;
+    const int n = a.nkx;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        o1[q] = i1[q];
+        o2[q] = i2[q];
+    }
+    X::template fft<-1, false>(o1, lds, a, t, pp);
+    X::template fft<-1, false>(o2, lds, a, t, pp);
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int k = t + q * T;
+        const float kx = k < n ? ((float)k < (float)n / 2.f ? e.dkx * (float)k : -(e.dkx * (float)n - e.dkx * (float)k))
+                               : 0.f;
+        float2 r1, r2;
+        esep_k(kx, w, vp, vs, sep, adj, o1[q], o2[q], r1, r2);
+        o1[q] = k < n ? cscale(r1, a.inv_nk) : make_float2(0.f, 0.f);
+        o2[q] = k < n ? cscale(r2, a.inv_nk) : make_float2(0.f, 0.f);
+    }
+    X::template fft<+1, false>(o1, lds, a, t, pp);
+    X::template fft<+1, false>(o2, lds, a, t, pp);
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+        if (t + q * T >= a.nmx) o1[q] = o2[q] = make_float2(0.f, 0.f);
+}
+
+template <int N, int P, class X, bool ADJ>
+__global__ __launch_bounds__(N / P) __attribute__((amdgpu_waves_per_eu(2))) void ewem_march(March2DArgs ap,
+                                                                                             March2DArgs as, EwemArgs e)
+{
+    constexpr int T = N / P;
+    extern __shared__ float2 smem[];
+    int g, shot;
+    decode_block(ap.ngroups, ap.nshot, g, shot);
+    if (g < 0) return;
+    const int t = threadIdx.x;
+    const int fc = g;
+    const bool active = fc < ap.nfc;
+    const int fl = ap.f0 + fc;
+    const int iw = ap.ifmin + fl;
+    const float w = (float)iw * e.dwf;
+    StepCtx c;
+    c.t = t;
+    c.fc = __builtin_amdgcn_readfirstlane(active ? fc : 0);
+    int pp = 0;
+    float tap[P];
+    load_taper<N, P, kModeSSF>(tap, ap, t);
+    const int igx = ap.igx[0];
+    const float factor = iw == 0 ? 1.f : 2.f;
+    float2 us[P], rx[P], rz[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int x = t + q * T;
+        us[q] = (active && x == igx) ? ap.spec_s[fl] : make_float2(0.f, 0.f);
+        rx[q] = (ADJ && active && x < ap.nmx) ? e.spx[(size_t)x * e.nf + fl] : make_float2(0.f, 0.f);
+        rz[q] = (ADJ && active && x < ap.nmx) ? e.spz[(size_t)x * e.nf + fl] : make_float2(0.f, 0.f);
+    }
+    const size_t vol = (size_t)(active ? fc : 0) * ap.nz * ap.nmx;
+    if constexpr (ADJ) {
+        for (int iz = 0; iz < ap.nz; ++iz) {  // elastic_extrap1f, ewem.c:302-327
+            c.iz = iz;
+            const float z = ap.oz + ap.dz * (float)iz;
+            const bool ds = z >= ap.sz, dr = z >= ap.gz;
+            if (ds) ssf_adj_step<N, P, false, true, X>(us, smem, pp, ap, c, true, tap);
+            float2 up[P], u2[P];
+            if (dr) {
+                const float vp = e.vp0[iz], vs = e.vs0[iz];
+                esep<N, P, X>(rx, rz, up, u2, smem, pp, ap, e, t, w, vp, vs, true, true);
+                ssf_adj_step<N, P, false, true, X>(up, smem, pp, ap, c, false, tap);
+                ssf_adj_step<N, P, false, true, X>(u2, smem, pp, as, c, false, tap);
+                esep<N, P, X>(up, u2, rx, rz, smem, pp, ap, e, t, w, vp, vs, false, true);
+            }
+            if (active) {
+                const auto rp = buf_rsrc(e.ppart + vol + (size_t)iz * ap.nmx, (uint32_t)ap.nmx * 4u);
+                const auto rs = buf_rsrc(e.spart + vol + (size_t)iz * ap.nmx, (uint32_t)ap.nmx * 4u);
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const float2 sm = ds ? cscale(us[q], e.inv_maxs) : make_float2(0.f, 0.f);
+                    const uint32_t off = (uint32_t)(t + q * T) * 4u;
+                    buf_store1(rp, off, dr ? factor * (sm.x * up[q].x + sm.y * up[q].y) : 0.f);
+                    buf_store1(rs, off, dr ? factor * (sm.x * u2[q].x + sm.y * u2[q].y) : 0.f);
+                }
+            }
+        }
+    } else {
+        for (int iz = 0; iz < ap.nz; ++iz) {  // source, stored per depth (ewem.c:302-310)
+            c.iz = iz;
+            const float z = ap.oz + ap.dz * (float)iz;
+            const bool ds = z >= ap.sz;
+            if (ds) ssf_adj_step<N, P, false, true, X>(us, smem, pp, ap, c, true, tap);
+            if (active) {
+                const auto sr = buf_rsrc(e.smig + vol + (size_t)iz * ap.nmx, (uint32_t)ap.nmx * 8u);
+#pragma unroll
+                for (int q = 0; q < P; ++q)
+                    buf_store2(sr, (uint32_t)(t + q * T) * 8u, ds ? cscale(us[q], e.inv_maxs) : make_float2(0.f, 0.f));
+            }
+        }
+        for (int iz = ap.nz - 1; iz >= 0; --iz) {  // receiver upward (ewem.c:328-346)
+            c.iz = iz;
+            const float z = ap.oz + ap.dz * (float)iz;
+            if (!(z >= ap.gz)) continue;
+            const float vp = e.vp0[iz], vs = e.vs0[iz];
+            float2 up[P], u2[P];
+            esep<N, P, X>(rx, rz, up, u2, smem, pp, ap, e, t, w, vp, vs, true, false);
+            if (active) {
+                const auto sr = buf_rsrc(e.smig + vol + (size_t)iz * ap.nmx, (uint32_t)ap.nmx * 8u);
+                const auto mp = buf_rsrc(e.mppT + (size_t)iz * ap.nmx, (uint32_t)ap.nmx * 4u);
+                const auto ms = buf_rsrc(e.mpsT + (size_t)iz * ap.nmx, (uint32_t)ap.nmx * 4u);
+                float2 sv[P];
+                float mpv[P], msv[P];
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const uint32_t x = (uint32_t)(t + q * T);
+                    sv[q] = buf_load2(sr, x * 8u);
+                    mpv[q] = buf_load1(mp, x * 4u);
+                    msv[q] = buf_load1(ms, x * 4u);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    up[q] = cadd(up[q], cscale(sv[q], mpv[q]));
+                    u2[q] = cadd(u2[q], cscale(sv[q], msv[q]));
+                }
+            }
+            ssf_fwd_step<N, P, false, X>(up, smem, pp, ap, c, tap);
+            ssf_fwd_step<N, P, false, X>(u2, smem, pp, as, c, tap);
+            esep<N, P, X>(up, u2, rx, rz, smem, pp, ap, e, t, w, vp, vs, false, false);
+        }
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int x = t + q * T;
+                if (x < ap.nmx) {
+                    e.dgx[(size_t)x * e.nf + fl] = rx[q];
+                    e.dgz[(size_t)x * e.nf + fl] = rz[q];
+                }
+            }
+        }
+    }
+}
+
+template <int N, class X>
+static int launch_ewem_n(const March2DArgs& ap, const March2DArgs& as, const EwemArgs& e, int adj, hipStream_t st)
+{
+    constexpr int P = N >= 1024 ? 8 : N / 64, T = N / P;
+    const size_t lds = (size_t)LdsLen<N>::value * 2 * sizeof(float2);
+    auto k = adj ? ewem_march<N, P, X, true> : ewem_march<N, P, X, false>;
+    SWEM_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k, dim3(march2d_grid_blocks(ap.ngroups, ap.nshot)), dim3(T), lds, st, ap, as, e);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
+template <int N>
+static int launch_ewem_len(int blue, const March2DArgs& ap, const March2DArgs& as, const EwemArgs& e, int adj,
+                           hipStream_t st)
+{
+    constexpr int P = N >= 1024 ? 8 : N / 64;
+    return blue ? launch_ewem_n<N, XBlue<N, P>>(ap, as, e, adj, st) : launch_ewem_n<N, XPow2<N, P>>(ap, as, e, adj, st);
+}
+
+int launch_ewem(const March2DArgs& ap, const March2DArgs& as, const EwemArgs& e, int adj, hipStream_t st)
+{
+    const int n = ap.nkx, m = march2d_blue_len(n), len = m ? m : n;
+    if (!angles_supported(n) || as.nkx != n || (m && (!ap.chirp || !as.chirp)) || ap.nshot != 1 ||
+        ap.ngroups != ap.nfc || ap.nmx > n)
+        return set_error(SWEM_EINVAL, "ewem: inconsistent launch (nkx=%d nmx=%d)", n, ap.nmx);
+    switch (len) {
+    case 128: return launch_ewem_len<128>(m != 0, ap, as, e, adj, st);
+    case 256: return launch_ewem_len<256>(m != 0, ap, as, e, adj, st);
+    case 512: return launch_ewem_len<512>(m != 0, ap, as, e, adj, st);
+    case 1024: return launch_ewem_len<1024>(m != 0, ap, as, e, adj, st);
+    case 2048: return launch_ewem_len<2048>(m != 0, ap, as, e, adj, st);
+    case 4096: return launch_ewem_len<4096>(m != 0, ap, as, e, adj, st);
+    }
+    return set_error(SWEM_EUNSUPPORTED, "ewem: transform length %d", n);
+}
+
 }  // namespace swem

@@ -9,4 +9,5 @@ from .serve import Server  # noqa: F401
 from .files import ShotProfileWEMFiles  # noqa: F401
 from . import seis, solvers  # noqa: F401
 from .angles import ShotProfileWEMAngles, compute_angles  # noqa: F401
+from .elastic import ShotProfileEWEM, ewem  # noqa: F401
 from . import synth  # noqa: F401
