@@ -111,6 +111,11 @@ int swem_shot_main(int argc, char** argv);
  * of calling exit(1) when a required name is missing. */
 int swem_angles_main(int argc, char** argv);
 
+/* Replaces shotewem.so's `int main(int argc, char** argv)` (shotewem.c:11-223): keys ux, uy, uz,
+ * mpp, mps1, mps2, vp, vs, wav (required), adj, verbose, sx, sy, sz, gz, fmin, fmax, padt, padx;
+ * the header rewrites of shotewem.c:117-151 and 171-190.  shotewem.so exports it as `main`. */
+int swem_eshot_main(int argc, char** argv);
+
 /* Replaces wem() (wem.c:3-11) with host arrays: adj=1 migrates d [nm][nt] into m [nm][nz]
  * (overwritten), adj=0 models m into d (overwritten).  pspi selects PSPI (compat mode) or SSF. */
 int swem_wem(float* d, float* m, const float* wav,

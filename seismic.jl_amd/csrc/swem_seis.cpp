@@ -413,3 +413,114 @@ int swem_angles_main(int argc, char** argv)
     SWEM_TRY(swem_seis_write(angy_name.c_str(), angy.data(), ha.data(), &fh));
     return SWEM_OK;
 }
+
+// Drop-in for shotewem.so's main (shotewem.c:11-223), the library ShotProfileEWEM.jl calls: one
+// shot of elastic migration (ux, uy, uz -> mpp, mps1, mps2) or modelling, .seis files in and out,
+// with the reference's header rewrites (shotewem.c:117-151, 171-190).
+int swem_eshot_main(int argc, char** argv)
+{
+    bool adj = true, verbose = false;
+    std::string nm_[9];
+    const char* keys[9] = {"ux", "uy", "uz", "mpp", "mps1", "mps2", "vp", "vs", "wav"};
+    float sz = 0, gz = 0, sx = 0, sy = 0, fmin = 0, fmax = 80;
+    int padt = 1, padx = 1;
+    par_bool(argc, argv, "adj", adj);
+    par_bool(argc, argv, "verbose", verbose);
+    for (int k = 0; k < 9; k++)
+        if (!par_string(argc, argv, keys[k], nm_[k])) {
+            std::fprintf(stderr, "shotewem: one-shot elastic wave-equation migration (adj=y) or modelling (adj=n) of"
+                                 " .seis files; required: ux= uy= uz= mpp= mps1= mps2= vp= vs= wav=\n");
+            return set_error(SWEM_EINVAL, "shotewem: ux=, uy=, uz=, mpp=, mps1=, mps2=, vp=, vs= and wav= are required");
+        }
+    par_float(argc, argv, "sz", sz);
+    par_float(argc, argv, "gz", gz);
+    par_float(argc, argv, "sx", sx);
+    par_float(argc, argv, "sy", sy);
+    par_float(argc, argv, "fmin", fmin);
+    par_float(argc, argv, "fmax", fmax);
+    par_int(argc, argv, "padt", padt);
+    par_int(argc, argv, "padx", padx);
+
+    swem_seis_extent fh;
+    swem_seis_extent_init(&fh);
+    SWEM_TRY(swem_seis_read_header(nm_[8].c_str(), &fh));
+    const int nt = fh.n1;
+    const size_t nwtr = std::max<size_t>((size_t)fh.n2 * fh.n3 * fh.n4 * fh.n5, 1);
+    std::vector<float> wav((size_t)nt * nwtr);
+    std::vector<SeisHeader> hw(nwtr);
+    SWEM_TRY(swem_seis_read(nm_[8].c_str(), wav.data(), hw.data(), &fh));
+    const float ot = fh.o1, dt = fh.d1;
+    swem_seis_extent_init(&fh);
+    SWEM_TRY(swem_seis_read_header(nm_[6].c_str(), &fh));
+    const int nz = fh.n1;
+    const size_t ntraces = (size_t)fh.n2 * fh.n3 * fh.n4 * fh.n5;
+    if (ntraces < 2) return set_error(SWEM_EINVAL, "shotewem: velocity %s has %zu traces", nm_[6].c_str(), ntraces);
+    std::vector<float> vp((size_t)nz * ntraces), vs((size_t)nz * ntraces);
+    std::vector<SeisHeader> hv(ntraces), hvs(ntraces);
+    SWEM_TRY(swem_seis_read(nm_[6].c_str(), vp.data(), hv.data(), &fh));
+    SWEM_TRY(swem_seis_read(nm_[7].c_str(), vs.data(), hvs.data(), &fh));
+    const int nx = hv[ntraces - 1].imx - hv[0].imx + 1;
+    const int ny = hv[ntraces - 1].imy - hv[0].imy + 1;
+    if (nx < 1 || ny < 1 || (size_t)nx * ny != ntraces)
+        return set_error(SWEM_EINVAL, "shotewem: velocity headers give nx=%d ny=%d for %zu traces", nx, ny, ntraces);
+    const float oz = fh.o1, dz = fh.d1;
+    const float ox = hv[0].mx, dx = hv[1].mx - hv[0].mx, oy = hv[0].my, dy = dx;
+    const size_t nm = ntraces;
+    std::vector<float> d[3], m[3];
+    std::vector<SeisHeader> hd[3], hm[3];
+    for (int k = 0; k < 3; k++) {
+        d[k].assign(nm * nt, 0.f);
+        m[k].assign(nm * nz, 0.f);
+        hd[k].resize(nm);
+        hm[k].resize(nm);
+        const bool in_d = adj;
+        const std::string& name = in_d ? nm_[k] : nm_[3 + k];
+        const int n1 = in_d ? nt : nz;
+        swem_seis_extent_init(&fh);
+        SWEM_TRY(swem_seis_read_header(name.c_str(), &fh));
+        if ((size_t)fh.n2 * fh.n3 * fh.n4 * fh.n5 != nm || fh.n1 != n1)
+            return set_error(SWEM_EINVAL, "shotewem: %s is %d x %d, expected %d x %zu", name.c_str(), fh.n1,
+                             fh.n2 * fh.n3 * fh.n4 * fh.n5, n1, nm);
+        SWEM_TRY(swem_seis_read(name.c_str(), in_d ? d[k].data() : m[k].data(), in_d ? hd[k].data() : hm[k].data(), &fh));
+    }
+    SWEM_TRY(swem_ewem(d[0].data(), d[1].data(), d[2].data(), m[0].data(), m[1].data(), m[2].data(), wav.data(), nt, ot,
+                       dt, nx, ox, dx, ny, oy, dy, sx, sy, nz, oz, dz, gz, sz, vp.data(), vs.data(), fmin, fmax, padt,
+                       padx, adj ? 1 : 0, verbose ? 1 : 0));
+    swem_seis_extent_init(&fh);
+    if (adj) {
+        fh.n1 = nz; fh.o1 = oz; fh.d1 = dz;
+        fh.n2 = nx; fh.o2 = ox; fh.d2 = dx;
+        fh.n3 = ny; fh.o3 = oy; fh.d3 = dy;
+        for (int k = 0; k < 3; k++) {
+            for (size_t ix = 0; ix < nm; ix++) {
+                SeisHeader h = hv[ix];
+                h.trid = k == 0 ? 1 : 2;
+                h.mx = hd[0][ix].gx;
+                h.my = hd[0][ix].gy;
+                h.imx = (int)std::trunc((h.mx - ox) / dx);
+                h.imy = (int)std::trunc((h.my - oy) / dy);
+                hm[k][ix] = h;
+            }
+            SWEM_TRY(swem_seis_write(nm_[3 + k].c_str(), m[k].data(), hm[k].data(), &fh));
+        }
+    } else {
+        fh.n1 = nt; fh.o1 = ot; fh.d1 = dt;
+        fh.n2 = nx; fh.o2 = ox; fh.d2 = dx;
+        fh.n3 = ny; fh.o3 = oy; fh.d3 = dy;
+        for (int k = 0; k < 3; k++) {
+            for (size_t ix = 0; ix < nm; ix++) {
+                SeisHeader h = hm[0][ix];
+                h.n1 = nt;
+                h.d1 = dt;
+                h.o1 = ot;
+                h.gx = hm[0][ix].mx;
+                h.gy = hm[0][ix].my;
+                h.hx = hm[0][ix].mx - sx;
+                h.hy = hm[0][ix].my - sy;
+                hd[k][ix] = h;
+            }
+            SWEM_TRY(swem_seis_write(nm_[k].c_str(), d[k].data(), hd[k].data(), &fh));
+        }
+    }
+    return SWEM_OK;
+}

@@ -13,6 +13,7 @@ LIB_DIR = os.path.join(_HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libseiswem.so")
 SHOTWEM_PATH = os.path.join(LIB_DIR, "shotwem.so")
 ANGLES_PATH = os.path.join(LIB_DIR, "compute_angles.so")
+SHOTEWEM_PATH = os.path.join(LIB_DIR, "shotewem.so")
 
 SWEM_OK, SWEM_EINVAL, SWEM_ENOMEM, SWEM_EDEVICE, SWEM_EIO, SWEM_EUNSUPPORTED, SWEM_EBUSY = range(7)
 PSPI_COMPAT, PSPI_LINEAR = 0, 1
@@ -22,7 +23,7 @@ VARIANT_AUTO, VARIANT_SUBGROUP, VARIANT_WAVE, VARIANT_SUBGROUP16, VARIANT_PAIR, 
 # Every symbol declared in include/seiswem.h.
 EXPORTS = (
     "swem_version", "swem_last_error", "swem_options_init", "swem_device_count",
-    "swem_shot_main", "swem_angles_main", "swem_wem", "swem_wem_ex", "swem_shots", "swem_compute_angles", "swem_ewem",
+    "swem_shot_main", "swem_angles_main", "swem_eshot_main", "swem_wem", "swem_wem_ex", "swem_shots", "swem_compute_angles", "swem_ewem",
     "swem_plan_create", "swem_plan_destroy", "swem_plan_migrate", "swem_plan_migrate_each",
     "swem_plan_model", "swem_plan_timing", "swem_plan_bytes",
     "swem_malloc", "swem_free", "swem_memcpy_htod", "swem_memcpy_dtoh", "swem_memcpy_dtod", "swem_memset", "swem_synchronize",
@@ -139,6 +140,7 @@ def load():
         "swem_device_count": ([i32p], ctypes.c_int),
         "swem_shot_main": ([ctypes.c_int, P(ctypes.c_char_p)], ctypes.c_int),
         "swem_angles_main": ([ctypes.c_int, P(ctypes.c_char_p)], ctypes.c_int),
+        "swem_eshot_main": ([ctypes.c_int, P(ctypes.c_char_p)], ctypes.c_int),
         "swem_wem": ([f32p, f32p, f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                       ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                       ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,

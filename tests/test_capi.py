@@ -40,6 +40,16 @@ def test_shotwem_alias_exports_main():
     assert re.search(r"\bT main$", out, re.M)
 
 
+def test_shotewem_alias_exports_main_and_refuses_missing_names():
+    out = subprocess.run(["nm", "-D", "--defined-only", seiswem.SHOTEWEM_PATH], capture_output=True, text=True).stdout
+    assert re.search(r"\bT main$", out, re.M)
+    so = ctypes.CDLL(seiswem.SHOTEWEM_PATH)
+    so.main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    so.main.restype = ctypes.c_int
+    argv = [b"0", b"adj=y", b"ux=a", b"uy=b", b"uz=c", b"mpp=d", b"mps1=e", b"mps2=f", b"vp=g", b"vs=h"]
+    assert so.main(len(argv), (ctypes.c_char_p * len(argv))(*argv)) == _lib.SWEM_EINVAL  # no wav=
+
+
 def test_compute_angles_alias_exports_main():
     out = subprocess.run(["nm", "-D", "--defined-only", seiswem.ANGLES_PATH], capture_output=True, text=True).stdout
     assert re.search(r"\bT main$", out, re.M)

@@ -214,3 +214,48 @@ def test_shot_profile_wem_files_angle_gathers(gpu, tmp_path, monkeypatch):
     dm = seiswem.ShotProfileWEMFiles("m.seis", "dm.seis", False, sx=sx, sy=np.zeros(2, np.float32), **common)
     dref = seiswem.ShotProfileWEMAngles(gath, False, vel=vel, wav=wav, geom=g, sx=sx, angx=angx, angy=angy, **akw)
     assert rel_l2(dm, dref) <= 1e-6
+
+
+def test_shotewem_main_migrates_and_models_through_files(gpu, tmp_path, monkeypatch):
+    """shotewem.so's main (shotewem.c:11-223): the three component files in, the three image files
+    out (trid 1 / 2, mx from the data's gx), equal to swem_ewem on the same arrays; then modelling
+    back to component files with the gx / hx rewrites."""
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("DATAPATH", raising=False)
+    lib = seiswem.load()
+    nx, nz, nt, dx, dz, dt = 64, 30, 256, 10.0, 10.0, 0.004
+    rng = np.random.default_rng(11)
+    vp, wav, mx = _grid_files(lib, nx, nz, nt, dx, dz, dt, rng)
+    vs = (vp / 1.8).astype(np.float32)
+    write_seis(lib, "vs.seis", vs, header_array(nx, tracenum=np.arange(1, nx + 1), n1=nz, d1=dz, mx=mx,
+                                                imx=np.arange(nx)), nz, nx, o1=0.0, d1=dz, o2=0.0, d2=dx)
+    comps = rng.standard_normal((3, nx, nt)).astype(np.float32)
+    for k, name in enumerate(("ux", "uy", "uz")):
+        write_seis(lib, f"{name}.seis", comps[k], header_array(nx, n1=nt, d1=dt, sx=250.0, gx=mx), nt, nx, o1=0.0,
+                   d1=dt, o2=0.0, d2=dx)
+    so = ctypes.CDLL(seiswem.SHOTEWEM_PATH)
+    so.main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    so.main.restype = ctypes.c_int
+
+    def run(args):
+        argv = [b"0"] + [a.encode() for a in args]
+        return so.main(len(argv), (ctypes.c_char_p * len(argv))(*argv))
+
+    files = ["ux=ux.seis", "uy=uy.seis", "uz=uz.seis", "mpp=mpp.seis", "mps1=mps1.seis", "mps2=mps2.seis",
+             "vp=vel.seis", "vs=vs.seis", "wav=wav.seis", "sx=250", "fmax=30", "padt=2", "padx=2"]
+    assert run(["adj=y"] + files) == 0
+    ref = seiswem.ewem(comps[0], comps[1], comps[2], None, None, None, wav, nt, 0.0, dt, nx, 0.0, dx, 1, 0.0, dx,
+                       250.0, 0.0, nz, 0.0, dz, 0.0, 0.0, vp, vs, 0.0, 30.0, 2, 2, True)
+    for k, name in enumerate(("mpp", "mps1", "mps2")):
+        img, h, fh = read_seis(f"{name}.seis", nz, nx)
+        assert np.array_equal(img, ref[k]) and (fh.n1, fh.n2) == (nz, nx)
+        assert np.all(h["trid"] == (1 if k == 0 else 2)) and np.array_equal(h["imx"], np.arange(nx))
+    assert np.linalg.norm(ref[0]) > 0 and np.all(ref[1] == 0)
+    assert run(["adj=n"] + [f.replace("ux.seis", "mx.seis").replace("uy.seis", "my.seis").replace("uz.seis", "mz.seis")
+                            for f in files]) == 0
+    dref = seiswem.ewem(None, None, None, ref[0], ref[1], ref[2], wav, nt, 0.0, dt, nx, 0.0, dx, 1, 0.0, dx, 250.0, 0.0,
+                        nz, 0.0, dz, 0.0, 0.0, vp, vs, 0.0, 30.0, 2, 2, False)
+    for k, name in enumerate(("mx", "my", "mz")):
+        d_k, h, fh = read_seis(f"{name}.seis", nt, nx)
+        assert np.array_equal(d_k, dref[k]) and (fh.n1, fh.n2) == (nt, nx)
+        assert np.array_equal(h["gx"], mx.astype(np.float32)) and np.array_equal(h["hx"], mx.astype(np.float32) - 250.0)
