@@ -348,6 +348,53 @@ def bench_angles(args):
     print(json.dumps(out))
 
 
+def bench_elastic(args):
+    """Elastic migration (ewem.c) on the config-2 grid: nx=1024, nz=500, nt=2000, dt=2 ms, 0-80 Hz
+    (padt = padx = 2: 641 bins, nkx = 2048), vp = the config-2 velocity, vs = vp / 1.8, synthetic
+    three-component records; one shot per call of swem_ewem (host arrays in and out).  CPU
+    baseline: the FP32 oracle on one shot of a reduced band."""
+    import seiswem
+    from seiswem import synth
+
+    g, vp, _, wav, sx = synth.config2(nshot=args.shots)
+    vs = (vp / 1.8).astype(np.float32)
+    rng = np.random.default_rng(0)
+    comps = rng.standard_normal((3, g.nm, g.nt)).astype(np.float32)
+    geo = (g.nt, 0.0, g.dt, g.nmx, 0.0, g.dmx, 1, 0.0, g.dmx)
+
+    def shot(s):
+        return seiswem.ewem(comps[0], comps[1], comps[2], None, None, None, wav, *geo, float(s), 0.0, g.nz, 0.0,
+                            g.dz, 0.0, 0.0, vp, vs, 0.0, 80.0, 2, 2, True)
+
+    for s in sx[: args.warmup]:
+        shot(s)
+    k = min(args.steps, sx.size)
+    t0 = time.perf_counter()
+    for s in sx[:k]:
+        shot(s)
+    el = time.perf_counter() - t0
+    out = {
+        "metric": "elastic migration: shots/sec (ewem, 2-D)", "value": k / el, "unit": "shots/s", "n_gpus": 1,
+        "steps": k, "warmup": args.warmup, "ms_per_step": el / k * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (config-2 velocity, vs = vp / 1.8, random records)",
+        "config": {"workload": "ewem, nx=1024 nz=500 nt=2000 dt=2 ms, 0-80 Hz (641 bins), nkx=2048, one shot per call"},
+    }
+    if not args.no_cpu:
+        import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        p = O.WemParams(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, fmin=79.0, fmax=80.0, pspi=False)
+        sp = seiswem.WemGeometry(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, fmin=79.0, fmax=80.0,
+                                 pspi=False).sampling()
+        nb = sp["ifmax"] - sp["ifmin"]
+        t0 = time.perf_counter()
+        O.ewem(p, vp, vs, wav, float(sx[0]), ux=comps[0], uz=comps[2], adj=True, f64=False)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": nb / tc / 641, "unit": "shots/s", "cores": 1, "kind": "port",
+                               "sample": f"1 shot, {nb} of 641 frequency bins (79-80 Hz) at full nx/nz, {tc:.2f} s, "
+                                         "scaled by bins (the FP32 oracle is serial over frequencies)"}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,7 +406,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="pspi2d: shots timed one per call on the plain and resident single-shot paths (0: skip)")
-    ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm", "angles"), default="pspi2d")
+    ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm", "angles", "elastic"), default="pspi2d")
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--plain-sample", type=int, default=24, help="serving: requests timed on the plain path")
@@ -380,6 +427,8 @@ def main():
         return bench_lsm(args)
     if args.workload == "angles":
         return bench_angles(args)
+    if args.workload == "elastic":
+        return bench_elastic(args)
 
     import seiswem
     from seiswem import synth
