@@ -158,6 +158,15 @@ int swem_ewem(float* ux, float* uy, float* uz, float* mpp, float* mps1, float* m
               const float* vp, const float* vs, float fmin, float fmax,
               int padt, int padx, int adj, int verbose);
 
+/* ShotProfileEWEM (ShotProfileEWEM.jl) in memory, 2-D: nshot shots over one pair of velocity
+ * grids, all shots of a chunk in one launch.  adj=1: mpp, mps1, mps2 [nm][nz] = sums over shots of
+ * ewem's images of ux, uz [nshot][nm][nt] (uy unused); adj=0: ux, uy, uz [nshot][nm][nt] from the
+ * images (uy zero).  sy may be NULL. */
+int swem_eshots(const swem_geometry* g, const float* vp, const float* vs, const float* wav,
+                const float* sx, const float* sy, int nshot,
+                float* ux, float* uy, float* uz, float* mpp, float* mps1, float* mps2,
+                int adj, const swem_options* opt);
+
 /* wem() with explicit options (pspi_mode, precision, device...). */
 int swem_wem_ex(float* d, float* m, const float* wav, const swem_geometry* g,
                 float sx, float sy, const float* vel, int adj, const swem_options* opt);

@@ -1370,8 +1370,10 @@ static void ewem_sampling(const swem_geometry& g, int& ntfft, int& ifmin, int& i
     ifmin = t + 1 < (float)ifmax ? (int)std::trunc(t) : 0;
 }
 
-static int ewem_impl(swem_plan* pp, swem_plan* ps, int igx, float* ux, float* uy, float* uz, float* mpp, float* mps1,
-                     float* mps2, const float* wav, int adj)
+// nshot shots of elastic migration (records [nshot][nm][nt] -> images [nm][nz] summed over shots)
+// or modelling (images -> records of every shot), all shots of a chunk in one launch.
+static int eshots_impl(swem_plan* pp, swem_plan* ps, int nshot, const int* igx, float* ux, float* uy, float* uz,
+                       float* mpp, float* mps1, float* mps2, const float* wav, int adj)
 {
     hipStream_t st = nullptr;
     const int nmx = pp->g.nmx, nz = pp->g.nz, nt = pp->g.nt, nf = pp->nf;
@@ -1381,9 +1383,9 @@ static int ewem_impl(swem_plan* pp, swem_plan* ps, int igx, float* ux, float* uy
         std::fill(mps1, mps1 + nmz, 0.f);
         std::fill(mps2, mps2 + nmz, 0.f);
     } else {
-        std::fill(ux, ux + nmt, 0.f);
-        std::fill(uy, uy + nmt, 0.f);
-        std::fill(uz, uz + nmt, 0.f);
+        std::fill(ux, ux + nshot * nmt, 0.f);
+        std::fill(uy, uy + nshot * nmt, 0.f);
+        std::fill(uz, uz + nshot * nmt, 0.f);
     }
     if (nf < 1) return SWEM_OK;
     if (pp->nchunks != ps->nchunks || pp->nfc != ps->nfc)
@@ -1424,18 +1426,20 @@ static int ewem_impl(swem_plan* pp, swem_plan* ps, int igx, float* ux, float* uy
     DevBuf<int> dig;
     SWEM_TRY(dvp0.upload(vp0.data(), nz));
     SWEM_TRY(dvs0.upload(vs0.data(), nz));
-    SWEM_TRY(dig.upload(&igx, 1));
+    SWEM_TRY(dig.upload(igx, nshot));
     const size_t nfc = (size_t)pp->nfc;
+    // shots per launch: records, spectra and the per-frequency partial images or stored sources
+    const size_t per_shot = 2 * nmt * sizeof(float) + 2 * (size_t)nmx * nf * sizeof(float2) +
+                            (adj ? 2 * nfc * nmz * sizeof(float) : nfc * nmz * sizeof(float2));
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nshot, pp->budget / std::max<size_t>(per_shot, 1)));
+    SWEM_TRY(drx.alloc((size_t)chunk * nmt));
+    SWEM_TRY(drz.alloc((size_t)chunk * nmt));
     if (adj) {
         SWEM_TRY(dbadj.upload(badj.data(), badj.size()));
-        SWEM_TRY(drx.upload(ux, nmt));
-        SWEM_TRY(drz.upload(uz, nmt));
-        SWEM_TRY(spx.alloc((size_t)nmx * nf));
-        SWEM_TRY(spz.alloc((size_t)nmx * nf));
-        SWEM_TRY(launch_sgemm(nmx, 2 * nf, nt, drx.p, nt, dbadj.p, 2 * nf, reinterpret_cast<float*>(spx.p), 2 * nf, st));
-        SWEM_TRY(launch_sgemm(nmx, 2 * nf, nt, drz.p, nt, dbadj.p, 2 * nf, reinterpret_cast<float*>(spz.p), 2 * nf, st));
-        SWEM_TRY(ppart.alloc(nfc * nmz));
-        SWEM_TRY(spart.alloc(nfc * nmz));
+        SWEM_TRY(spx.alloc((size_t)chunk * nmx * nf));
+        SWEM_TRY(spz.alloc((size_t)chunk * nmx * nf));
+        SWEM_TRY(ppart.alloc((size_t)chunk * nfc * nmz));
+        SWEM_TRY(spart.alloc((size_t)chunk * nfc * nmz));
         SWEM_TRY(dmpp.alloc(nmz));
         SWEM_TRY(dmps.alloc(nmz));
     } else {
@@ -1448,51 +1452,99 @@ static int ewem_impl(swem_plan* pp, swem_plan* ps, int igx, float* ux, float* uy
             }
         SWEM_TRY(mppT.upload(t1.data(), nmz));
         SWEM_TRY(mpsT.upload(t2.data(), nmz));
-        SWEM_TRY(smig.alloc(nfc * nmz));
-        SWEM_TRY(dgx.alloc((size_t)nmx * nf));
-        SWEM_TRY(dgz.alloc((size_t)nmx * nf));
-        SWEM_TRY(drx.alloc(nmt));
-        SWEM_TRY(drz.alloc(nmt));
+        SWEM_TRY(smig.alloc((size_t)chunk * nfc * nmz));
+        SWEM_TRY(dgx.alloc((size_t)chunk * nmx * nf));
+        SWEM_TRY(dgz.alloc((size_t)chunk * nmx * nf));
     }
-    for (int c = 0; c < pp->nchunks; c++) {
-        SWEM_TRY(ensure_chunk(pp, c, st));
-        SWEM_TRY(ensure_chunk(ps, c, st));
-        March2DArgs ap = base_args(pp, c, 1), as = base_args(ps, c, 1);
-        ap.nshot = as.nshot = 1;
-        ap.igx = as.igx = dig.p;
-        EwemArgs e{};
-        e.nf = nf;
-        e.dwf = pp->s.dw;
-        e.dkx = pp->s.dkx;
-        e.inv_maxs = 1.f / maxs;
-        e.vp0 = dvp0.p;
-        e.vs0 = dvs0.p;
-        e.spx = spx.p;
-        e.spz = spz.p;
-        e.ppart = ppart.p;
-        e.spart = spart.p;
-        e.smig = smig.p;
-        e.mppT = mppT.p;
-        e.mpsT = mpsT.p;
-        e.dgx = dgx.p;
-        e.dgz = dgz.p;
-        SWEM_TRY(launch_ewem(ap, as, e, adj, st));
+    bool first_image = true;
+    for (int s0 = 0; s0 < nshot; s0 += chunk) {
+        const int ns = std::min(chunk, nshot - s0);
         if (adj) {
-            SWEM_TRY(launch_reduce_image(ppart.p, 1, ap.ngroups, nz, nmx, dmpp.p, c > 0, 0, st));
-            SWEM_TRY(launch_reduce_image(spart.p, 1, ap.ngroups, nz, nmx, dmps.p, c > 0, 0, st));
+            SWEM_HIP(hipMemcpyAsync(drx.p, ux + s0 * nmt, ns * nmt * sizeof(float), hipMemcpyHostToDevice, st));
+            SWEM_HIP(hipMemcpyAsync(drz.p, uz + s0 * nmt, ns * nmt * sizeof(float), hipMemcpyHostToDevice, st));
+            SWEM_TRY(launch_sgemm(ns * nmx, 2 * nf, nt, drx.p, nt, dbadj.p, 2 * nf, reinterpret_cast<float*>(spx.p), 2 * nf,
+                                  st));
+            SWEM_TRY(launch_sgemm(ns * nmx, 2 * nf, nt, drz.p, nt, dbadj.p, 2 * nf, reinterpret_cast<float*>(spz.p), 2 * nf,
+                                  st));
+        }
+        for (int c = 0; c < pp->nchunks; c++) {
+            SWEM_TRY(ensure_chunk(pp, c, st));
+            SWEM_TRY(ensure_chunk(ps, c, st));
+            March2DArgs ap = base_args(pp, c, 1), as = base_args(ps, c, 1);
+            ap.nshot = as.nshot = ns;
+            ap.igx = as.igx = dig.p + s0;
+            EwemArgs e{};
+            e.nf = nf;
+            e.dwf = pp->s.dw;
+            e.dkx = pp->s.dkx;
+            e.inv_maxs = 1.f / maxs;
+            e.vp0 = dvp0.p;
+            e.vs0 = dvs0.p;
+            e.spx = spx.p;
+            e.spz = spz.p;
+            e.ppart = ppart.p;
+            e.spart = spart.p;
+            e.smig = smig.p;
+            e.mppT = mppT.p;
+            e.mpsT = mpsT.p;
+            e.dgx = dgx.p;
+            e.dgz = dgz.p;
+            SWEM_TRY(launch_ewem(ap, as, e, adj, st));
+            if (adj) {
+                SWEM_TRY(launch_reduce_image(ppart.p, ns, ap.ngroups, nz, nmx, dmpp.p, !first_image, 0, st));
+                SWEM_TRY(launch_reduce_image(spart.p, ns, ap.ngroups, nz, nmx, dmps.p, !first_image, 0, st));
+                first_image = false;
+            }
+        }
+        if (!adj) {
+            SWEM_TRY(launch_sgemm(ns * nmx, nt, 2 * nf, reinterpret_cast<const float*>(dgx.p), 2 * nf, dbfwd.p, nt, drx.p,
+                                  nt, st));
+            SWEM_TRY(launch_sgemm(ns * nmx, nt, 2 * nf, reinterpret_cast<const float*>(dgz.p), 2 * nf, dbfwd.p, nt, drz.p,
+                                  nt, st));
+            SWEM_HIP(hipMemcpyAsync(ux + s0 * nmt, drx.p, ns * nmt * sizeof(float), hipMemcpyDeviceToHost, st));
+            SWEM_HIP(hipMemcpyAsync(uz + s0 * nmt, drz.p, ns * nmt * sizeof(float), hipMemcpyDeviceToHost, st));
+            SWEM_HIP(hipStreamSynchronize(st));  // the record buffers are reused by the next chunk
         }
     }
     if (adj) {
         SWEM_HIP(hipMemcpyAsync(mpp, dmpp.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
         SWEM_HIP(hipMemcpyAsync(mps2, dmps.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
-    } else {
-        SWEM_TRY(launch_sgemm(nmx, nt, 2 * nf, reinterpret_cast<const float*>(dgx.p), 2 * nf, dbfwd.p, nt, drx.p, nt, st));
-        SWEM_TRY(launch_sgemm(nmx, nt, 2 * nf, reinterpret_cast<const float*>(dgz.p), 2 * nf, dbfwd.p, nt, drz.p, nt, st));
-        SWEM_HIP(hipMemcpyAsync(ux, drx.p, nmt * sizeof(float), hipMemcpyDeviceToHost, st));
-        SWEM_HIP(hipMemcpyAsync(uz, drz.p, nmt * sizeof(float), hipMemcpyDeviceToHost, st));
     }
     SWEM_HIP(hipStreamSynchronize(st));
     return SWEM_OK;
+}
+
+int swem_eshots(const swem_geometry* g, const float* vp, const float* vs, const float* wav, const float* sx,
+                const float* sy, int nshot, float* ux, float* uy, float* uz, float* mpp, float* mps1, float* mps2,
+                int adj, const swem_options* opt)
+{
+    if (!g || !vp || !vs || !wav || !sx || !ux || !uy || !uz || !mpp || !mps1 || !mps2 || nshot < 1)
+        return set_error(SWEM_EINVAL, "ewem: null argument or nshot < 1");
+    if (g->nmy > 1) return set_error(SWEM_EUNSUPPORTED, "ewem: 3-D (nmy > 1) is not available in this build");
+    swem_geometry ge = *g;
+    ge.pspi = 0;
+    int ntfft, ifmin, ifmax;
+    ewem_sampling(ge, ntfft, ifmin, ifmax);
+    std::vector<int> igx(nshot);
+    for (int i = 0; i < nshot; i++) {
+        igx[i] = (int)std::trunc((sx[i] - ge.omx) / ge.dmx);  // ewem.c:99
+        if (igx[i] < 0 || igx[i] >= ge.nmx) return set_error(SWEM_EINVAL, "ewem: source x %g outside the grid", (double)sx[i]);
+    }
+    (void)sy;
+    swem_options o;
+    if (opt) o = *opt;
+    else swem_options_init(&o);
+    swem_plan *pp = nullptr, *ps = nullptr;
+    SWEM_TRY(swem_plan_create(&ge, vp, wav, &o, &pp));
+    int rc = swem_plan_create(&ge, vs, wav, &o, &ps);
+    if (!rc && (pp->s.ntfft != ntfft || pp->s.ifmin != ifmin || pp->s.ifmax != ifmax))
+        rc = set_error(SWEM_EUNSUPPORTED, "ewem: this geometry's band differs between ewem.c (ntfft %d, [%d, %d)) and "
+                       "wem.c (ntfft %d, [%d, %d)), whose sampling the plans use", ntfft, ifmin, ifmax, pp->s.ntfft,
+                       pp->s.ifmin, pp->s.ifmax);
+    if (!rc) rc = eshots_impl(pp, ps, nshot, igx.data(), ux, uy, uz, mpp, mps1, mps2, wav, adj);
+    if (ps) swem_plan_destroy(ps);
+    swem_plan_destroy(pp);
+    return rc;
 }
 
 int swem_ewem(float* ux, float* uy, float* uz, float* mpp, float* mps1, float* mps2, const float* wav, int nt,
@@ -1502,27 +1554,12 @@ int swem_ewem(float* ux, float* uy, float* uz, float* mpp, float* mps1, float* m
 {
     if (!ux || !uy || !uz || !mpp || !mps1 || !mps2 || !wav || !vp || !vs)
         return set_error(SWEM_EINVAL, "ewem: null argument");
-    if (nmy > 1) return set_error(SWEM_EUNSUPPORTED, "ewem: 3-D (nmy > 1) is not available in this build");
     swem_geometry g;
     geom_from_args(&g, nt, ot, dt, nmx, omx, dmx, nmy, omy, dmy, nz, oz, dz, gz, sz, 1, fmin, fmax, padt, padx, 0);
-    int ntfft, ifmin, ifmax;
-    ewem_sampling(g, ntfft, ifmin, ifmax);
-    const int igx = (int)std::trunc((sx - omx) / dmx);
-    if (igx < 0 || igx >= nmx) return set_error(SWEM_EINVAL, "ewem: source x %g outside the grid", (double)sx);
     swem_options o;
     swem_options_init(&o);
     o.verbose = verbose;
-    swem_plan *pp = nullptr, *ps = nullptr;
-    SWEM_TRY(swem_plan_create(&g, vp, wav, &o, &pp));
-    int rc = swem_plan_create(&g, vs, wav, &o, &ps);
-    if (!rc && (pp->s.ntfft != ntfft || pp->s.ifmin != ifmin || pp->s.ifmax != ifmax))
-        rc = set_error(SWEM_EUNSUPPORTED, "ewem: this geometry's band differs between ewem.c (ntfft %d, [%d, %d)) and "
-                       "wem.c (ntfft %d, [%d, %d)), whose sampling the plans use", ntfft, ifmin, ifmax, pp->s.ntfft,
-                       pp->s.ifmin, pp->s.ifmax);
-    if (!rc) rc = ewem_impl(pp, ps, igx, ux, uy, uz, mpp, mps1, mps2, wav, adj);
-    if (ps) swem_plan_destroy(ps);
-    swem_plan_destroy(pp);
-    return rc;
+    return swem_eshots(&g, vp, vs, wav, &sx, &sy, 1, ux, uy, uz, mpp, mps1, mps2, adj, &o);
 }
 
 int swem_malloc(void** ptr, size_t bytes)

@@ -108,23 +108,23 @@ struct AngleArgs {
 int launch_angles(int which, const March2DArgs& ax, const March2DArgs& az, const AngleArgs& b, int F, hipStream_t st);
 int angles_supported(int n);
 
-// Elastic WEM (ewem.c), 2-D.  One workgroup per frequency of the chunk; ap / as carry the P and
-// S plans' tables (kz phases of po_p / po_s, split-step phases of pd_p / pd_s), ap also the source
-// spectrum, igx and the geometry.
+// Elastic WEM (ewem.c), 2-D.  One workgroup per (frequency of the chunk, shot); ap / as carry
+// the P and S plans' tables (kz phases of po_p / po_s, split-step phases of pd_p / pd_s), ap also
+// the source spectrum, igx [nshot], nshot and the geometry.
 struct EwemArgs {
     int nf;                 // band size (stride of the spectra)
     float dwf, dkx;         // float dw and dkx, as the reference computes w and kx
     float inv_maxs;         // 1 / (max|wav| / sqrt(ntfft)) (ewem.c:146)
     const float* vp0;       // [nz] 1 / po_p (separation)
     const float* vs0;       // [nz] 1 / po_s
-    const float2* spx;      // [nmx][nf] R2C(ux) / sqrt(ntfft) (migration)
-    const float2* spz;      // [nmx][nf]
-    float* ppart;           // [nfc][nz][nmx] partial P image (migration)
-    float* spart;           // [nfc][nz][nmx] partial S image
-    float2* smig;           // [nfc][nz][nmx] stored source (modelling)
+    const float2* spx;      // [nshot][nmx][nf] R2C(ux) / sqrt(ntfft) (migration)
+    const float2* spz;      // [nshot][nmx][nf]
+    float* ppart;           // [nshot][nfc][nz][nmx] partial P image (migration)
+    float* spart;           // [nshot][nfc][nz][nmx] partial S image
+    float2* smig;           // [nshot][nfc][nz][nmx] stored source (modelling)
     const float* mppT;      // [nz][nmx] images (modelling input)
     const float* mpsT;
-    float2* dgx;            // [nmx][nf] modelled spectra (modelling output)
+    float2* dgx;            // [nshot][nmx][nf] modelled spectra (modelling output)
     float2* dgz;
 };
 int launch_ewem(const March2DArgs& ap, const March2DArgs& as, const EwemArgs& e, int adj, hipStream_t st);

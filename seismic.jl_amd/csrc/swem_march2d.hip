@@ -1439,17 +1439,19 @@ __global__ __launch_bounds__(N / P) __attribute__((amdgpu_waves_per_eu(2))) void
     int pp = 0;
     float tap[P];
     load_taper<N, P, kModeSSF>(tap, ap, t);
-    const int igx = ap.igx[0];
+    const int igx = ap.igx[shot];
     const float factor = iw == 0 ? 1.f : 2.f;
+    const size_t sp = (size_t)shot * ap.nmx * e.nf;  // this shot's spectra
     float2 us[P], rx[P], rz[P];
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const int x = t + q * T;
         us[q] = (active && x == igx) ? ap.spec_s[fl] : make_float2(0.f, 0.f);
-        rx[q] = (ADJ && active && x < ap.nmx) ? e.spx[(size_t)x * e.nf + fl] : make_float2(0.f, 0.f);
-        rz[q] = (ADJ && active && x < ap.nmx) ? e.spz[(size_t)x * e.nf + fl] : make_float2(0.f, 0.f);
+        rx[q] = (ADJ && active && x < ap.nmx) ? e.spx[sp + (size_t)x * e.nf + fl] : make_float2(0.f, 0.f);
+        rz[q] = (ADJ && active && x < ap.nmx) ? e.spz[sp + (size_t)x * e.nf + fl] : make_float2(0.f, 0.f);
     }
-    const size_t vol = (size_t)(active ? fc : 0) * ap.nz * ap.nmx;
+    // partial images / stored source of (shot, frequency)
+    const size_t vol = ((size_t)shot * ap.ngroups + (active ? fc : 0)) * ap.nz * ap.nmx;
     if constexpr (ADJ) {
         for (int iz = 0; iz < ap.nz; ++iz) {  // elastic_extrap1f, ewem.c:302-327
             c.iz = iz;
@@ -1525,8 +1527,8 @@ __global__ __launch_bounds__(N / P) __attribute__((amdgpu_waves_per_eu(2))) void
             for (int q = 0; q < P; ++q) {
                 const int x = t + q * T;
                 if (x < ap.nmx) {
-                    e.dgx[(size_t)x * e.nf + fl] = rx[q];
-                    e.dgz[(size_t)x * e.nf + fl] = rz[q];
+                    e.dgx[sp + (size_t)x * e.nf + fl] = rx[q];
+                    e.dgz[sp + (size_t)x * e.nf + fl] = rz[q];
                 }
             }
         }
@@ -1556,7 +1558,7 @@ static int launch_ewem_len(int blue, const March2DArgs& ap, const March2DArgs& a
 int launch_ewem(const March2DArgs& ap, const March2DArgs& as, const EwemArgs& e, int adj, hipStream_t st)
 {
     const int n = ap.nkx, m = march2d_blue_len(n), len = m ? m : n;
-    if (!angles_supported(n) || as.nkx != n || (m && (!ap.chirp || !as.chirp)) || ap.nshot != 1 ||
+    if (!angles_supported(n) || as.nkx != n || (m && (!ap.chirp || !as.chirp)) || ap.nshot < 1 ||
         ap.ngroups != ap.nfc || ap.nmx > n)
         return set_error(SWEM_EINVAL, "ewem: inconsistent launch (nkx=%d nmx=%d)", n, ap.nmx);
     switch (len) {

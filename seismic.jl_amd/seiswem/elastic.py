@@ -10,6 +10,9 @@ from __future__ import annotations
 
 import numpy as np
 
+import ctypes
+
+from . import _lib
 from ._lib import check, load
 from .wem import WemGeometry, _fp
 
@@ -34,26 +37,23 @@ def ewem(ux, uy, uz, mpp, mps1, mps2, wav, nt, ot, dt, nmx, omx, dmx, nmy, omy, 
 
 
 def ShotProfileEWEM(x, adj: bool = True, *, vp, vs, wav, geom: WemGeometry, sx=(0.0,), sy=None):
-    """ShotProfileEWEM (ShotProfileEWEM.jl) on arrays.  adj: x = (ux, uy, uz), each
-    [nshot, nm, nt] -> (mpp, mps1, mps2) [nm, nz] summed over shots; else x = (mpp, mps1, mps2)
-    -> (ux, uy, uz) [nshot, nm, nt]."""
+    """ShotProfileEWEM (ShotProfileEWEM.jl) on arrays, one device call for all shots
+    (swem_eshots).  adj: x = (ux, uy, uz), each [nshot, nm, nt] -> (mpp, mps1, mps2) [nm, nz]
+    summed over shots; else x = (mpp, mps1, mps2) -> (ux, uy, uz) [nshot, nm, nt]."""
     g = geom
-    sx = np.atleast_1d(np.asarray(sx, np.float32))
-    sy = np.zeros_like(sx) if sy is None else np.atleast_1d(np.asarray(sy, np.float32))
-    dmy = g.dmy if g.dmy > 0 else g.dmx
-    common = (g.nt, g.ot, g.dt, g.nmx, g.omx, g.dmx, g.nmy, g.omy, dmy)
+    sx = np.ascontiguousarray(np.atleast_1d(sx), dtype=np.float32)
+    ns = sx.size
+    syv = None if sy is None else np.ascontiguousarray(np.atleast_1d(sy), dtype=np.float32)
     if adj:
-        out = [np.zeros((g.nm, g.nz), np.float32) for _ in range(3)]
-        for i in range(sx.size):
-            m = ewem(x[0][i], x[1][i], x[2][i], None, None, None, wav, *common, float(sx[i]), float(sy[i]), g.nz, g.oz,
-                     g.dz, g.gz, g.sz, vp, vs, g.fmin, g.fmax, g.padt, g.padx, True)
-            for k in range(3):
-                out[k] += m[k]
-        return tuple(out)
-    out = [np.zeros((sx.size, g.nm, g.nt), np.float32) for _ in range(3)]
-    for i in range(sx.size):
-        d = ewem(None, None, None, x[0], x[1], x[2], wav, *common, float(sx[i]), float(sy[i]), g.nz, g.oz, g.dz, g.gz,
-                 g.sz, vp, vs, g.fmin, g.fmax, g.padt, g.padx, False)
-        for k in range(3):
-            out[k][i] = d[k]
-    return tuple(out)
+        d = [np.ascontiguousarray(a, dtype=np.float32).reshape(ns, g.nm, g.nt) for a in x]
+        m = [np.zeros((g.nm, g.nz), np.float32) for _ in range(3)]
+    else:
+        m = [np.ascontiguousarray(a, dtype=np.float32).reshape(g.nm, g.nz) for a in x]
+        d = [np.zeros((ns, g.nm, g.nt), np.float32) for _ in range(3)]
+    geo = g.c()
+    opts = _lib.options()
+    check(load().swem_eshots(ctypes.byref(geo), _fp(np.ascontiguousarray(vp, np.float32)),
+                             _fp(np.ascontiguousarray(vs, np.float32)),
+                             _fp(np.ascontiguousarray(wav, np.float32).ravel()), _fp(sx),
+                             None if syv is None else _fp(syv), ns, *[_fp(a) for a in d + m], int(adj), ctypes.byref(opts)))
+    return tuple(m) if adj else tuple(d)

@@ -73,6 +73,8 @@ def test_ewem_dot_product(gpu):
 
 
 def test_shot_profile_ewem_stacks_shots(gpu):
+    """Two shots in one launch (swem_eshots): the stack against the oracle, the modelled shots
+    against the same shots modelled alone."""
     g, p, vp, vs, wav, rng = case(64, seed=7)
     sx = np.array([150.0, 400.0], np.float32)
     ux, uy, uz = rng.standard_normal((3, 2, g.nm, g.nt)).astype(np.float32)
@@ -81,6 +83,11 @@ def test_shot_profile_ewem_stacks_shots(gpu):
     assert rel_l2(mpp, refs[0][0] + refs[1][0]) <= TOL and rel_l2(mps2, refs[0][1] + refs[1][1]) <= TOL
     dx, dy, dz = seiswem.ShotProfileEWEM((mpp, mps1, mps2), False, vp=vp, vs=vs, wav=wav, geom=g, sx=sx)
     assert dx.shape == (2, g.nm, g.nt) and np.all(dy == 0)
+    for i, s in enumerate(sx):  # each shot of the batched launch equals the shot alone
+        one = run(g, vp, vs, wav, float(s), False, mpp, mps1, mps2)
+        assert np.array_equal(dx[i], one[0]) and np.array_equal(dz[i], one[2])
+    alone = [run(g, vp, vs, wav, float(s), True, ux[i], uy[i], uz[i]) for i, s in enumerate(sx)]
+    assert rel_l2(mpp, alone[0][0] + alone[1][0]) <= 1e-6
 
 
 def test_ewem_refuses_3d_and_mismatched_band(gpu):
