@@ -142,6 +142,11 @@ int swem_compute_angles(float* angx, float* angy, const float* wav,
                         int nz, float oz, float dz, float sz,
                         const float* vel, float fmin, float fmax, int verbose);
 
+/* compute_angles with a geometry and explicit options (device, work budget scratch_bytes, table
+ * cap table_bytes); padt, padx, pspi, nref and gz of the geometry are not used. */
+int swem_compute_angles_ex(float* angx, float* angy, const float* wav, const swem_geometry* g,
+                           float sx, float sy, const float* vel, const swem_options* opt);
+
 /* Replaces ewem() (ewem.c:3-15), elastic shot-profile migration (adj=1: components ux, uy, uz
  * [nm][nt] -> images mpp, mps1, mps2 [nm][nz]) and modelling (adj=0: the reverse), with the P
  * source, the P / S separation of the receiver components at every depth and split-step

@@ -1161,8 +1161,10 @@ static int angles_impl(swem_plan* p, float sx, float sy, float* angx, float* ang
     DevBuf<float> dzmag, acc, sgn;
     March2DArgs az;
     SWEM_TRY(angle_z_tables(nz, twz, chz, bchz, az));
-    SWEM_TRY(vol.alloc((size_t)p->nfc * nmz));
-    SWEM_TRY(dzmag.alloc((size_t)p->nfc * nmz));
+    // frequencies per pass: the volume (complex) and |u_z| (float) of each within half the budget
+    const int nsub = (int)std::max<size_t>(1, std::min<size_t>(p->nfc, p->budget / 2 / (nmz * 12)));
+    SWEM_TRY(vol.alloc((size_t)nsub * nmz));
+    SWEM_TRY(dzmag.alloc((size_t)nsub * nmz));
     SWEM_TRY(acc.alloc(nmz));
     SWEM_TRY(sgn.alloc(nmz));
     SWEM_HIP(hipMemsetAsync(acc.p, 0, nmz * sizeof(float), st));
@@ -1185,18 +1187,26 @@ static int angles_impl(swem_plan* p, float sx, float sy, float* angx, float* ang
     b.sgn = sgn.p;
     for (int c = 0; c < p->nchunks; c++) {
         SWEM_TRY(ensure_chunk(p, c, st));
-        March2DArgs ax = base_args(p, c, F);
-        ax.nshot = 1;
-        ax.igx = p->igx.p;
-        ax.smig = vol.p;
-        b.nfc = ax.nfc;
-        SWEM_TRY(launch_angles(0, ax, az, b, F, st));
-        // z pass: the z line of every (frequency, x)
-        b.inner = nmx, b.lines = ax.nfc * nmx, b.outer = (long long)nmz, b.inner_stride = 1, b.elem = nmx;
-        SWEM_TRY(launch_angles(1, ax, az, b, F, st));
-        // x pass: the x line of every depth
-        b.inner = 1, b.lines = nz, b.outer = nmx, b.inner_stride = 0, b.elem = 1;
-        SWEM_TRY(launch_angles(2, ax, az, b, F, st));
+        const March2DArgs a0 = base_args(p, c, F);
+        for (int f1 = 0; f1 < a0.nfc; f1 += nsub) {  // a pass over frequencies f1 .. of the chunk's tables
+            March2DArgs ax = a0;
+            ax.f0 = a0.f0 + f1;
+            ax.nfc = std::min(nsub, a0.nfc - f1);
+            ax.ngroups = (ax.nfc + F - 1) / F;
+            ax.ktab = a0.ktab + (size_t)f1 * nz * a0.nref * a0.nkh;
+            ax.stab = a0.stab + (size_t)f1 * nmz;
+            ax.nshot = 1;
+            ax.igx = p->igx.p;
+            ax.smig = vol.p;
+            b.nfc = ax.nfc;
+            SWEM_TRY(launch_angles(0, ax, az, b, F, st));
+            // z pass: the z line of every (frequency, x)
+            b.inner = nmx, b.lines = ax.nfc * nmx, b.outer = (long long)nmz, b.inner_stride = 1, b.elem = nmx;
+            SWEM_TRY(launch_angles(1, ax, az, b, F, st));
+            // x pass: the x line of every depth
+            b.inner = 1, b.lines = nz, b.outer = nmx, b.inner_stride = 0, b.elem = 1;
+            SWEM_TRY(launch_angles(2, ax, az, b, F, st));
+        }
     }
     std::vector<float> ha(nmz), hs(nmz);
     SWEM_HIP(hipMemcpyAsync(ha.data(), acc.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -1335,27 +1345,42 @@ static int angles3d_impl(swem_plan* p, float sx, float sy, float* angx, float* a
     return SWEM_OK;
 }
 
+int swem_compute_angles_ex(float* angx, float* angy, const float* wav, const swem_geometry* g0, float sx, float sy,
+                           const float* vel, const swem_options* opt)
+{
+    if (!angx || !angy || !wav || !vel || !g0) return set_error(SWEM_EINVAL, "compute_angles: null argument");
+    const swem_geometry& g = *g0;
+    if (g.nmy > 1 && !march3d_supported(g.nmx, g.nmy))
+        return set_error(SWEM_EUNSUPPORTED, "compute_angles: 3-D needs nmx and nmy powers of two 128..1024 (%d x %d)",
+                         g.nmx, g.nmy);
+    if (g.nmx < 2 || !angles_supported(g.nmx) || !angles_supported(g.nz))
+        return set_error(SWEM_EUNSUPPORTED, "compute_angles: nmx = %d, nz = %d: this build handles 2..2048 and powers of "
+                         "two to 4096 (nmx >= 2)", g.nmx, g.nz);
+    swem_geometry ga = g;
+    ga.padt = ga.padx = 1;
+    ga.pspi = 0;
+    ga.nref = 1;
+    ga.gz = 0.f;
+    swem_options o;
+    if (opt) o = *opt;
+    else swem_options_init(&o);
+    swem_plan* p = nullptr;
+    SWEM_TRY(swem_plan_create(&ga, vel, wav, &o, &p));
+    const int rc = angles_impl(p, sx, sy, angx, angy);
+    swem_plan_destroy(p);
+    return rc;
+}
+
 int swem_compute_angles(float* angx, float* angy, const float* wav, int nt, float ot, float dt, int nmx, float omx,
                         float dmx, int nmy, float omy, float dmy, float sx, float sy, int nz, float oz, float dz,
                         float sz, const float* vel, float fmin, float fmax, int verbose)
 {
-    if (!angx || !angy || !wav || !vel) return set_error(SWEM_EINVAL, "compute_angles: null argument");
-    if (nmy > 1 && !march3d_supported(nmx, nmy))
-        return set_error(SWEM_EUNSUPPORTED, "compute_angles: 3-D needs nmx and nmy powers of two 128..1024 (%d x %d)",
-                         nmx, nmy);
-    if (nmx < 2 || !angles_supported(nmx) || !angles_supported(nz))
-        return set_error(SWEM_EUNSUPPORTED, "compute_angles: nmx = %d, nz = %d: this build handles 2..2048 and powers of "
-                         "two to 4096 (nmx >= 2)", nmx, nz);
     swem_geometry g;
     geom_from_args(&g, nt, ot, dt, nmx, omx, dmx, nmy, omy, dmy, nz, oz, dz, 0.f, sz, 1, fmin, fmax, 1, 1, 0);
     swem_options o;
     swem_options_init(&o);
     o.verbose = verbose;
-    swem_plan* p = nullptr;
-    SWEM_TRY(swem_plan_create(&g, vel, wav, &o, &p));
-    const int rc = angles_impl(p, sx, sy, angx, angy);
-    swem_plan_destroy(p);
-    return rc;
+    return swem_compute_angles_ex(angx, angy, wav, &g, sx, sy, vel, &o);
 }
 
 // ------------------------------------------------------------------------ elastic WEM (2-D)

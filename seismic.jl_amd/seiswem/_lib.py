@@ -23,7 +23,7 @@ VARIANT_AUTO, VARIANT_SUBGROUP, VARIANT_WAVE, VARIANT_SUBGROUP16, VARIANT_PAIR, 
 # Every symbol declared in include/seiswem.h.
 EXPORTS = (
     "swem_version", "swem_last_error", "swem_options_init", "swem_device_count",
-    "swem_shot_main", "swem_angles_main", "swem_eshot_main", "swem_wem", "swem_wem_ex", "swem_shots", "swem_compute_angles", "swem_ewem", "swem_eshots",
+    "swem_shot_main", "swem_angles_main", "swem_eshot_main", "swem_wem", "swem_wem_ex", "swem_shots", "swem_compute_angles", "swem_compute_angles_ex", "swem_ewem", "swem_eshots",
     "swem_plan_create", "swem_plan_destroy", "swem_plan_migrate", "swem_plan_migrate_each",
     "swem_plan_model", "swem_plan_timing", "swem_plan_bytes",
     "swem_malloc", "swem_free", "swem_memcpy_htod", "swem_memcpy_dtoh", "swem_memcpy_dtod", "swem_memset", "swem_synchronize",
@@ -152,6 +152,7 @@ def load():
                                    ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                    ctypes.c_float, ctypes.c_float, f32p, f32p, ctypes.c_float, ctypes.c_float,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "swem_compute_angles_ex": ([f32p, f32p, f32p, gp, ctypes.c_float, ctypes.c_float, f32p, op_], ctypes.c_int),
         "swem_eshots": ([gp, f32p, f32p, f32p, f32p, f32p, ctypes.c_int] + [f32p] * 6 + [ctypes.c_int, op_],
                         ctypes.c_int),
         "swem_compute_angles": ([f32p, f32p, f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float,

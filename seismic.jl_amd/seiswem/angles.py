@@ -14,6 +14,8 @@ the reference's file, iang = iangx * nangy + iangy).
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -21,7 +23,7 @@ from ._lib import check, load
 from .wem import DeviceArray, Plan, WemGeometry, _fp
 
 
-def compute_angles(vel, wav, geom: WemGeometry, sx: float, sy: float = 0.0, verbose: bool = False):
+def compute_angles(vel, wav, geom: WemGeometry, sx: float, sy: float = 0.0, verbose: bool = False, **opts):
     """Source incidence angles (degrees) angx, angy [nm, nz] of the shot at (sx, sy): the
     reference's compute_angles with its own sampling (ntfft = nt, nkx = nmx, nky = nmy; padt,
     padx, pspi and nref of `geom` are not used).  angy is zero in 2-D."""
@@ -31,6 +33,12 @@ def compute_angles(vel, wav, geom: WemGeometry, sx: float, sy: float = 0.0, verb
     angy = np.zeros((nm, g.nz), np.float32)
     vel = np.ascontiguousarray(vel, dtype=np.float32)
     wav = np.ascontiguousarray(wav, dtype=np.float32).ravel()[: g.nt]
+    if opts:  # explicit options (device, scratch_bytes, table_bytes ...): swem_compute_angles_ex
+        o = _lib.options(verbose=verbose, **opts)
+        geo = g.c()
+        check(load().swem_compute_angles_ex(_fp(angx), _fp(angy), _fp(wav), ctypes.byref(geo), float(sx), float(sy),
+                                            _fp(vel), ctypes.byref(o)))
+        return angx, angy
     dmy = g.dmy if g.dmy > 0 else g.dmx
     check(load().swem_compute_angles(_fp(angx), _fp(angy), _fp(wav), g.nt, g.ot, g.dt, g.nmx, g.omx, g.dmx, g.nmy,
                                      g.omy, dmy, float(sx), float(sy), g.nz, g.oz, g.dz, g.sz, _fp(vel), g.fmin,

@@ -112,3 +112,19 @@ def test_compute_angles_3d_refused_off_grid(gpu):
     with pytest.raises(seiswem.SwemError) as e:
         seiswem.compute_angles(vel, synth.wavelet_trace(64, 0.004, 20.0), g, 80.0, 80.0)
     assert e.value.code == seiswem.SWEM_EUNSUPPORTED
+
+
+def test_compute_angles_in_small_passes_is_identical(gpu):
+    """A work budget that fits a few frequencies at a time (2-D) or a small chunk (3-D) splits the
+    band into passes; the sums run in the same frequency order, so the angles are bit-identical."""
+    g, vel, wav, _ = case(128, 100, blobs=True, seed=4)
+    a1 = seiswem.compute_angles(vel, wav, g, 640.0)
+    a2 = seiswem.compute_angles(vel, wav, g, 640.0, scratch_bytes=8 * g.nmx * g.nz * 12)
+    assert np.array_equal(a1[0], a2[0]) and np.array_equal(a1[1], a2[1])
+    g3 = seiswem.WemGeometry(nt=64, dt=0.004, nmx=128, dmx=10.0, nmy=128, dmy=10.0, nz=16, dz=10.0, fmin=5.0,
+                             fmax=40.0, pspi=False)
+    vel3 = np.full((g3.nm, g3.nz), 2000.0, np.float32)
+    wav3 = synth.wavelet_trace(g3.nt, g3.dt, 20.0)
+    b1 = seiswem.compute_angles(vel3, wav3, g3, 600.0, 700.0)
+    b2 = seiswem.compute_angles(vel3, wav3, g3, 600.0, 700.0, scratch_bytes=6 * g3.nm * g3.nz * 12 * 2)
+    assert np.array_equal(b1[0], b2[0]) and np.array_equal(b1[1], b2[1])
