@@ -206,6 +206,13 @@ int swem_plan_model(swem_plan* plan, const float* m_dev, const float* sx, const 
                     int nshot, float* d_dev, void* stream);
 /* Timing of the last migrate/model call on this plan (synchronises its stream). */
 int swem_plan_timing(swem_plan* plan, swem_timing* out);
+
+/* Measured kernel choice for a 2-D plan (no reference counterpart): every migration kernel variant
+ * that fits the plan's nkx is timed on nshot shots of zero records (the work does not depend on the
+ * data), and the fastest becomes the plan's variant for later migrations; *best receives it and
+ * times[0..SWEM_VARIANT_PACKED] each variant's march time in ms (0: not a distinct candidate; may
+ * be NULL).  Takes two migrations per candidate; synchronous.  3-D plans: SWEM_EUNSUPPORTED. */
+int swem_plan_autotune(swem_plan* plan, int nshot, int* best, float* times);
 /* Device bytes held by the plan's tables and its largest work chunk. */
 int swem_plan_bytes(const swem_plan* plan, size_t* table_bytes, size_t* work_bytes);
 

@@ -1060,6 +1060,65 @@ int swem_plan_timing(swem_plan* p, swem_timing* out)
     return SWEM_OK;
 }
 
+// Measured choice of the 2-D migration kernel: every variant whose shape fits the plan's nkx is
+// timed on nshot zero records (the march does the same work whatever the data) and the fastest
+// becomes the plan's variant.  times[SWEM_VARIANT_PACKED + 1] receives each variant's march time in
+// ms (0: not a distinct candidate).
+int swem_plan_autotune(swem_plan* p, int nshot, int* best, float* times)
+{
+    if (!p || nshot < 1) return set_error(SWEM_EINVAL, "autotune: null plan or nshot < 1");
+    if (p->three_d) return set_error(SWEM_EUNSUPPORTED, "autotune: 3-D plans have one kernel shape");
+    SWEM_HIP(hipSetDevice(p->opt.device));
+    const size_t nm = p->nm, nt = p->g.nt, nz = p->g.nz;
+    float *d = nullptr, *m = nullptr;
+    SWEM_HIP(hipMalloc((void**)&d, (size_t)nshot * nm * nt * sizeof(float)));
+    hipError_t e = hipMalloc((void**)&m, nm * nz * sizeof(float));
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return hip_error(e, "hipMalloc", __FILE__, __LINE__);
+    }
+    std::vector<float> sx(nshot);
+    for (int i = 0; i < nshot; i++) sx[i] = p->g.omx + p->g.dmx * (float)((i * (p->g.nmx - 1)) / std::max(1, nshot - 1));
+    const swem_options keep = p->opt;
+    const int keepF = p->F;
+    int rc = SWEM_OK;
+    if (hipMemset(d, 0, (size_t)nshot * nm * nt * sizeof(float)) != hipSuccess)
+        rc = set_error(SWEM_EDEVICE, "autotune: memset");
+    int bestv = -1;
+    float bestms = 0.f;
+    int seen[SWEM_VARIANT_PACKED + 1] = {};
+    for (int v = SWEM_VARIANT_SUBGROUP; v <= SWEM_VARIANT_PACKED && !rc; v++) {
+        if (times) times[v] = 0.f;
+        if (v == SWEM_VARIANT_PACKED && !p->packed_ok) continue;
+        const March2DShape sh = march2d_shape(p->nkx, v, 1);
+        if (sh.T <= 0 || seen[sh.variant]) continue;  // not available, or the same kernel as an earlier one
+        seen[sh.variant] = 1;
+        p->opt.variant = v;
+        p->F = choose_group(p, 1, keep.freqs_per_block);
+        if (sh.per_freq * p->F * sh.T > sh.max_block || march2d_lds_bytes(p->nkx, sh, p->F) > 160 * 1024) continue;
+        swem_timing t{};
+        for (int rep = 0; rep < 2 && !rc; rep++) {  // a warm-up, then the timed call
+            rc = migrate_impl(p, d, sx.data(), nullptr, nshot, m, 0, 0, nullptr);
+            if (!rc) rc = swem_plan_timing(p, &t);
+        }
+        if (rc) break;
+        const float ms = (float)t.march_ms;
+        if (times) times[v] = ms;
+        if (bestv < 0 || ms < bestms) bestv = v, bestms = ms;
+    }
+    (void)hipDeviceSynchronize();
+    (void)hipFree(d);
+    (void)hipFree(m);
+    p->opt = keep;
+    p->F = keepF;
+    if (rc) return rc;
+    if (bestv < 0) return set_error(SWEM_EUNSUPPORTED, "autotune: no 2-D kernel fits nkx=%d", p->nkx);
+    p->opt.variant = bestv;
+    p->F = choose_group(p, 1, keep.freqs_per_block);
+    if (best) *best = bestv;
+    return SWEM_OK;
+}
+
 int swem_plan_bytes(const swem_plan* p, size_t* table_bytes, size_t* work_bytes)
 {
     if (!p) return set_error(SWEM_EINVAL, "plan_bytes: null plan");

@@ -25,7 +25,7 @@ EXPORTS = (
     "swem_version", "swem_last_error", "swem_options_init", "swem_device_count",
     "swem_shot_main", "swem_angles_main", "swem_eshot_main", "swem_wem", "swem_wem_ex", "swem_shots", "swem_compute_angles", "swem_compute_angles_ex", "swem_ewem", "swem_eshots",
     "swem_plan_create", "swem_plan_destroy", "swem_plan_migrate", "swem_plan_migrate_each",
-    "swem_plan_model", "swem_plan_timing", "swem_plan_bytes",
+    "swem_plan_model", "swem_plan_timing", "swem_plan_autotune", "swem_plan_bytes",
     "swem_malloc", "swem_free", "swem_memcpy_htod", "swem_memcpy_dtoh", "swem_memcpy_dtod", "swem_memset", "swem_synchronize",
     "swem_sampling", "swem_source_index", "swem_pspi_tables", "swem_taper",
     "swem_seis_extent_init", "swem_seis_read_header", "swem_seis_read", "swem_seis_write",
@@ -166,6 +166,7 @@ def load():
         "swem_plan_migrate_each": ([vp, vp, f32p, f32p, ctypes.c_int, vp, vp], ctypes.c_int),
         "swem_plan_model": ([vp, vp, f32p, f32p, ctypes.c_int, vp, vp], ctypes.c_int),
         "swem_plan_timing": ([vp, P(Timing)], ctypes.c_int),
+        "swem_plan_autotune": ([vp, ctypes.c_int, i32p, f32p], ctypes.c_int),
         "swem_plan_bytes": ([vp, P(ctypes.c_size_t), P(ctypes.c_size_t)], ctypes.c_int),
         "swem_malloc": ([P(vp), ctypes.c_size_t], ctypes.c_int),
         "swem_free": ([vp], ctypes.c_int),

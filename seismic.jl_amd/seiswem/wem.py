@@ -164,6 +164,14 @@ class Plan:
         check(load().swem_plan_timing(self._p, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in _lib.Timing._fields_}
 
+    def autotune(self, nshot: int = 1) -> dict:
+        """Times every 2-D migration kernel variant that fits this plan on nshot zero records and
+        keeps the fastest (swem_plan_autotune).  Returns {"best": variant, "ms": {variant: ms}}."""
+        best = ctypes.c_int(-1)
+        times = (ctypes.c_float * (_lib.VARIANT_PACKED + 1))()
+        check(load().swem_plan_autotune(self._p, int(nshot), ctypes.byref(best), times))
+        return {"best": best.value, "ms": {v: float(times[v]) for v in range(len(times)) if times[v] > 0}}
+
     def bytes(self):
         a, b = ctypes.c_size_t(), ctypes.c_size_t()
         check(load().swem_plan_bytes(self._p, ctypes.byref(a), ctypes.byref(b)))

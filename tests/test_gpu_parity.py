@@ -210,3 +210,19 @@ def test_packed_compat_migration(gpu, case):
     assert rel_l2(m_packed, m_ref) <= TOL
     assert rel_l2(m_packed, m_pair) <= 1e-5
     assert np.array_equal(m_auto, m_packed)
+
+
+@pytest.mark.parametrize("pspi,mode", [(True, 0), (False, 0)])
+def test_autotune_picks_the_fastest_measured_variant_and_keeps_parity(gpu, pspi, mode):
+    """swem_plan_autotune times the distinct kernel variants for the plan's nkx and keeps the
+    fastest; migrations after tuning still match the oracle."""
+    g, vel, wav, rng = small_case(pspi, nx=256, nz=30)
+    plan = seiswem.Plan(g, vel, wav, pspi_mode=mode)
+    r = plan.autotune(nshot=2)
+    print(f"autotune pspi={pspi}: best {r['best']}, ms {r['ms']}")
+    assert r["best"] in r["ms"] and r["ms"][r["best"]] == min(r["ms"].values()) and len(r["ms"]) >= 3
+    sx = 320.0
+    d = rng.standard_normal((1, g.nm, g.nt)).astype(np.float32)
+    m = plan.migrate(seiswem.DeviceArray(d.shape, d), [sx]).download()
+    m_ref = O.wem(oparams(g, mode), vel, wav, sx, d=d[0], adj=True)
+    assert rel_l2(m, m_ref) <= TOL
