@@ -88,14 +88,25 @@ __global__ __launch_bounds__(256) void reduce_image(const float* __restrict__ pa
     const int s1 = per_shot ? s0 + 1 : nshot;
     if (x >= nmx) return;
     float sum = 0.f, comp = 0.f;
+    auto add = [&](float v) {
+        const float y = v - comp;
+        const float t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
+    };
+    const size_t gs = (size_t)nz * nmx;
     for (int s = s0; s < s1; ++s) {
         const float* p = partial + ((size_t)s * ngroups * nz + iz) * nmx + x;
-        for (int g = 0; g < ngroups; ++g) {
-            const float y = p[(size_t)g * nz * nmx] - comp;
-            const float t = sum + y;
-            comp = (t - sum) - y;
-            sum = t;
+        int g = 0;
+        // eight loads in flight ahead of the (serial, fixed-order) compensated sum
+        for (; g + 8 <= ngroups; g += 8) {
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = p[(size_t)(g + k) * gs];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) add(v[k]);
         }
+        for (; g < ngroups; ++g) add(p[(size_t)g * gs]);
     }
     const size_t o = per_shot ? ((size_t)s0 * nmx + x) * nz + iz : (size_t)x * nz + iz;
     out[o] = accumulate ? out[o] + sum : sum;
