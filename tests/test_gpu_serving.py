@@ -13,7 +13,8 @@ pytestmark = pytest.mark.gpu
 
 
 def mixed(n, seed=0):
-    """Small requests of every kind: 2-D SSF, 2-D PSPI (compat and linear), 3-D SSF."""
+    """Small requests of every kind: 2-D SSF, 2-D PSPI (compat and linear; nx 100 takes the
+    Bluestein kernels), 3-D SSF."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -22,7 +23,7 @@ def mixed(n, seed=0):
             g, vel, _, wav, sx, sy = synth.config3(seed=i, nx=64, ny=64, nz=16, nt=128, fmax=20.0)
             mode = seiswem.PSPI_COMPAT
         else:
-            nx = int(rng.choice([64, 128]))
+            nx = int(rng.choice([64, 100, 128]))
             g = seiswem.WemGeometry(nt=256, dt=0.004, nmx=nx, dmx=10.0, nz=30, dz=10.0, fmin=2.0, fmax=40.0, nref=4,
                                     pspi=kind > 0)
             vel = synth.blob_velocity(nx, g.nz, g.dmx, g.dz, rng, nblobs=3, v0=2000.0)
