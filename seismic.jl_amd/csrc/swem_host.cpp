@@ -211,13 +211,19 @@ static int build_tables(swem_plan* p, const float* wav)
     taper_f32(nmx, p->s.lmx, tap.data());
     SWEM_TRY(p->taper.upload(tap.data(), tap.size()));
     if (p->three_d) {
-        const int nky = p->s.nky;
-        std::vector<float2> twy(nky);
-        for (int m = 0; m < nky; m++) {
-            double a = -2.0 * kPi * (double)m / (double)nky;
+        const int nky = p->s.nky, my = p->narr_y;
+        std::vector<float2> twy(my);
+        for (int m = 0; m < my; m++) {
+            double a = -2.0 * kPi * (double)m / (double)my;
             twy[m] = make_float2((float)std::cos(a), (float)std::sin(a));
         }
         SWEM_TRY(p->twy.upload(twy.data(), twy.size()));
+        if (my != nky) {
+            std::vector<float2> chirp, bchirp;
+            bluestein_tables(nky, my, chirp, bchirp);
+            SWEM_TRY(p->chirp_y.upload(chirp.data(), chirp.size()));
+            SWEM_TRY(p->bchirp_y.upload(bchirp.data(), bchirp.size()));
+        }
         std::vector<float> tapy(g.nmy);
         taper_f32(g.nmy, p->s.lmy, tapy.data());
         SWEM_TRY(p->taper_y.upload(tapy.data(), tapy.size()));
@@ -583,6 +589,10 @@ static March3DArgs base3(const swem_plan* p)
     a.taper_y = p->taper_y.p;
     a.twx = p->tw.p;
     a.twy = p->twy.p;
+    a.chx = p->chirp.p;
+    a.bchx = p->bchirp.p;
+    a.chy = p->chirp_y.p;
+    a.bchy = p->bchirp_y.p;
     a.spec_s = p->spec_s.p;
     a.nref = p->g.pspi ? p->g.nref : 1;
     a.sel = p->sel.p;
@@ -892,8 +902,10 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
         if (g->pspi && o.pspi_mode != SWEM_PSPI_COMPAT)
             return set_error(SWEM_EUNSUPPORTED, "3-D PSPI: the linear mode is not available in this build (compat is)");
         if (g->nmx < 2 || !march3d_supported(p->s.nkx, p->s.nky))
-            return set_error(SWEM_EUNSUPPORTED, "3-D: nkx=%d, nky=%d: this build handles powers of two 128..1024", p->s.nkx,
-                             p->s.nky);
+            return set_error(SWEM_EUNSUPPORTED, "3-D: nkx=%d, nky=%d: this build handles 2..512 and powers of two to 1024",
+                             p->s.nkx, p->s.nky);
+        p->narr = march3d_len(p->s.nkx);
+        p->narr_y = march3d_len(p->s.nky);
     } else if (!march2d_supported(p->nkx)) {
         return set_error(SWEM_EUNSUPPORTED, "nkx = padx*nmx = %d: this build handles 2..2048 and powers of two to 4096",
                          p->nkx);
@@ -940,7 +952,7 @@ void plan_free(swem_plan* p)
     for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
     if (p->igx_pinned) (void)hipHostFree(p->igx_pinned);
     p->twy.release(); p->taper_y.release(); p->W.release(); p->wavd.release();
-    p->tw.release(); p->chirp.release(); p->bchirp.release(); p->spec_s.release(); p->ktab.release(); p->stab.release(); p->pd.release(); p->pdf.release();
+    p->tw.release(); p->chirp.release(); p->bchirp.release(); p->chirp_y.release(); p->bchirp_y.release(); p->spec_s.release(); p->ktab.release(); p->stab.release(); p->pd.release(); p->pdf.release();
     p->taper.release(); p->p2f.release(); p->w2.release();
     p->own_bases.adj.release(); p->own_bases.fwd.release(); p->own_bases.src.release();
     p->p2.release(); p->sel.release(); p->sel2.release();
@@ -969,6 +981,7 @@ size_t plan_upload_bytes(const swem_plan* p, int nshot)
     else
         b += pad256(nz * nm * 8) + pad256(nz * nm * 4);
     if (p->narr != p->nkx) b += pad256(p->nkx * 8) + pad256(p->narr * 16);
+    if (p->three_d && p->narr_y != p->s.nky) b += pad256(p->narr_y * 8) + pad256(p->s.nky * 8) + pad256(p->narr_y * 16);
     return b + pad256((size_t)nshot * 4);
 }
 
@@ -1126,7 +1139,8 @@ int swem_plan_bytes(const swem_plan* p, size_t* table_bytes, size_t* work_bytes)
         *table_bytes = p->tw.bytes() + p->spec_s.bytes() + p->ktab.bytes() + p->stab.bytes() + p->pd.bytes() +
                        p->pdf.bytes() + p->taper.bytes() + p->p2f.bytes() +
                        p->w2.bytes() + p->own_bases.adj.bytes() + p->own_bases.fwd.bytes() + p->own_bases.src.bytes() + p->wavd.bytes() + p->p2.bytes() + p->sel.bytes() +
-                       p->sel2.bytes() + p->used.bytes() + p->chirp.bytes() + p->bchirp.bytes();
+                       p->sel2.bytes() + p->used.bytes() + p->chirp.bytes() + p->bchirp.bytes() +
+                       p->twy.bytes() + p->chirp_y.bytes() + p->bchirp_y.bytes() + p->taper_y.bytes();
     if (work_bytes)
         *work_bytes = p->spec_r.bytes() + p->smig.bytes() + p->dgw.bytes() + p->partial.bytes() + p->mT.bytes() +
                       p->igx.bytes();
@@ -1409,7 +1423,7 @@ int swem_compute_angles_ex(float* angx, float* angy, const float* wav, const swe
 {
     if (!angx || !angy || !wav || !vel || !g0) return set_error(SWEM_EINVAL, "compute_angles: null argument");
     const swem_geometry& g = *g0;
-    if (g.nmy > 1 && !march3d_supported(g.nmx, g.nmy))
+    if (g.nmy > 1 && !(march3d_len(g.nmx) == g.nmx && march3d_len(g.nmy) == g.nmy))
         return set_error(SWEM_EUNSUPPORTED, "compute_angles: 3-D needs nmx and nmy powers of two 128..1024 (%d x %d)",
                          g.nmx, g.nmy);
     if (g.nmx < 2 || !angles_supported(g.nmx) || !angles_supported(g.nz))

@@ -168,8 +168,11 @@ struct March3DArgs {
     const float* pdf;           // [nz][nm] float counterpart
     const float* taper_x;       // [nmx]
     const float* taper_y;       // [nmy]
-    const float2* twx;          // [nkx]
-    const float2* twy;          // [nky]
+    const float2* twx;          // [march3d_len(nkx)] twiddles of the x array length
+    const float2* twy;          // [march3d_len(nky)]
+    // Bluestein tables of an axis whose array length differs from its nk: [nk] chirp
+    // exp(-i pi j^2 / nk), [2][array length] kernel spectra (transform sign -1, +1)
+    const float2 *chx, *bchx, *chy, *bchy;
     const float2* spec_s;       // [nf]
     const float2* spec_r;       // [nm][nf] this shot's receiver spectra
     float2* W;                  // [nitem][2][nmx][nky] step state (y transformed)
@@ -185,6 +188,7 @@ struct March3DArgs {
     const uint32_t* used;
     float2* R;
 };
+int march3d_len(int n);
 bool march3d_supported(int nkx, int nky);
 // which: 0 init, 1 x-pass, 2 migration y-pass, 3 modelling y-pass (phase 0 source,
 // 1 receiver, 2 output), 4 PSPI x-pass, 5 PSPI y-pass (phase 0 migration, 1 modelling source,

@@ -19,7 +19,8 @@
 // 332, 362-369) taper -> SS -> FFT2 -> L -> IFFT2 as  IFFTy inject taper SS FFTy | FFTx L IFFTx.
 // An item is one frequency of one shot; a launch covers every item of the frequency chunk.
 // Transforms are wave-level (one 64-lane wave per row / column, N/64 points per lane, no
-// workgroup barrier inside a transform).
+// workgroup barrier inside a transform).  An axis whose length n is not a power of two 128..1024
+// is transformed through Bluestein's identity on an array of N >= 2n - 1 points (line_fft).
 #include "swem_fft.hpp"
 #include "swem_internal.hpp"
 
@@ -93,12 +94,69 @@ __device__ __forceinline__ float2* state(const March3DArgs& a, int item, int wf)
     return a.W + ((size_t)item * 2 + wf) * (size_t)a.nmx * a.nky;
 }
 
+// Point i of a y line is stored (lines hold nky points; a Bluestein array is longer).
+template <bool BY>
+__device__ __forceinline__ bool yin(const March3DArgs& a, int i)
+{
+    return !BY || i < a.nky;
+}
+
+// Transform of one line of n points held by a wave as N array points (lane t, slot q: point
+// t + 64 q).  BLUE = false: n = N, the wave FFT.  BLUE = true: Bluestein's chirp-z identity on the
+// power-of-two array length N >= 2n - 1, X_k = c_k IFFT_N(FFT_N(c x) B)_k with the chirp
+// c_j = exp(SIGN i pi j^2 / n) and B the spectrum of the wrapped conjugate chirp over N, built
+// in FP64 on the host (c = 0 for j >= n, so points beyond n enter and leave as zero).
+template <int N, int SIGN, bool BLUE>
+__device__ __forceinline__ void line_fft(float2 (&v)[N / 64], float2* buf, const float2* __restrict__ tw,
+                                         const float2* __restrict__ chirp, const float2* __restrict__ bch, int n, int t)
+{
+    constexpr int P = N / 64;
+    if constexpr (!BLUE) {
+        wfft<N, SIGN>(v, buf, tw, t);
+    } else {
+        auto chirp_mul = [&]() {
+            float2 c[P];
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int j = t + 64 * q;
+                const float2 cj = j < n ? chirp[j] : make_float2(0.f, 0.f);  // exp(-i pi j^2 / n)
+                c[q] = SIGN < 0 ? cj : make_float2(cj.x, -cj.y);
+            }
+#pragma unroll
+            for (int q = 0; q < P; ++q) v[q] = cmul(v[q], c[q]);
+        };
+        chirp_mul();
+        wfft<N, -1>(v, buf, tw, t);
+        const float2* B = bch + (SIGN < 0 ? 0 : N);
+        float2 bq[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) bq[q] = B[t + 64 * q];
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = cmul(v[q], bq[q]);
+        wave_sync_lds();
+        wfft<N, +1>(v, buf, tw, t);
+        chirp_mul();
+    }
+}
+
+template <int N, int SIGN, bool BX>
+__device__ __forceinline__ void xfft(float2 (&v)[N / 64], float2* buf, const March3DArgs& a, int t)
+{
+    line_fft<N, SIGN, BX>(v, buf, a.twx, a.chx, a.bchx, a.nkx, t);
+}
+
+template <int N, int SIGN, bool BY>
+__device__ __forceinline__ void yfft(float2 (&v)[N / 64], float2* buf, const March3DArgs& a, int t)
+{
+    line_fft<N, SIGN, BY>(v, buf, a.twy, a.chy, a.bchy, a.nky, t);
+}
+
 }  // namespace
 
 // Initial wavefields of the chunk (adjoint: source delta at the shot position and receiver
 // spectra; forward: source delta, receiver zero), forward y-FFT, stored as the step state.
 // Workgroup: WAVES rows (one wave each); grid: nitem * 2 * nmx rows.
-template <int NY>
+template <int NY, bool BY>
 __global__ __launch_bounds__(256) void march3d_init(March3DArgs a)
 {
     constexpr int P = NY / 64;
@@ -126,10 +184,10 @@ __global__ __launch_bounds__(256) void march3d_init(March3DArgs a)
         }
         v[q] = u;
     }
-    wfft<NY, -1>(v, lds + w * L, a.twy, t);
+    yfft<NY, -1, BY>(v, lds + w * L, a, t);
     float2* dst = state(a, item, wf) + (size_t)x * a.nky;
 #pragma unroll
-    for (int q = 0; q < P; ++q) dst[t + 64 * q] = v[q];
+    for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) dst[t + 64 * q] = v[q];
 }
 
 // kz phase of the KZ64 path with the column's part hoisted: c0 = w^2 po^2 - ky^2 (FP64, once per
@@ -153,13 +211,13 @@ __device__ __forceinline__ float2 kz_phase64(const March3DArgs& a, double c0, in
 // of C complex.  conj_src / conj_rec: use conj(L) (-dz).  Workgroups are renumbered so that each
 // XCD (consecutive workgroup ids go round robin over the 8 XCDs) takes a contiguous run of tiles:
 // neighbouring tiles share 128-byte lines, which then meet in one L2.
-template <int NX, int NWF, int C>
+template <int NX, int NWF, int C, bool BX>
 __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 1024 ? 2 : 4))) void march3d_cols(March3DArgs a)
 {
     constexpr int P = NX / 64;
     constexpr int L = LdsLen<NX>::value;
     extern __shared__ float2 smem[];
-    const int tiles = a.nky / C;
+    const int tiles = (a.nky + C - 1) / C;  // the last tile is partial when C does not divide nky
     const int nblk = a.nitem * tiles;
     const int full = nblk & ~7;
     int b = blockIdx.x;
@@ -172,6 +230,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
     // stage: buffer (c * NWF + k) holds column ky0 + c of the k-th stepped wavefield.  The loads of
     // one wavefield are all issued before its LDS writes (rows x0 + 64 j, j < P cover x < nmx).
     const int c = threadIdx.x % C, x0 = threadIdx.x / C;
+    const bool kin = ky0 + c < a.nky;
 #pragma unroll
     for (int k = 0; k < NWF; ++k) {
         const float2* src = state(a, item, wf_of(k)) + ky0 + c;
@@ -180,7 +239,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             const int x = x0 + 64 * j;
-            st[j] = x < a.nmx ? src[(size_t)x * a.nky] : make_float2(0.f, 0.f);
+            st[j] = x < a.nmx && kin ? src[(size_t)x * a.nky] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int j = 0; j < P; ++j) {
@@ -199,7 +258,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
             v[k][q] = x < a.nmx ? buf[lds_pad(x)] : make_float2(0.f, 0.f);
         }
         wave_sync_lds();
-        wfft<NX, -1>(v[k], buf, a.twx, t);
+        xfft<NX, -1, BX>(v[k], buf, a, t);
     }
     const int iw = a.ifmin + a.f0 + item;
     const double wd = (double)iw * a.dw;
@@ -221,7 +280,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
     for (int k = 0; k < NWF; ++k) {
         float2* buf = smem + (w * NWF + k) * L;
         wave_sync_lds();
-        wfft<NX, +1>(v[k], buf, a.twx, t);
+        xfft<NX, +1, BX>(v[k], buf, a, t);
         wave_sync_lds();
 #pragma unroll
         for (int q = 0; q < P; ++q) {
@@ -240,7 +299,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             const int x = x1 + 64 * j;
-            if (x < a.nmx) dst[(size_t)x * a.nky] = src[lds_pad(x)];
+            if (x < a.nmx && ky0 + c1 < a.nky) dst[(size_t)x * a.nky] = src[lds_pad(x)];
         }
     }
 }
@@ -259,14 +318,14 @@ __device__ __forceinline__ float2* rslice(const March3DArgs& a, int item, int wf
 
 // x-pass: as march3d_cols up to the forward x-FFT, then per vref in use: kz phase of
 // vref[ir][iz] (conj for a -dz step), inverse x-FFT, slice tile written through LDS.
-template <int NX, int NWF, int C>
+template <int NX, int NWF, int C, bool BX>
 __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 512 ? 2 : 4))) void march3d_cols_pspi(
     March3DArgs a)
 {
     constexpr int P = NX / 64;
     constexpr int L = LdsLen<NX>::value;
     extern __shared__ float2 smem[];
-    const int tiles = a.nky / C;
+    const int tiles = (a.nky + C - 1) / C;  // the last tile is partial when C does not divide nky
     const int nblk = a.nitem * tiles;
     const int full = nblk & ~7;
     int b = blockIdx.x;
@@ -277,6 +336,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 51
     const int wf1 = (a.wmask & 1) ? 0 : 1;
     auto wf_of = [&](int k) { return NWF == 2 ? k : wf1; };
     const int c = threadIdx.x % C, x0 = threadIdx.x / C;
+    const bool kin = ky0 + c < a.nky;
 #pragma unroll
     for (int k = 0; k < NWF; ++k) {
         const float2* src = state(a, item, wf_of(k)) + ky0 + c;
@@ -285,7 +345,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 51
 #pragma unroll
         for (int j = 0; j < P; ++j) {
             const int x = x0 + 64 * j;
-            st[j] = x < a.nmx ? src[(size_t)x * a.nky] : make_float2(0.f, 0.f);
+            st[j] = x < a.nmx && kin ? src[(size_t)x * a.nky] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int j = 0; j < P; ++j) {
@@ -304,7 +364,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 51
             v[k][q] = x < a.nmx ? buf[lds_pad(x)] : make_float2(0.f, 0.f);
         }
         wave_sync_lds();
-        wfft<NX, -1>(v[k], buf, a.twx, t);
+        xfft<NX, -1, BX>(v[k], buf, a, t);
     }
     const int iw = a.ifmin + a.f0 + item;
     const double wd = (double)iw * a.dw;
@@ -333,7 +393,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 51
             }
             float2* buf = smem + (w * NWF + k) * L;
             wave_sync_lds();
-            wfft<NX, +1>(bq, buf, a.twx, t);
+            xfft<NX, +1, BX>(bq, buf, a, t);
             wave_sync_lds();
 #pragma unroll
             for (int q = 0; q < P; ++q) {
@@ -349,7 +409,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 51
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 const int x = x1 + 64 * j;
-                if (x < a.nmx) dst[(size_t)x * a.nky] = src[lds_pad(x)];
+                if (x < a.nmx && ky0 + c1 < a.nky) dst[(size_t)x * a.nky] = src[lds_pad(x)];
             }
         }
         __syncthreads();  // the tile is rewritten by the next vref
@@ -358,7 +418,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 51
 
 // u = the pspiop selection of row x of wavefield wf from the slices of depth iz_sel: inverse
 // y-FFT of each slice in use, per point the slice of its iref1, real part where cut.
-template <int NY>
+template <int NY, bool BY>
 __device__ __forceinline__ void pspi_select_row(const March3DArgs& a, int item, int wf, int x, float2 (&u)[NY / 64],
                                                 float2* buf, int t)
 {
@@ -377,8 +437,8 @@ __device__ __forceinline__ void pspi_select_row(const March3DArgs& a, int item, 
         const float2* row = rslice(a, item, wf, ir) + (size_t)x * a.nky;
         float2 bq[P];
 #pragma unroll
-        for (int q = 0; q < P; ++q) bq[q] = row[t + 64 * q];
-        wfft<NY, +1>(bq, buf, a.twy, t);
+        for (int q = 0; q < P; ++q) bq[q] = yin<BY>(a, t + 64 * q) ? row[t + 64 * q] : make_float2(0.f, 0.f);
+        yfft<NY, +1, BY>(bq, buf, a, t);
         wave_sync_lds();
 #pragma unroll
         for (int q = 0; q < P; ++q)
@@ -395,7 +455,7 @@ __device__ __forceinline__ void pspi_select_row(const March3DArgs& a, int item, 
 //   phase 2 modelling receiver: select the previous step's result (zero at the first), inject
 //           m smig of this depth, forward y-FFT
 //   phase 3 modelling output: select the last step's result (zero when no step ran), spectra out
-template <int NY>
+template <int NY, bool BY>
 __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, int phase)
 {
     constexpr int P = NY / 64;
@@ -412,7 +472,7 @@ __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, i
         const int fl = a.f0 + item;
         float2 u[P];
         if (phase == 1) {
-            if (a.ds) pspi_select_row<NY>(a, item, 0, x, u, buf, t);
+            if (a.ds) pspi_select_row<NY, BY>(a, item, 0, x, u, buf, t);
             float2* sm = a.smig + ((size_t)item * a.nz + a.iz) * a.nm + (size_t)x * a.nmy;
 #pragma unroll
             for (int q = 0; q < P; ++q) {
@@ -425,7 +485,7 @@ __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, i
 #pragma unroll
                 for (int q = 0; q < P; ++q) u[q] = make_float2(0.f, 0.f);
             } else {
-                pspi_select_row<NY>(a, item, 1, x, u, buf, t);
+                pspi_select_row<NY, BY>(a, item, 1, x, u, buf, t);
             }
             if (phase == 3) {
 #pragma unroll
@@ -444,10 +504,10 @@ __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, i
             }
         }
         wave_sync_lds();
-        wfft<NY, -1>(u, buf, a.twy, t);
+        yfft<NY, -1, BY>(u, buf, a, t);
         float2* dst = state(a, item, phase == 1 ? 0 : 1) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) dst[t + 64 * q] = u[q];
+        for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) dst[t + 64 * q] = u[q];
         return;
     }
     // phase 0: migration
@@ -456,19 +516,19 @@ __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, i
 #pragma unroll
     for (int q = 0; q < P; ++q) vs[q] = vr[q] = make_float2(0.f, 0.f);
     if (active && a.ds) {
-        pspi_select_row<NY>(a, item, 0, x, vs, buf, t);
+        pspi_select_row<NY, BY>(a, item, 0, x, vs, buf, t);
     } else if (active && a.dr) {
         // an unstepped source: its state is FFT_y(u), so the inverse transform returns nky u
         const float2* src = state(a, item, 0) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) vs[q] = src[t + 64 * q];
-        wfft<NY, +1>(vs, buf, a.twy, t);
+        for (int q = 0; q < P; ++q) vs[q] = yin<BY>(a, t + 64 * q) ? src[t + 64 * q] : make_float2(0.f, 0.f);
+        yfft<NY, +1, BY>(vs, buf, a, t);
         wave_sync_lds();
-        const float rn = 1.f / (float)a.nky;  // exact: nky is a power of two
+        const float rn = 1.f / (float)a.nky;  // exact when nky is a power of two
 #pragma unroll
         for (int q = 0; q < P; ++q) vs[q] = cscale(vs[q], rn);
     }
-    if (active && a.dr) pspi_select_row<NY>(a, item, 1, x, vr, buf, t);
+    if (active && a.dr) pspi_select_row<NY, BY>(a, item, 1, x, vr, buf, t);
     float cc[P];
     const float factor = iw == 0 ? 1.f : 2.f;
 #pragma unroll
@@ -489,17 +549,17 @@ __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, i
     }
     __syncthreads();
     if (active && a.ds && a.next_s) {
-        wfft<NY, -1>(vs, buf, a.twy, t);
+        yfft<NY, -1, BY>(vs, buf, a, t);
         float2* dst = state(a, item, 0) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) dst[t + 64 * q] = vs[q];
+        for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) dst[t + 64 * q] = vs[q];
         wave_sync_lds();
     }
     if (active && a.dr && a.next_r) {
-        wfft<NY, -1>(vr, buf, a.twy, t);
+        yfft<NY, -1, BY>(vr, buf, a, t);
         float2* dst = state(a, item, 1) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) dst[t + 64 * q] = vr[q];
+        for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) dst[t + 64 * q] = vr[q];
     }
 }
 
@@ -507,7 +567,7 @@ __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, i
 // of frequency item g F + f.  ds / dr: the source / receiver was stepped at this depth; next_s /
 // next_r: it is stepped at the next depth (its forward y-FFT is needed); the imaging runs when dr.
 // The split-step phase is computed once and applied conjugated to the source.
-template <int NY>
+template <int NY, bool BY>
 __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
 {
     constexpr int P = NY / 64;
@@ -529,23 +589,23 @@ __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
     if (need_s) {
         const float2* src = state(a, item, 0) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) vs[q] = src[t + 64 * q];
+        for (int q = 0; q < P; ++q) vs[q] = yin<BY>(a, t + 64 * q) ? src[t + 64 * q] : make_float2(0.f, 0.f);
     }
     if (need_r) {
         const float2* src = state(a, item, 1) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) vr[q] = src[t + 64 * q];
+        for (int q = 0; q < P; ++q) vr[q] = yin<BY>(a, t + 64 * q) ? src[t + 64 * q] : make_float2(0.f, 0.f);
     }
     if (need_s) {
-        wfft<NY, +1>(vs, buf, a.twy, t);
+        yfft<NY, +1, BY>(vs, buf, a, t);
         wave_sync_lds();
         if (!a.ds) {  // unstepped source: its state is FFT_y(u), the inverse returns nky u
-            const float rn = 1.f / (float)a.nky;  // exact: nky is a power of two
+            const float rn = 1.f / (float)a.nky;  // exact when nky is a power of two
 #pragma unroll
             for (int q = 0; q < P; ++q) vs[q] = cscale(vs[q], rn);
         }
     }
-    if (need_r) wfft<NY, +1>(vr, buf, a.twy, t);
+    if (need_r) yfft<NY, +1, BY>(vr, buf, a, t);
     float cc[P];
     const float factor = iw == 0 ? 1.f : 2.f;
     const float tx = a.taper_x[x];
@@ -579,17 +639,17 @@ __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
     __syncthreads();
     // forward y-FFTs that start the next step
     if (active && a.ds && a.next_s) {
-        wfft<NY, -1>(vs, buf, a.twy, t);
+        yfft<NY, -1, BY>(vs, buf, a, t);
         float2* dst = state(a, item, 0) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) dst[t + 64 * q] = vs[q];
+        for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) dst[t + 64 * q] = vs[q];
         wave_sync_lds();
     }
     if (active && a.dr && a.next_r) {
-        wfft<NY, -1>(vr, buf, a.twy, t);
+        yfft<NY, -1, BY>(vr, buf, a, t);
         float2* dst = state(a, item, 1) + (size_t)x * a.nky;
 #pragma unroll
-        for (int q = 0; q < P; ++q) dst[t + 64 * q] = vr[q];
+        for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) dst[t + 64 * q] = vr[q];
     }
 }
 
@@ -603,7 +663,7 @@ __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
 //           receiver's step; at the first call the state is the source trace, later IFFTy of
 //           the previous step's result is stored in smig at depth iz_sel; with ds the next step
 //           starts (taper, conj(SS), FFTy).
-template <int NY>
+template <int NY, bool BY>
 __global__ __launch_bounds__(1024) void march3d_rows_fwd(March3DArgs a, int F, int phase)
 {
     constexpr int P = NY / 64;
@@ -630,8 +690,8 @@ __global__ __launch_bounds__(1024) void march3d_rows_fwd(March3DArgs a, int F, i
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < P; ++q) v[q] = row[t + 64 * q];
-            wfft<NY, +1>(v, buf, a.twy, t);
+            for (int q = 0; q < P; ++q) v[q] = yin<BY>(a, t + 64 * q) ? row[t + 64 * q] : make_float2(0.f, 0.f);
+            yfft<NY, +1, BY>(v, buf, a, t);
             float2* sm = a.smig + ((size_t)item * a.nz + a.iz_sel) * a.nm + (size_t)x * a.nmy;
 #pragma unroll
             for (int q = 0; q < P; ++q) {
@@ -647,9 +707,9 @@ __global__ __launch_bounds__(1024) void march3d_rows_fwd(March3DArgs a, int F, i
                              : make_float2(0.f, 0.f);
         }
         wave_sync_lds();
-        wfft<NY, -1>(v, buf, a.twy, t);
+        yfft<NY, -1, BY>(v, buf, a, t);
 #pragma unroll
-        for (int q = 0; q < P; ++q) row[t + 64 * q] = v[q];
+        for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) row[t + 64 * q] = v[q];
         return;
     }
     if (phase == 1 && a.first) {
@@ -657,8 +717,8 @@ __global__ __launch_bounds__(1024) void march3d_rows_fwd(March3DArgs a, int F, i
         for (int q = 0; q < P; ++q) v[q] = make_float2(0.f, 0.f);
     } else {
 #pragma unroll
-        for (int q = 0; q < P; ++q) v[q] = row[t + 64 * q];
-        wfft<NY, +1>(v, buf, a.twy, t);
+        for (int q = 0; q < P; ++q) v[q] = yin<BY>(a, t + 64 * q) ? row[t + 64 * q] : make_float2(0.f, 0.f);
+        yfft<NY, +1, BY>(v, buf, a, t);
     }
     const float tx = a.taper_x[x];
     float2* sm = a.smig + ((size_t)item * a.nz + a.iz) * a.nm + (size_t)x * a.nmy;
@@ -688,18 +748,25 @@ __global__ __launch_bounds__(1024) void march3d_rows_fwd(March3DArgs a, int F, i
     }
     if (phase == 0 && !(a.ds && a.next_s)) return;  // state unchanged / no further step
     wave_sync_lds();
-    wfft<NY, -1>(v, buf, a.twy, t);
+    yfft<NY, -1, BY>(v, buf, a, t);
 #pragma unroll
-    for (int q = 0; q < P; ++q) row[t + 64 * q] = v[q];
+    for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) row[t + 64 * q] = v[q];
 }
 
 // ------------------------------------------------------------------------ dispatch
 
-bool march3d_supported(int nkx, int nky)
+// Array length of a 3-D line transform of n points: n for powers of two 128..1024, else the
+// Bluestein length (the power of two >= max(128, 2n - 1), up to 1024: n <= 512); 0 if neither.
+int march3d_len(int n)
 {
-    auto ok = [](int n) { return n >= 128 && n <= 1024 && (n & (n - 1)) == 0; };
-    return ok(nkx) && ok(nky);
+    if (n >= 128 && n <= 1024 && (n & (n - 1)) == 0) return n;
+    if (n < 2 || n > 512) return 0;
+    int m = 128;
+    while (m < 2 * n - 1) m <<= 1;
+    return m;
 }
+
+bool march3d_supported(int nkx, int nky) { return march3d_len(nkx) > 0 && march3d_len(nky) > 0; }
 
 // Columns (waves) per x-pass workgroup: 16 buffers of staging per workgroup (8 columns of two
 // wavefields or 16 of one) below nkx = 1024, half that at 1024.
@@ -715,13 +782,13 @@ static int launch3(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, 
     return SWEM_OK;
 }
 
-template <int N>
+template <int N, bool BL>
 static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStream_t st)
 {
     constexpr int L = LdsLen<N>::value;
     if (which == 0) {  // init
         const size_t rows = (size_t)a.nitem * 2 * a.nmx;
-        hipLaunchKernelGGL(march3d_init<N>, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((march3d_init<N, BL>), dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st, a);
         SWEM_HIP(hipGetLastError());
         return SWEM_OK;
     }
@@ -729,33 +796,33 @@ static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStrea
         const int nwf = (a.wmask & 1) + ((a.wmask >> 1) & 1);
         const int C = march3d_cols_tile(nwf, N);
         const size_t lds = (size_t)C * nwf * L * sizeof(float2);
-        const dim3 grid(a.nitem * (a.nky / C)), block(64 * C);
+        const dim3 grid(a.nitem * ((a.nky + C - 1) / C)), block(64 * C);
         constexpr int C2 = N >= 1024 ? 4 : 8, C1 = 2 * C2;
-        return nwf == 2 ? launch3(march3d_cols<N, 2, C2>, grid, block, lds, st, a)
-                        : launch3(march3d_cols<N, 1, C1>, grid, block, lds, st, a);
+        return nwf == 2 ? launch3(march3d_cols<N, 2, C2, BL>, grid, block, lds, st, a)
+                        : launch3(march3d_cols<N, 1, C1, BL>, grid, block, lds, st, a);
     }
     if (which == 4) {  // PSPI x-pass
         const int nwf = (a.wmask & 1) + ((a.wmask >> 1) & 1);
         const int C = march3d_cols_tile(nwf, N);
         const size_t lds = (size_t)C * nwf * L * sizeof(float2);
-        const dim3 grid(a.nitem * (a.nky / C)), block(64 * C);
+        const dim3 grid(a.nitem * ((a.nky + C - 1) / C)), block(64 * C);
         constexpr int C2 = N >= 1024 ? 4 : 8, C1 = 2 * C2;
-        return nwf == 2 ? launch3(march3d_cols_pspi<N, 2, C2>, grid, block, lds, st, a)
-                        : launch3(march3d_cols_pspi<N, 1, C1>, grid, block, lds, st, a);
+        return nwf == 2 ? launch3(march3d_cols_pspi<N, 2, C2, BL>, grid, block, lds, st, a)
+                        : launch3(march3d_cols_pspi<N, 1, C1, BL>, grid, block, lds, st, a);
     }
     const int ngroups = (a.nitem + F - 1) / F;
     if (which == 5) {
         const size_t lds = (size_t)F * L * sizeof(float2);
-        return launch3(march3d_rows_pspi<N>, dim3(ngroups * a.nmx), dim3(64 * F), lds, st, a, F, phase);
+        return launch3(march3d_rows_pspi<N, BL>, dim3(ngroups * a.nmx), dim3(64 * F), lds, st, a, F, phase);
     }
     if (which == 2) {
         const size_t lds = (size_t)F * L * sizeof(float2);
-        return launch3(march3d_rows_adj<N>, dim3(ngroups * a.nmx), dim3(64 * F), lds, st, a, F);
+        return launch3(march3d_rows_adj<N, BL>, dim3(ngroups * a.nmx), dim3(64 * F), lds, st, a, F);
     }
     const size_t lds = (size_t)F * L * sizeof(float2);
-    SWEM_HIP(hipFuncSetAttribute((const void*)march3d_rows_fwd<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    SWEM_HIP(hipFuncSetAttribute((const void*)march3d_rows_fwd<N, BL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
-    hipLaunchKernelGGL(march3d_rows_fwd<N>, dim3(ngroups * a.nmx), dim3(64 * F), lds, st, a, F, phase);
+    hipLaunchKernelGGL((march3d_rows_fwd<N, BL>), dim3(ngroups * a.nmx), dim3(64 * F), lds, st, a, F, phase);
     SWEM_HIP(hipGetLastError());
     return SWEM_OK;
 }
@@ -764,18 +831,22 @@ static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStrea
 // (phase 0 source, 1 receiver, 2 output).  F: frequencies per y-pass workgroup.
 int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_t st)
 {
-    const int n = (which == 1 || which == 4) ? a.nkx : a.nky;
+    const bool xpass = which == 1 || which == 4;
+    const int n = xpass ? a.nkx : a.nky, m = march3d_len(n);
     if (!march3d_supported(a.nkx, a.nky) || a.nmx > a.nkx || a.nmy > a.nky || a.nitem < 1)
         return set_error(SWEM_EINVAL, "3-D march: inconsistent launch (nkx=%d nky=%d)", a.nkx, a.nky);
+    if (m != n && (xpass ? !(a.chx && a.bchx) : !(a.chy && a.bchy)))
+        return set_error(SWEM_EINVAL, "3-D march: Bluestein tables missing (n=%d)", n);
     if (((which == 2 || which == 3) && 64 * F > 1024) || (which == 5 && 64 * F > 512) || F < 1)
         return set_error(SWEM_EINVAL, "3-D march: %d frequencies per workgroup", F);
     if ((which == 4 || which == 5) && (!a.R || !a.sel || !a.used || a.nref < 1 || a.nref > 31))
         return set_error(SWEM_EINVAL, "3-D PSPI march: tables missing");
-    switch (n) {
-    case 128: return launch3_n<128>(which, a, F, phase, st);
-    case 256: return launch3_n<256>(which, a, F, phase, st);
-    case 512: return launch3_n<512>(which, a, F, phase, st);
-    case 1024: return launch3_n<1024>(which, a, F, phase, st);
+    const bool blue = m != n;
+    switch (m) {
+    case 128: return blue ? launch3_n<128, true>(which, a, F, phase, st) : launch3_n<128, false>(which, a, F, phase, st);
+    case 256: return blue ? launch3_n<256, true>(which, a, F, phase, st) : launch3_n<256, false>(which, a, F, phase, st);
+    case 512: return blue ? launch3_n<512, true>(which, a, F, phase, st) : launch3_n<512, false>(which, a, F, phase, st);
+    case 1024: return blue ? launch3_n<1024, true>(which, a, F, phase, st) : launch3_n<1024, false>(which, a, F, phase, st);
     }
     return set_error(SWEM_EUNSUPPORTED, "3-D march: n=%d", n);
 }

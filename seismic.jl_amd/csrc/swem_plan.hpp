@@ -83,7 +83,8 @@ struct swem_plan {
     swem_options opt{};
     swem::Sampling s{};
     int nm = 0, nf = 0, nkx = 0, T = 0, F = 1, Ffwd = 1, mode = 0;
-    int narr = 0;  // 2-D transform array length: nkx, or the Bluestein length for other nkx
+    int narr = 0;  // x transform array length: nkx, or the Bluestein length for other nkx
+    int narr_y = 0;  // 3-D y transform array length: nky, or its Bluestein length
     // frequency chunks: [c * nfc, min(nf, (c + 1) * nfc)); the phase tables hold one chunk
     int nfc = 0, nchunks = 1, resident = -1;
     bool three_d = false;
@@ -95,7 +96,8 @@ struct swem_plan {
     swem::BasisCache* basis_cache = nullptr;
     // tables
     swem::DevBuf<float2> tw, twy, spec_s, ktab, stab;
-    swem::DevBuf<float2> chirp, bchirp;  // 2-D Bluestein: [nkx] chirp, [2][narr] kernel spectra
+    swem::DevBuf<float2> chirp, bchirp;  // x Bluestein: [nkx] chirp, [2][narr] kernel spectra
+    swem::DevBuf<float2> chirp_y, bchirp_y;  // 3-D y Bluestein: [nky] chirp, [2][narr_y]
     swem::DevBuf<float> taper, taper_y, p2f, pdf, w2, wavd;
     swem::Bases own_bases;
     const float* basis_adj = nullptr;  // [nt][2nf] (own_bases or a shared cache entry)

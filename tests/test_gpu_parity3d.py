@@ -162,3 +162,65 @@ def test_pspi3d_linear_mode_is_refused(gpu):
     with pytest.raises(seiswem.SwemError) as e:
         seiswem.Plan(g, vel, wav, pspi_mode=seiswem.PSPI_LINEAR)
     assert e.value.code == seiswem.SWEM_EUNSUPPORTED
+
+
+# 3-D grids whose padded lengths (padx 2) are not powers of two 128..1024: the x-pass and the
+# y-pass take those axes through Bluestein's identity on a power-of-two array length M >= 2n - 1
+BLUE3 = [
+    (50, 40, 12),   # nkx 100 (M 256), nky 80 (M 256)
+    (64, 37, 12),   # nkx 128 (a power of two), nky 74 (M 256)
+    (150, 32, 8),   # nkx 300 (M 1024, taper on x), nky 64 (a power of two below 128: M 128)
+    (30, 130, 8),   # nkx 60 (M 128), nky 260 (M 1024, taper on y)
+]
+
+
+@pytest.mark.parametrize("nx,ny,nz", BLUE3)
+def test_migrate3d_bluestein(gpu, nx, ny, nz):
+    g, vel, refl, wav, sx, sy = small3d(nx, ny, nz)
+    rng = np.random.default_rng(nx * ny)
+    d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
+    m_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), d=d, adj=True)
+    m_gpu = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)
+    err = rel_l2(m_gpu, m_ref)
+    print(f"3-D Bluestein migrate {nx}x{ny}x{nz}: rel-L2 {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("nx,ny,nz", BLUE3)
+def test_model3d_bluestein(gpu, nx, ny, nz):
+    g, vel, refl, wav, sx, sy = small3d(nx, ny, nz)
+    d_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), m=refl, adj=False)
+    d_gpu = seiswem.ShotProfileWEM(refl, False, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)[0]
+    err = rel_l2(d_gpu, d_ref)
+    print(f"3-D Bluestein model {nx}x{ny}x{nz}: rel-L2 {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+def test_dot_product_3d_bluestein(gpu):
+    g, vel, refl, wav, sx, sy = small3d(50, 40, 12)
+    a, b, e = dot_test(g, vel, wav, sx, sy, 9)
+    print(f"3-D Bluestein dot: {a:.9e} {b:.9e} rel {e:.3e}", flush=True)
+    assert e <= DOT_TOL
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(50, 40, 10), (150, 32, 8)])
+def test_pspi3d_bluestein(gpu, nx, ny, nz):
+    """3-D PSPI (compat) on Bluestein lengths, migration and modelling against the FP64 oracle."""
+    g, vel, refl, wav, sx, sy = pspi3d(nx, ny, nz)
+    rng = np.random.default_rng(nx + ny)
+    d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
+    m_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), d=d, adj=True)
+    m_gpu = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)
+    d_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), m=refl, adj=False)
+    d_gpu = seiswem.ShotProfileWEM(refl, False, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)[0]
+    e1, e2 = rel_l2(m_gpu, m_ref), rel_l2(d_gpu, d_ref)
+    print(f"3-D PSPI Bluestein {nx}x{ny}x{nz}: migrate {e1:.3e} model {e2:.3e}", flush=True)
+    assert e1 <= TOL and e2 <= TOL
+
+
+def test_3d_lengths_beyond_bluestein_are_refused(gpu):
+    """nky = 2 * 300 = 600 is neither a power of two nor <= 512: SWEM_EUNSUPPORTED, named."""
+    g, vel, refl, wav, sx, sy = small3d(64, 300, 4)
+    with pytest.raises(seiswem.SwemError) as e:
+        seiswem.Plan(g, vel, wav)
+    assert e.value.code == seiswem.SWEM_EUNSUPPORTED
