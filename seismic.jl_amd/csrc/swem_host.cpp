@@ -115,20 +115,45 @@ int build_bases(const swem_geometry& g, const Sampling& sm, int nf, Bases& b)
     const double rnt = 1.0 / std::sqrt((double)sm.ntfft);
     const double dw = 2.0 * kPi / ((double)sm.ntfft * (double)g.dt);
     std::vector<float> badj((size_t)nt * 2 * nf), bsrc((size_t)nt * 2 * nf), bfwd((size_t)2 * nf * nt);
+    // the angles are 2 pi e / n for the residues e = iw t mod n: one cos / sin per residue
+    std::vector<double> ct(n), st(n);
+    for (int e = 0; e < n; e++) {
+        const double ang = 2.0 * kPi * (double)e / (double)n;
+        ct[e] = std::cos(ang);
+        st[e] = std::sin(ang);
+    }
+    std::vector<double> w2s(nf);
+    for (int f = 0; f < nf; f++) {
+        const double w = (sm.ifmin + f) * dw;
+        w2s[f] = w * w * rnt;
+    }
+    // [t][f] layouts, rows in order (iw < n and t < n: the residue advances by one subtraction)
+    for (int t = 0; t < nt; t++) {
+        int e = (int)(((long long)sm.ifmin * t) % n);
+        float* ra = badj.data() + (size_t)t * 2 * nf;
+        float* rs = bsrc.data() + (size_t)t * 2 * nf;
+        for (int f = 0; f < nf; f++) {
+            const double c = ct[e], sn = st[e];
+            ra[2 * f] = (float)(w2s[f] * c);
+            ra[2 * f + 1] = (float)(-w2s[f] * sn);
+            rs[2 * f] = (float)(rnt * c);
+            rs[2 * f + 1] = (float)(-rnt * sn);
+            e += t;
+            if (e >= n) e -= n;
+        }
+    }
+    // [f][t] layout
     for (int f = 0; f < nf; f++) {
         const int iw = sm.ifmin + f;
-        const double w = iw * dw, w2s = w * w * rnt;
         const double cf = (iw == 0 || 2 * iw == n) ? 1.0 : 2.0;
+        float* rc = bfwd.data() + (size_t)(2 * f) * nt;
+        float* rn = rc + nt;
+        int e = 0;
         for (int t = 0; t < nt; t++) {
-            const long long e = ((long long)iw * t) % n;
-            const double ang = 2.0 * kPi * (double)e / (double)n;
-            const double c = std::cos(ang), sn = std::sin(ang);
-            badj[(size_t)t * 2 * nf + 2 * f] = (float)(w2s * c);
-            badj[(size_t)t * 2 * nf + 2 * f + 1] = (float)(-w2s * sn);
-            bsrc[(size_t)t * 2 * nf + 2 * f] = (float)(rnt * c);
-            bsrc[(size_t)t * 2 * nf + 2 * f + 1] = (float)(-rnt * sn);
-            bfwd[(size_t)(2 * f) * nt + t] = (float)(cf * w2s * c);
-            bfwd[(size_t)(2 * f + 1) * nt + t] = (float)(-cf * w2s * sn);
+            rc[t] = (float)(cf * w2s[f] * ct[e]);
+            rn[t] = (float)(-cf * w2s[f] * st[e]);
+            e += iw;
+            if (e >= n) e -= n;
         }
     }
     SWEM_TRY(b.adj.upload(badj.data(), badj.size()));

@@ -11,86 +11,93 @@
 // KZ64: every quantity in double (s, sqrt, sincos, exp), rounded to float at the end.
 // FP32: the reference's float expressions (s = (w*w)*p2 - kx*kx in float, sqrtf, sincosf).
 #include <algorithm>
+#include <cstdint>
 
 #include "swem_internal.hpp"
 
 namespace swem {
 
+// One workgroup per table row (f, iz, ir) at a time: the row's w and p2 are loaded once and the
+// entries need no index division.
 __global__ __launch_bounds__(256) void build_kz_table(PhaseTableArgs a)
 {
-    const size_t total = (size_t)a.nfc * a.nz * a.nref * a.nkh;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-        const int kk = (int)(i % a.nkh);
-        size_t r = i / a.nkh;
-        const int ir = (int)(r % a.nref);
-        r /= a.nref;
-        const int iz = (int)(r % a.nz);
-        const int f = (int)(r / a.nz);
+    const int rows = a.nfc * a.nz * a.nref;
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int ir = r % a.nref;
+        const int iz = (r / a.nref) % a.nz;
+        const int f = r / (a.nref * a.nz);
         const int iw = a.ifmin + a.f0 + f;
-        float2 L;
+        float2* row = a.ktab + (size_t)r * a.nkh;  // = ((f nz + iz) nref + ir) nkh
         if (a.prec == SWEM_PREC_KZ64) {
             const double w = (double)iw * a.dw;
-            const double kx = (double)kk * a.dkx;
-            const double s = w * w * a.p2[(size_t)iz * a.nref + ir] - kx * kx;
-            double re, im;
-            if (s >= 0.0) {
-                sincos(sqrt(s) * a.dz, &im, &re);
-            } else {
-                re = exp(-0.2 * sqrt(-s) * fabs(a.dz));
-                im = 0.0;
+            const double p2 = a.p2[(size_t)iz * a.nref + ir];
+            for (int kk = threadIdx.x; kk < a.nkh; kk += blockDim.x) {
+                const double kx = (double)kk * a.dkx;
+                const double s = w * w * p2 - kx * kx;
+                double re, im;
+                if (s >= 0.0) {
+                    sincos(sqrt(s) * a.dz, &im, &re);
+                } else {
+                    re = exp(-0.2 * sqrt(-s) * fabs(a.dz));
+                    im = 0.0;
+                }
+                row[kk] = make_float2((float)(re * a.inv_nk), (float)(im * a.inv_nk));
             }
-            L = make_float2((float)(re * a.inv_nk), (float)(im * a.inv_nk));
         } else {
             // wem.c:336-343 in float: kx = (float)dkx*ikx, s = (w*w)*p2 - kx*kx
             const float w = (float)iw * a.dwf;
-            const float kx = a.dkxf * (float)kk;
-            const float s = (w * w) * a.p2f[(size_t)iz * a.nref + ir] - kx * kx;
-            float re, im;
-            if (s >= 0.f) {
-                sincosf(sqrtf(s) * (float)a.dz, &im, &re);
-            } else {
-                re = expf(-0.2f * sqrtf(fabsf(s)) * fabsf((float)a.dz));
-                im = 0.f;
+            const float p2 = a.p2f[(size_t)iz * a.nref + ir];
+            for (int kk = threadIdx.x; kk < a.nkh; kk += blockDim.x) {
+                const float kx = a.dkxf * (float)kk;
+                const float s = (w * w) * p2 - kx * kx;
+                float re, im;
+                if (s >= 0.f) {
+                    sincosf(sqrtf(s) * (float)a.dz, &im, &re);
+                } else {
+                    re = expf(-0.2f * sqrtf(fabsf(s)) * fabsf((float)a.dz));
+                    im = 0.f;
+                }
+                row[kk] = make_float2(re * (float)a.inv_nk, im * (float)a.inv_nk);
             }
-            L = make_float2(re * (float)a.inv_nk, im * (float)a.inv_nk);
         }
-        a.ktab[i] = L;
     }
 }
 
+// One workgroup per row (f, iz) at a time.
 __global__ __launch_bounds__(256) void build_ss_table(PhaseTableArgs a)
 {
-    const size_t total = (size_t)a.nfc * a.nz * a.nmx;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-        const int x = (int)(i % a.nmx);
-        size_t r = i / a.nmx;
-        const int iz = (int)(r % a.nz);
-        const int f = (int)(r / a.nz);
+    const int rows = a.nfc * a.nz;
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int iz = r % a.nz;
+        const int f = r / a.nz;
         const int iw = a.ifmin + a.f0 + f;
-        float2 S;
-        if (a.prec == SWEM_PREC_KZ64) {
-            double sn, cs;
-            sincos((double)iw * a.dw * a.pd[(size_t)iz * a.nmx + x] * a.dz, &sn, &cs);
-            S = make_float2((float)cs, (float)sn);
-        } else {
-            float sn, cs;
-            sincosf((float)iw * a.dwf * a.pdf[(size_t)iz * a.nmx + x] * (float)a.dz, &sn, &cs);
-            S = make_float2(cs, sn);
+        float2* row = a.stab + (size_t)r * a.nmx;  // = (f nz + iz) nmx
+        for (int x = threadIdx.x; x < a.nmx; x += blockDim.x) {
+            float2 S;
+            if (a.prec == SWEM_PREC_KZ64) {
+                double sn, cs;
+                sincos((double)iw * a.dw * a.pd[(size_t)iz * a.nmx + x] * a.dz, &sn, &cs);
+                S = make_float2((float)cs, (float)sn);
+            } else {
+                float sn, cs;
+                sincosf((float)iw * a.dwf * a.pdf[(size_t)iz * a.nmx + x] * (float)a.dz, &sn, &cs);
+                S = make_float2(cs, sn);
+            }
+            row[x] = S;
         }
-        a.stab[i] = S;
     }
 }
 
 int launch_phase_tables(const PhaseTableArgs& a, bool ss, hipStream_t st)
 {
-    const size_t nk = (size_t)a.nfc * a.nz * a.nref * a.nkh;
-    const int blocks = (int)std::min<size_t>((nk + 255) / 256, 65536);
-    hipLaunchKernelGGL(build_kz_table, dim3(blocks), dim3(256), 0, st, a);
+    const long long rows = (long long)a.nfc * a.nz * a.nref;
+    if (rows < 1 || rows > INT32_MAX || a.nkh < 1 || a.nmx < 1)
+        return set_error(SWEM_EINVAL, "phase tables: %lld rows", rows);
+    hipLaunchKernelGGL(build_kz_table, dim3((unsigned)std::min<long long>(rows, 1 << 20)), dim3(256), 0, st, a);
     SWEM_HIP(hipGetLastError());
     if (ss) {
-        const size_t ns = (size_t)a.nfc * a.nz * a.nmx;
-        const int b2 = (int)std::min<size_t>((ns + 255) / 256, 65536);
-        hipLaunchKernelGGL(build_ss_table, dim3(b2), dim3(256), 0, st, a);
+        const int rows_s = a.nfc * a.nz;
+        hipLaunchKernelGGL(build_ss_table, dim3((unsigned)std::min(rows_s, 1 << 20)), dim3(256), 0, st, a);
         SWEM_HIP(hipGetLastError());
     }
     return SWEM_OK;
