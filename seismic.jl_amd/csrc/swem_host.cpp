@@ -108,25 +108,34 @@ static int choose_group(const swem_plan* p, int adj, int forced)
 }
 
 // Fourier bases of the band (f_op, wem.c:480-514, with the w^2 of wem.c:129,182 and the
-// 1/sqrt(ntfft) of wem.c:231-232,278), built in FP64.  Angle index (iw*t) mod n exact in integers.
+// 1/sqrt(ntfft) of wem.c:231-232,278), built in FP64.  Angle index (iw*t) mod n exact in integers:
+// the angles are 2 pi e / n for the residues e, one cos / sin per residue ([ct | st], n each),
+// and w^2 / sqrt(ntfft) per frequency (w2s, nf).
+static void basis_angles(const swem_geometry& g, const Sampling& sm, int nf, std::vector<double>& tab)
+{
+    const int n = sm.fop_n;
+    const double rnt = 1.0 / std::sqrt((double)sm.ntfft);
+    const double dw = 2.0 * kPi / ((double)sm.ntfft * (double)g.dt);
+    tab.resize((size_t)2 * n + nf);
+    for (int e = 0; e < n; e++) {
+        const double ang = 2.0 * kPi * (double)e / (double)n;
+        tab[e] = std::cos(ang);
+        tab[(size_t)n + e] = std::sin(ang);
+    }
+    for (int f = 0; f < nf; f++) {
+        const double w = (sm.ifmin + f) * dw;
+        tab[(size_t)2 * n + f] = w * w * rnt;
+    }
+}
+
 int build_bases(const swem_geometry& g, const Sampling& sm, int nf, Bases& b)
 {
     const int nt = g.nt, n = sm.fop_n;
     const double rnt = 1.0 / std::sqrt((double)sm.ntfft);
-    const double dw = 2.0 * kPi / ((double)sm.ntfft * (double)g.dt);
+    std::vector<double> tab;
+    basis_angles(g, sm, nf, tab);
+    const double *ct = tab.data(), *st = ct + n, *w2s = st + n;
     std::vector<float> badj((size_t)nt * 2 * nf), bsrc((size_t)nt * 2 * nf), bfwd((size_t)2 * nf * nt);
-    // the angles are 2 pi e / n for the residues e = iw t mod n: one cos / sin per residue
-    std::vector<double> ct(n), st(n);
-    for (int e = 0; e < n; e++) {
-        const double ang = 2.0 * kPi * (double)e / (double)n;
-        ct[e] = std::cos(ang);
-        st[e] = std::sin(ang);
-    }
-    std::vector<double> w2s(nf);
-    for (int f = 0; f < nf; f++) {
-        const double w = (sm.ifmin + f) * dw;
-        w2s[f] = w * w * rnt;
-    }
     // [t][f] layouts, rows in order (iw < n and t < n: the residue advances by one subtraction)
     for (int t = 0; t < nt; t++) {
         int e = (int)(((long long)sm.ifmin * t) % n);
@@ -159,6 +168,24 @@ int build_bases(const swem_geometry& g, const Sampling& sm, int nf, Bases& b)
     SWEM_TRY(b.adj.upload(badj.data(), badj.size()));
     SWEM_TRY(b.fwd.upload(bfwd.data(), bfwd.size()));
     SWEM_TRY(b.src.upload(bsrc.data(), bsrc.size()));
+    return SWEM_OK;
+}
+
+// The same bases filled on the device from the uploaded angle table (the same FP64 products,
+// rounded to float): 2 n + nf doubles cross PCIe instead of the three bases.
+static int build_bases_device(const swem_geometry& g, const Sampling& sm, int nf, Bases& b, hipStream_t st)
+{
+    const int nt = g.nt, n = sm.fop_n;
+    std::vector<double> tab;
+    basis_angles(g, sm, nf, tab);
+    DevBuf<double> dtab;
+    SWEM_TRY(dtab.upload(tab.data(), tab.size()));
+    SWEM_TRY(b.adj.alloc((size_t)nt * 2 * nf));
+    SWEM_TRY(b.fwd.alloc((size_t)2 * nf * nt));
+    SWEM_TRY(b.src.alloc((size_t)nt * 2 * nf));
+    SWEM_TRY(launch_build_bases(nt, nf, n, sm.ifmin, 1.0 / std::sqrt((double)sm.ntfft), dtab.p, b.adj.p, b.src.p,
+                                b.fwd.p, st));
+    SWEM_HIP(hipStreamSynchronize(st));  // dtab is freed on return
     return SWEM_OK;
 }
 
@@ -359,7 +386,10 @@ static int build_tables(swem_plan* p, const float* wav)
         p->basis_fwd = b->fwd.p;
         bsrc = b->src.p;
     } else {
-        SWEM_TRY(build_bases(g, p->s, nf, p->own_bases));
+        if (p->served)
+            SWEM_TRY(build_bases(g, p->s, nf, p->own_bases));
+        else
+            SWEM_TRY(build_bases_device(g, p->s, nf, p->own_bases, p->build_stream));
         p->basis_adj = p->own_bases.adj.p;
         p->basis_fwd = p->own_bases.fwd.p;
         bsrc = p->own_bases.src.p;

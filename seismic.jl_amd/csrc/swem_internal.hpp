@@ -201,6 +201,10 @@ int launch_sgemm(int M, int Nc, int K, const float* A, int lda, const float* B, 
 
 // Image reduction: out[(s)][x][iz] (+)= sum_{shot in group, g} partial[shot][g][iz][x], Kahan,
 // fixed order.  per_shot = 1 writes one image per shot (no stacking).
+// Fourier bases from tab = [cos | sin] (n each) of 2 pi e / n and w^2 / sqrt(ntfft) (nf):
+// badj, bsrc [nt][2 nf], bfwd [2 nf][nt] (f_op with the w^2 and 1/sqrt(ntfft) factors).
+int launch_build_bases(int nt, int nf, int n, int ifmin, double rnt, const double* tab, float* badj, float* bsrc,
+                       float* bfwd, hipStream_t st);
 int launch_reduce_image(const float* partial, int nshot, int ngroups, int nz, int nmx, float* out, int accumulate,
                         int per_shot, hipStream_t st);
 // Solver vector operations: m1 = a m1 + b m2; <x, y> in double into part[kDotBlocks] (part has

@@ -250,4 +250,40 @@ int launch_transpose(const float* in, int rows, int cols, float* out, hipStream_
     return SWEM_OK;
 }
 
+// Entry (t, f): residue e = iw t mod n (exact in 64-bit integers), then the products of the host
+// builder in the same order, so every float matches it.
+__global__ __launch_bounds__(256) void build_bases_k(int nt, int nf, int n, int ifmin, double rnt,
+                                                     const double* __restrict__ tab, float* badj, float* bsrc,
+                                                     float* bfwd)
+{
+    const long long total = (long long)nt * nf;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int t = (int)(i / nf), f = (int)(i - (long long)t * nf);
+        const int iw = ifmin + f;
+        const int e = (int)(((long long)iw * t) % n);
+        const double c = tab[e], sn = tab[n + e], w2s = tab[2 * n + f];
+        const double cf = (iw == 0 || 2 * iw == n) ? 1.0 : 2.0;
+        const size_t o = (size_t)t * 2 * nf + 2 * f;
+        badj[o] = (float)(w2s * c);
+        badj[o + 1] = (float)(-w2s * sn);
+        bsrc[o] = (float)(rnt * c);
+        bsrc[o + 1] = (float)(-rnt * sn);
+        bfwd[(size_t)(2 * f) * nt + t] = (float)(cf * w2s * c);
+        bfwd[(size_t)(2 * f + 1) * nt + t] = (float)(-cf * w2s * sn);
+    }
+}
+
+int launch_build_bases(int nt, int nf, int n, int ifmin, double rnt, const double* tab, float* badj, float* bsrc,
+                       float* bfwd, hipStream_t st)
+{
+    if (nt < 1 || nf < 1 || n < nt || ifmin < 0 || ifmin + nf > n)
+        return set_error(SWEM_EINVAL, "bases: nt=%d nf=%d n=%d ifmin=%d", nt, nf, n, ifmin);
+    const long long total = (long long)nt * nf;
+    hipLaunchKernelGGL(build_bases_k, dim3((unsigned)std::min<long long>((total + 255) / 256, 1 << 16)), dim3(256),
+                       0, st, nt, nf, n, ifmin, rnt, tab, badj, bsrc, bfwd);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
 }  // namespace swem
