@@ -132,8 +132,8 @@ int swem_wem(float* d, float* m, const float* wav,
  * source-side incidence angles in degrees, angx and angy [nm][nz] (host arrays, overwritten),
  * of one shot at (sx, sy), from a split-step source march with ntfft = nt and nkx = nmx and the
  * spatial derivatives of the source wavefield per frequency (wem.c:694-981).  2-D: nmx and nz up
- * to 2048, or powers of two to 4096; 3-D: nmx and nmy powers of two 128..1024 (others return
- * SWEM_EUNSUPPORTED).  angy is zero in 2-D, as in the reference. */
+ * to 2048, or powers of two to 4096; 3-D: nmx and nmy in 2..512 or powers of two to 1024 (others
+ * return SWEM_EUNSUPPORTED).  angy is zero in 2-D, as in the reference. */
 int swem_compute_angles(float* angx, float* angy, const float* wav,
                         int nt, float ot, float dt,
                         int nmx, float omx, float dmx,

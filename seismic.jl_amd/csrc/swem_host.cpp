@@ -1400,10 +1400,14 @@ static int angles3d_impl(swem_plan* p, float sx, float sy, float* angx, float* a
     SWEM_TRY(set_igx(p, &sx, &sy, 1, st));
     const int F = p->F3f;
     March2DArgs ax{}, ay{};
-    ax.nkx = nmx;
+    ax.nkx = nmx;  // nkx = nmx, nky = nmy (padx = 1); Bluestein tables where the march has them
     ax.tw = p->tw.p;
+    ax.chirp = p->chirp.p;
+    ax.bchirp = p->bchirp.p;
     ay.nkx = nmy;
     ay.tw = p->twy.p;
+    ay.chirp = p->chirp_y.p;
+    ay.bchirp = p->bchirp_y.p;
     AngleArgs b{};
     b.nz = nz;
     b.nw = p->s.nw;
@@ -1478,9 +1482,9 @@ int swem_compute_angles_ex(float* angx, float* angy, const float* wav, const swe
 {
     if (!angx || !angy || !wav || !vel || !g0) return set_error(SWEM_EINVAL, "compute_angles: null argument");
     const swem_geometry& g = *g0;
-    if (g.nmy > 1 && !(march3d_len(g.nmx) == g.nmx && march3d_len(g.nmy) == g.nmy))
-        return set_error(SWEM_EUNSUPPORTED, "compute_angles: 3-D needs nmx and nmy powers of two 128..1024 (%d x %d)",
-                         g.nmx, g.nmy);
+    if (g.nmy > 1 && !march3d_supported(g.nmx, g.nmy))
+        return set_error(SWEM_EUNSUPPORTED, "compute_angles: 3-D takes nmx and nmy in 2..512 or powers of two to 1024 "
+                         "(%d x %d)", g.nmx, g.nmy);
     if (g.nmx < 2 || !angles_supported(g.nmx) || !angles_supported(g.nz))
         return set_error(SWEM_EUNSUPPORTED, "compute_angles: nmx = %d, nz = %d: this build handles 2..2048 and powers of "
                          "two to 4096 (nmx >= 2)", g.nmx, g.nz);

@@ -1,8 +1,8 @@
 """Angle-domain shot-profile imaging.
 
   compute_angles         wem.c:566-992 (compute_angles.c:12-118 without the file I/O): source
-                         incidence angles of one shot on the device (2-D, and 3-D on
-                         power-of-two grids 128..1024)
+                         incidence angles of one shot on the device (2-D, and 3-D with nmx,
+                         nmy up to 512 or powers of two up to 1024)
   angle_bin / angle_unbin   the bilinear angle binning of ShotProfileWEM.jl:162-188 (migration:
                          a shot image spread into angle gathers) and its transpose
                          ShotProfileWEM.jl:246-275 (modelling: angle gathers collapsed to a shot

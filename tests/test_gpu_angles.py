@@ -84,30 +84,33 @@ def test_angle_gather_pair_dot_test(gpu):
     assert abs(a - b) / abs(b) <= DOT_TOL
 
 
-def test_compute_angles_3d_matches_oracle(gpu):
+@pytest.mark.parametrize("nx,ny,nz", [(128, 128, 16), (100, 60, 12), (64, 150, 10)])
+def test_compute_angles_3d_matches_oracle(gpu, nx, ny, nz):
     """3-D (nkx = nmx, nky = nmy: the 3-D march's x- and y-passes with the receiver's conj(L)
-    step, then z, x and y passes): both angle volumes against the oracle's 3-D transforms."""
-    g = seiswem.WemGeometry(nt=64, dt=0.004, nmx=128, dmx=10.0, nmy=128, dmy=10.0, nz=16, dz=10.0, fmin=5.0,
+    step, then z, x and y passes): both angle volumes against the oracle's 3-D transforms.  100, 60,
+    64 and 150 are Bluestein lengths of the march and the lateral passes (arrays 256, 128, 128, 512)."""
+    g = seiswem.WemGeometry(nt=64, dt=0.004, nmx=nx, dmx=10.0, nmy=ny, dmy=10.0, nz=nz, dz=10.0, fmin=5.0,
                             fmax=40.0, pspi=False)
     rng = np.random.default_rng(6)
     vel = (2000.0 + 200.0 * rng.random((g.nm, g.nz))).astype(np.float32)
     wav = synth.wavelet_trace(g.nt, g.dt, 20.0)
-    sx, sy = 600.0, 700.0
+    sx, sy = 10.0 * (nx // 2 - 4), 10.0 * (ny // 2 + 6)  # (600, 700) at 128 x 128
     ax, ay = seiswem.compute_angles(vel, wav, g, sx, sy)
     rx, ry = O.angles(oparams(g), vel, wav, sx, sy)
     for name, a, r in (("x", ax, rx), ("y", ay, ry)):
         mag = rel_l2(np.abs(a), np.abs(r))
         agree = float(np.mean(np.sign(a) == np.sign(r)))
-        print(f"angles 3-D 128 x 128 x 16, {name}: |angle| rel-L2 {mag:.3e}, signed {rel_l2(a, r):.3e}, "
+        print(f"angles 3-D {nx} x {ny} x {nz}, {name}: |angle| rel-L2 {mag:.3e}, signed {rel_l2(a, r):.3e}, "
               f"sign agreement {agree:.6f}", flush=True)
         # the sign is the reference's discrete left / right decision summed over frequencies
         # (wem.c:739-740, 671); where that sum is near zero (near the surface, far from the
-        # source) FP32 and FP64 may decide differently: 2 of 262144 points in x here
+        # source) FP32 and FP64 may decide differently: 2 of 262144 points in x at 128 x 128 x 16
         assert mag <= ANG_TOL and agree >= 0.9999
 
 
 def test_compute_angles_3d_refused_off_grid(gpu):
-    g = seiswem.WemGeometry(nt=64, dt=0.004, nmx=100, dmx=10.0, nmy=16, dmy=10.0, nz=8, dz=10.0, pspi=False)
+    """nmx = 600 is neither a power of two nor <= 512: SWEM_EUNSUPPORTED."""
+    g = seiswem.WemGeometry(nt=64, dt=0.004, nmx=600, dmx=10.0, nmy=16, dmy=10.0, nz=8, dz=10.0, pspi=False)
     vel = np.full((g.nm, g.nz), 2000.0, np.float32)
     with pytest.raises(seiswem.SwemError) as e:
         seiswem.compute_angles(vel, synth.wavelet_trace(64, 0.004, 20.0), g, 80.0, 80.0)
