@@ -14,13 +14,14 @@ pytestmark = pytest.mark.gpu
 
 def mixed(n, seed=0):
     """Small requests of every kind: 2-D SSF, 2-D PSPI (compat and linear; nx 100 takes the
-    Bluestein kernels), 3-D SSF."""
+    Bluestein kernels), 3-D SSF (every second one on 50 x 40 points: Bluestein lengths 100 x 80)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
         kind = i % 4
         if kind == 3:
-            g, vel, _, wav, sx, sy = synth.config3(seed=i, nx=64, ny=64, nz=16, nt=128, fmax=20.0)
+            nx3, ny3 = (64, 64) if (i // 4) % 2 == 0 else (50, 40)
+            g, vel, _, wav, sx, sy = synth.config3(seed=i, nx=nx3, ny=ny3, nz=16, nt=128, fmax=20.0)
             mode = seiswem.PSPI_COMPAT
         else:
             nx = int(rng.choice([64, 100, 128]))
