@@ -365,17 +365,18 @@ static int build_tables(swem_plan* p, const float* wav)
         for (int iz = 0; iz < nz; iz++) {
             const float z = g.oz + g.dz * (float)iz;
             if (!(z >= g.sz && z >= g.gz)) packed = false;
-            int* row = cref.data() + (size_t)iz * kCrefStride;
-            row[0] = -1;
-            for (int ix = 0; ix < nm; ix++) {
+            cref[(size_t)iz * kCrefStride] = -1;
+        }
+        for (int ix = 0; ix < nm; ix++)  // traces in storage order; per depth ix ascends
+            for (int iz = 0; iz < nz; iz++) {
                 const size_t i = (size_t)ix * nz + iz;
                 if (cut[i]) continue;
+                int* row = cref.data() + (size_t)iz * kCrefStride;
                 if (row[0] < 0) row[0] = i1[i];
                 else if (row[0] != i1[i]) packed = false;
                 if (row[1] < kUncutSparse) row[2 + row[1]] = ix;
                 row[1]++;
             }
-        }
         SWEM_TRY(p->cref.upload(cref.data(), cref.size()));
         p->packed_ok = packed;
     }

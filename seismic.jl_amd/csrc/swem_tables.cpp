@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <vector>
+
 #include "swem_internal.hpp"
 
 namespace swem {
@@ -70,18 +72,26 @@ int source_index(const swem_geometry& g, float sx, float sy, int& igx, int& igy)
     return SWEM_OK;
 }
 
-// wem.c:27-36 in float (naive-FP32 tables): float sum, (float)nmx*nmy, 1./po in double.
+// wem.c:27-36 in float (naive-FP32 tables): float sum, (float)nmx*nmy, 1./po in double.  The
+// traces are read in storage order with one accumulator per depth, so each depth's sum runs over
+// ix = 0, 1, ... in the reference's order.
 void slowness_f32(const swem_geometry& g, const float* vel, float* po, float* pd)
 {
     const int nm = g.nmx * g.nmy, nz = g.nz;
+    std::vector<float> sum(nz, 0.f);
+    for (int ix = 0; ix < nm; ix++) {
+        const float* v = vel + (size_t)ix * nz;
+        for (int iz = 0; iz < nz; iz++) sum[iz] += v[iz];
+    }
     for (int iz = 0; iz < nz; iz++) {
-        float p = 0.f;
-        for (int ix = 0; ix < nm; ix++) p += vel[(size_t)ix * nz + iz];
+        float p = sum[iz];
         p /= (float)g.nmx * g.nmy;
-        p = (float)(1. / (double)p);
-        po[iz] = p;
-        for (int ix = 0; ix < nm; ix++)
-            pd[(size_t)ix * nz + iz] = (float)(1.0 / (double)vel[(size_t)ix * nz + iz] - (double)p);
+        po[iz] = (float)(1. / (double)p);
+    }
+    for (int ix = 0; ix < nm; ix++) {
+        const float* v = vel + (size_t)ix * nz;
+        float* d = pd + (size_t)ix * nz;
+        for (int iz = 0; iz < nz; iz++) d[iz] = (float)(1.0 / (double)v[iz] - (double)po[iz]);
     }
 }
 
@@ -93,26 +103,32 @@ int pspi_tables(const swem_geometry& g, const float* vel, float* vref, int* iref
     const int nmx = g.nmx, nm = g.nmx * g.nmy, nz = g.nz, nref = g.nref;
     if (nref < 2) return set_error(SWEM_EINVAL, "PSPI needs nref >= 2 (nref=%d divides by zero at wem.c:52)", nref);
     if (nref > 31) return set_error(SWEM_EUNSUPPORTED, "nref=%d > 31", nref);
-    for (int iz = 0; iz < nz; iz++) {
-        float vmin = vel[iz];
-        for (int ix = 0; ix < nmx; ix++)
-            if (vel[(size_t)ix * nz + iz] < vmin) vmin = vel[(size_t)ix * nz + iz];
-        float vmax = vel[(size_t)(nmx - 1) * nz + iz];
-        for (int ix = 0; ix < nm; ix++)
-            if (vel[(size_t)ix * nz + iz] > vmax) vmax = vel[(size_t)ix * nz + iz];
+    // per depth: vmin over the first nmx traces, vmax over all (traces in storage order)
+    std::vector<float> vmin(vel, vel + nz), vmax(vel + (size_t)(nmx - 1) * nz, vel + (size_t)nmx * nz);
+    for (int ix = 0; ix < nmx; ix++) {
+        const float* v = vel + (size_t)ix * nz;
+        for (int iz = 0; iz < nz; iz++)
+            if (v[iz] < vmin[iz]) vmin[iz] = v[iz];
+    }
+    for (int ix = 0; ix < nm; ix++) {
+        const float* v = vel + (size_t)ix * nz;
+        for (int iz = 0; iz < nz; iz++)
+            if (v[iz] > vmax[iz]) vmax[iz] = v[iz];
+    }
+    for (int iz = 0; iz < nz; iz++)
         for (int ir = 0; ir < nref; ir++) {
-            float q = (float)ir * (vmax - vmin);
+            float q = (float)ir * (vmax[iz] - vmin[iz]);
             q = q / ((float)nref - 1);
-            vref[(size_t)ir * nz + iz] = vmin + q;
+            vref[(size_t)ir * nz + iz] = vmin[iz] + q;
         }
-        const float vlim = vmin + 10;
-        for (int ix = 0; ix < nm; ix++) {
+    for (int ix = 0; ix < nm; ix++)
+        for (int iz = 0; iz < nz; iz++) {
             const size_t i = (size_t)ix * nz + iz;
-            const float v = vel[i];
+            const float v = vel[i], lo = vmin[iz], hi = vmax[iz];
             int i1, i2;
-            if (vmax > vlim) {
-                float q = (float)(nref - 1) * (v - vmin);
-                q = q / (vmax - vmin);
+            if (hi > lo + 10) {
+                float q = (float)(nref - 1) * (v - lo);
+                q = q / (hi - lo);
                 int ir = (int)q;
                 if (ir < 0) ir = 0;
                 i1 = ir;
@@ -130,7 +146,6 @@ int pspi_tables(const swem_geometry& g, const float* vel, float* vref, int* iref
             const float dv = vref[(size_t)i2 * nz + iz] - vref[(size_t)i1 * nz + iz];
             if (cut) cut[i] = ((double)dv > 10.0) ? 1 : 0;
         }
-    }
     return SWEM_OK;
 }
 
