@@ -963,7 +963,7 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
         p->narr = march3d_len(p->s.nkx);
         p->narr_y = march3d_len(p->s.nky);
     } else if (!march2d_supported(p->nkx)) {
-        return set_error(SWEM_EUNSUPPORTED, "nkx = padx*nmx = %d: this build handles 2..2048 and powers of two to 4096",
+        return set_error(SWEM_EUNSUPPORTED, "nkx = padx*nmx = %d: this build handles 1..2048 and powers of two to 4096",
                          p->nkx);
     }
     if (p->nf < 1) return set_error(SWEM_EINVAL, "empty frequency band [ifmin=%d, ifmax=%d)", p->s.ifmin, p->s.ifmax);

@@ -959,12 +959,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void m
 //   PACKED       PAIR's shape; PSPI compat migration with both wavefields in one transform
 //                (the host selects it only where every depth steps both wavefields and the
 //                uncut points of a depth share one vref; other modes run PAIR).
-//   Bluestein    any other transform length up to 2048: the PAIR body (migration) or the
+//   Bluestein    any other transform length up to 2048 (nkx = 1 too: one trace, nmx = 1, whose
+//                transform is the identity): the PAIR body (migration) or the
 //                subgroup body (modelling) over the array length M = blue (P = 8 from M = 1024,
 //                one wave below); the variant option does not apply.
 int march2d_blue_len(int nkx)
 {
-    if (nkx < 2 || nkx > 2048 || (nkx >= 128 && !(nkx & (nkx - 1)))) return 0;
+    if (nkx < 1 || nkx > 2048 || (nkx >= 128 && !(nkx & (nkx - 1)))) return 0;
     int m = 128;
     while (m < 2 * nkx - 1) m <<= 1;
     return m;
@@ -1327,7 +1328,7 @@ static int launch_angles_len(int which, int blue, const March2DArgs& ax, const M
                 : launch_angles_n<N, XPow2<N, P>>(which, ax, az, b, F, st);
 }
 
-int angles_supported(int n) { return march2d_blue_len(n) || (n >= 128 && n <= 4096 && !(n & (n - 1))); }
+int angles_supported(int n) { return n >= 2 && (march2d_blue_len(n) || (n >= 128 && n <= 4096 && !(n & (n - 1)))); }
 
 int launch_angles(int which, const March2DArgs& ax, const March2DArgs& az, const AngleArgs& b, int F, hipStream_t st)
 {

@@ -1,4 +1,4 @@
-"""GPU edge cases against the FP64 oracle: the smallest grids (nmx 2 and 3, nz 1), the largest
+"""GPU edge cases against the FP64 oracle: the smallest grids (one trace, nmx 2 and 3, nz 1), the largest
 power-of-two transform (nkx 4096), receivers and sources below the surface with a shifted depth
 origin, and zero records (an exactly zero image)."""
 import numpy as np
@@ -19,7 +19,7 @@ def geom(nx, nz, nt=128, pspi=False, **kw):
                                pspi=pspi, **kw)
 
 
-@pytest.mark.parametrize("nx,nz", [(2, 6), (3, 5), (8, 1)])
+@pytest.mark.parametrize("nx,nz", [(1, 12), (2, 6), (3, 5), (8, 1)])
 @pytest.mark.parametrize("pspi,mode", [(False, 0), (True, 0), (True, 1)])
 def test_smallest_grids(gpu, nx, nz, pspi, mode):
     g = geom(nx, nz, pspi=pspi)
