@@ -957,8 +957,8 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
     if (p->three_d) {
         if (g->pspi && o.pspi_mode != SWEM_PSPI_COMPAT)
             return set_error(SWEM_EUNSUPPORTED, "3-D PSPI: the linear mode is not available in this build (compat is)");
-        if (g->nmx < 2 || !march3d_supported(p->s.nkx, p->s.nky))
-            return set_error(SWEM_EUNSUPPORTED, "3-D: nkx=%d, nky=%d: this build handles 2..512 and powers of two to 1024",
+        if (!march3d_supported(p->s.nkx, p->s.nky))
+            return set_error(SWEM_EUNSUPPORTED, "3-D: nkx=%d, nky=%d: this build handles 1..512 and powers of two to 1024",
                              p->s.nkx, p->s.nky);
         p->narr = march3d_len(p->s.nkx);
         p->narr_y = march3d_len(p->s.nky);

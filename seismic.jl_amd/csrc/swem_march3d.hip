@@ -756,11 +756,12 @@ __global__ __launch_bounds__(1024) void march3d_rows_fwd(March3DArgs a, int F, i
 // ------------------------------------------------------------------------ dispatch
 
 // Array length of a 3-D line transform of n points: n for powers of two 128..1024, else the
-// Bluestein length (the power of two >= max(128, 2n - 1), up to 1024: n <= 512); 0 if neither.
+// Bluestein length (the power of two >= max(128, 2n - 1), up to 1024: n <= 512; n = 1 is the
+// nkx of a single inline, nmx = 1); 0 if neither.
 int march3d_len(int n)
 {
     if (n >= 128 && n <= 1024 && (n & (n - 1)) == 0) return n;
-    if (n < 2 || n > 512) return 0;
+    if (n < 1 || n > 512) return 0;
     int m = 128;
     while (m < 2 * n - 1) m <<= 1;
     return m;

@@ -224,3 +224,19 @@ def test_3d_lengths_beyond_bluestein_are_refused(gpu):
     with pytest.raises(seiswem.SwemError) as e:
         seiswem.Plan(g, vel, wav)
     assert e.value.code == seiswem.SWEM_EUNSUPPORTED
+
+
+@pytest.mark.parametrize("ny,pspi", [(40, False), (128, False), (40, True)])
+def test_single_inline_3d(gpu, ny, pspi):
+    """nmx = 1 with nmy > 1: nkx = 1 (wem.c:80), the x-pass transforms one point through the
+    Bluestein array; migration and modelling against the FP64 oracle."""
+    g, vel, refl, wav, sx, sy = pspi3d(1, ny, 8) if pspi else small3d(1, ny, 8)
+    rng = np.random.default_rng(ny)
+    d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
+    m_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), d=d, adj=True)
+    m_gpu = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)
+    d_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), m=refl, adj=False)
+    d_gpu = seiswem.ShotProfileWEM(refl, False, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)[0]
+    e1, e2 = rel_l2(m_gpu, m_ref), rel_l2(d_gpu, d_ref)
+    print(f"3-D single inline 1 x {ny} pspi={pspi}: migrate {e1:.3e} model {e2:.3e}", flush=True)
+    assert e1 <= TOL and e2 <= TOL
