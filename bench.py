@@ -88,7 +88,7 @@ def bench_ssf3d(args):
     import seiswem
     from seiswem import synth
 
-    g, vel, refl, wav, sx, sy = synth.config3()
+    g, vel, refl, wav, sx, sy = synth.config3(nx=args.grid, ny=args.grid)
     if args.nz:  # fewer depths of the same grid (per-dispatch profiling; every depth is the same work)
         g = dataclasses.replace(g, nz=args.nz)
         vel, refl = np.ascontiguousarray(vel[:, :args.nz]), np.ascontiguousarray(refl[:, :args.nz])
@@ -125,7 +125,7 @@ def bench_ssf3d(args):
         "unit": "shots/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (3-D generator; record by the library's own demigration)",
-        "config": {"workload": "3-D SSF migration, 1 shot, nx=ny=256 nz=300 nt=1500 3-50 Hz", "nkx": s["nkx"],
+        "config": {"workload": f"3-D SSF migration, 1 shot, nx=ny={g.nmx} nz={g.nz} nt=1500 3-50 Hz", "nkx": s["nkx"],
                    "nky": s["nky"], "nf": nf, "nz": g.nz},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "march3d x-pass + y-pass",
@@ -435,6 +435,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nz", type=int, default=0, help="ssf3d: march only the first NZ depths (profiling)")
     ap.add_argument("--no-dot", action="store_true", help="ssf3d: skip the full-size dot test")
+    ap.add_argument("--grid", type=int, default=256,
+                    help="ssf3d: nx = ny of the 3-D grid (256: the 3-D configuration; others, e.g. 250, take "
+                         "the Bluestein kernels)")
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="pspi2d: shots timed one per call on the plain and resident single-shot paths (0: skip)")
     ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm", "angles", "elastic"), default="pspi2d")

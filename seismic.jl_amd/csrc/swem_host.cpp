@@ -973,8 +973,9 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
     if (p->three_d) {
         // y-pass workgroups: one wave per frequency (both wavefields in migration)
         if (o.freqs_per_block > 0) p->F3 = p->F3f = o.freqs_per_block;
-        if (p->F3 < 1 || 64 * p->F3 > (g->pspi ? 512 : 1024) || 64 * p->F3f > (g->pspi ? 512 : 1024))
-            return set_error(SWEM_EINVAL, "3-D: freqs_per_block=%d (1..16)", o.freqs_per_block);
+        const int fmax = (g->pspi || p->narr_y != p->s.nky) ? 8 : 16;  // Bluestein y-passes: 512 threads
+        if (p->F3 < 1 || p->F3 > fmax || p->F3f > fmax)
+            return set_error(SWEM_EINVAL, "3-D: freqs_per_block=%d (1..%d)", o.freqs_per_block, fmax);
     } else {
         const March2DShape sa = march2d_shape(p->nkx, o.variant, 1), sf = march2d_shape(p->nkx, o.variant, 0);
         p->T = sa.T;

@@ -568,7 +568,7 @@ __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, i
 // next_r: it is stepped at the next depth (its forward y-FFT is needed); the imaging runs when dr.
 // The split-step phase is computed once and applied conjugated to the source.
 template <int NY, bool BY>
-__global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
+__global__ __launch_bounds__(BY ? 512 : 1024) void march3d_rows_adj(March3DArgs a, int F)
 {
     constexpr int P = NY / 64;
     constexpr int L = LdsLen<NY>::value;
@@ -664,7 +664,7 @@ __global__ __launch_bounds__(1024) void march3d_rows_adj(March3DArgs a, int F)
 //           the previous step's result is stored in smig at depth iz_sel; with ds the next step
 //           starts (taper, conj(SS), FFTy).
 template <int NY, bool BY>
-__global__ __launch_bounds__(1024) void march3d_rows_fwd(March3DArgs a, int F, int phase)
+__global__ __launch_bounds__(BY ? 512 : 1024) void march3d_rows_fwd(March3DArgs a, int F, int phase)
 {
     constexpr int P = NY / 64;
     constexpr int L = LdsLen<NY>::value;
@@ -838,8 +838,11 @@ int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_
         return set_error(SWEM_EINVAL, "3-D march: inconsistent launch (nkx=%d nky=%d)", a.nkx, a.nky);
     if (m != n && (xpass ? !(a.chx && a.bchx) : !(a.chy && a.bchy)))
         return set_error(SWEM_EINVAL, "3-D march: Bluestein tables missing (n=%d)", n);
-    if (((which == 2 || which == 3) && 64 * F > 1024) || (which == 5 && 64 * F > 512) || F < 1)
-        return set_error(SWEM_EINVAL, "3-D march: %d frequencies per workgroup", F);
+    // y-passes: up to 16 frequencies per workgroup (8 on a Bluestein y axis, whose kernels are
+    // compiled for 512 threads to keep their longer transforms in registers; 8 for PSPI)
+    const int fmax = which == 5 ? 8 : (!xpass && m != n ? 8 : 16);
+    if (((which == 2 || which == 3 || which == 5) && F > fmax) || F < 1)
+        return set_error(SWEM_EINVAL, "3-D march: %d frequencies per workgroup (at most %d here)", F, fmax);
     if ((which == 4 || which == 5) && (!a.R || !a.sel || !a.used || a.nref < 1 || a.nref > 31))
         return set_error(SWEM_EINVAL, "3-D PSPI march: tables missing");
     const bool blue = m != n;
