@@ -30,6 +30,15 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
+def host_cores():
+    """Threads the CPU baseline runs on: OMP_NUM_THREADS when the environment sets the process's
+    CPU share, else every CPU this process may run on (sched_getaffinity, i.e. nproc)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(g, vel, wav, sx, m_gpu_fn, cores):
     """Oracle (C port of wem.c, naive FP32 = the reference arithmetic) on a bounded sample:
     one shot, a band of 2*cores frequency bins at full nx/nz.  Also returns the FP64-oracle
@@ -58,7 +67,10 @@ def cpu_baseline(g, vel, wav, sx, m_gpu_fn, cores):
     err = float(np.linalg.norm(m_gpu - m64) / np.linalg.norm(m64))
     sample = (f"1 shot, {nf_samp} of {nf_full} frequency bins ({fmin:g}-{fmax:g} Hz) at full nx={g.nmx}, nz={g.nz}, "
               f"{t_cpu:.2f} s, scaled by bins")
-    return {"value": shots_per_s, "unit": "shots/s", "cores": cores, "kind": "port", "sample": sample}, err, nf_samp
+    return {"value": shots_per_s, "unit": "shots/s", "cores": cores, "kind": "port", "sample": sample,
+            "host_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "threads_from": "OMP_NUM_THREADS" if os.environ.get("OMP_NUM_THREADS", "").isdigit() else "nproc"}, \
+        err, nf_samp
 
 
 COPY_BYTES = 1 << 30
@@ -81,28 +93,29 @@ def traffic_per_launch(nshot, kernel):
     return (fetch + write) * 1024
 
 
-def bench_ssf3d(args):
+def measure_ssf3d(args, steps, warmup, grid=256, nz=0, dot=True):
     """3-D split-step migration of one shot (nx=ny=256, dx=dy=12.5 m; nz=300, dz=10 m; nt=1500,
     dt=4 ms; 3-50 Hz = 565 bins; nkx=nky=512).  HBM-bound: algorithmic bytes per (w, iz) are
-    64 nk (two wavefields x two passes x read+write of the padded complex slice)."""
+    64 nk (two wavefields x two passes x read+write of the padded complex slice).  Returns the
+    measured figures as a dict."""
     import seiswem
     from seiswem import synth
 
-    g, vel, refl, wav, sx, sy = synth.config3(nx=args.grid, ny=args.grid)
-    if args.nz:  # fewer depths of the same grid (per-dispatch profiling; every depth is the same work)
-        g = dataclasses.replace(g, nz=args.nz)
-        vel, refl = np.ascontiguousarray(vel[:, :args.nz]), np.ascontiguousarray(refl[:, :args.nz])
+    g, vel, refl, wav, sx, sy = synth.config3(nx=grid, ny=grid)
+    if nz:  # fewer depths of the same grid (per-dispatch profiling; every depth is the same work)
+        g = dataclasses.replace(g, nz=nz)
+        vel, refl = np.ascontiguousarray(vel[:, :nz]), np.ascontiguousarray(refl[:, :nz])
     plan = seiswem.Plan(g, vel, wav)
     mm = seiswem.DeviceArray(refl.shape, refl)
     d = plan.model(mm, sx, sy)
     seiswem.synchronize()
     m = seiswem.DeviceArray((g.nm, g.nz))
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         plan.migrate(d, sx, sy, m=m)
     seiswem.synchronize()
     march_ms = 0.0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         plan.migrate(d, sx, sy, m=m)
         march_ms += plan.timing()["march_ms"]
     seiswem.synchronize()
@@ -110,31 +123,41 @@ def bench_ssf3d(args):
     s = g.sampling()
     nf, nk = s["ifmax"] - s["ifmin"], s["nkx"] * s["nky"]
     byts = 64.0 * nk * nf * g.nz
-    achieved = byts / (march_ms / args.steps / 1e3) / 1e9
-    # adjoint dot test at full size
-    a = b = 1.0
-    if not args.no_dot:
+    achieved = byts / (march_ms / steps / 1e3) / 1e9
+    out = {"shots_per_s": steps / elapsed, "ms_per_shot": elapsed / steps * 1e3, "steps": steps, "warmup": warmup,
+           "workload": f"3-D SSF migration, 1 shot, nx=ny={g.nmx} nz={g.nz} nt=1500 3-50 Hz",
+           "nkx": s["nkx"], "nky": s["nky"], "nf": nf, "nz": g.nz,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "march3d x-pass + y-pass",
+                        "algorithmic_bytes_per_shot": byts,
+                        "note": "algorithmic bytes 64 nk per (w, iz); the kernels skip the pad rows and move half"},
+           "dot_test_rel": None}
+    if dot:
         rng = np.random.default_rng(7)
         mr = rng.standard_normal((g.nm, g.nz)).astype(np.float32)
         dr = rng.standard_normal((1, g.nm, g.nt)).astype(np.float32)
         lm = plan.model(seiswem.DeviceArray(mr.shape, mr), sx, sy).download()
         lhd = plan.migrate(seiswem.DeviceArray(dr.shape, dr), sx, sy).download()
         a, b = float(np.sum(dr.astype(np.float64) * lm)), float(np.sum(mr.astype(np.float64) * lhd))
+        out["dot_test_rel"] = abs(a - b) / abs(b)
+    plan.close()
+    return out
+
+
+def bench_ssf3d(args):
+    r = measure_ssf3d(args, args.steps, args.warmup, grid=args.grid, nz=args.nz, dot=not args.no_dot)
     out = {
-        "metric": "migrated shots/sec (3D split-step) at <=1e-4 rel-L2 vs FP64", "value": args.steps / elapsed,
+        "metric": "migrated shots/sec (3D split-step) at <=1e-4 rel-L2 vs FP64", "value": r["shots_per_s"],
         "unit": "shots/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "ms_per_step": r["ms_per_shot"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (3-D generator; record by the library's own demigration)",
-        "config": {"workload": f"3-D SSF migration, 1 shot, nx=ny={g.nmx} nz={g.nz} nt=1500 3-50 Hz", "nkx": s["nkx"],
-                   "nky": s["nky"], "nf": nf, "nz": g.nz},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "march3d x-pass + y-pass",
-                     "algorithmic_bytes_per_shot": byts},
-        "dot_test_rel": None if args.no_dot else abs(a - b) / abs(b),
+        "config": {"workload": r["workload"], "nkx": r["nkx"], "nky": r["nky"], "nf": r["nf"], "nz": r["nz"]},
+        "roofline": r["roofline"], "dot_test_rel": r["dot_test_rel"],
     }
     if args.nz:
         out["config"]["workload"] += f" (first {args.nz} depths only)"
     if args.copy_calibration:
+        import seiswem
         src = seiswem.DeviceArray((COPY_BYTES // 4,))
         dst = seiswem.DeviceArray((COPY_BYTES // 4,))
         src.zero()
@@ -144,7 +167,7 @@ def bench_ssf3d(args):
     print(json.dumps(out))
 
 
-def bench_serving(args):
+def measure_serving(args):
     """Serving workload: Poisson arrivals (default 50 requests/s) of mixed 2-D / 3-D SSF
     migration requests (80 % 2-D: nx 128..1024, nz 200/400, nt 1000..4000; 20 % 3-D: 128^2 or
     256^2 x 200).  Reports p50 / p99 latency (submission -> image ready) at that rate, the
@@ -202,17 +225,28 @@ def bench_serving(args):
         seiswem.ShotProfileWEM(r[3], True, vel=r[1], wav=r[2], geom=r[0], sx=r[4], sy=r[5])
     plain = k / (time.perf_counter() - t0)
     n3 = sum(1 for r in reqs if r[0].nmy > 1)
-    out = {
-        "metric": "migrated shots/sec (2D & 3D) at <=1e-4 rel-L2 vs FP64; p50/p99 request latency",
-        "value": served, "unit": "shots/s", "n_gpus": 1, "steps": len(reqs), "warmup": len(seen),
-        "ms_per_step": 1e3 / served, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (per-request seeded velocity; random records)",
-        "config": {"workload": f"serving: {len(reqs)} requests ({n3} 3-D), Poisson {args.rate:g}/s",
-                   "streams": 4, "max_inflight": 8},
+    return {
+        "workload": f"serving: {len(reqs)} requests ({n3} 3-D), Poisson {args.rate:g}/s", "requests": len(reqs),
+        "requests_3d": n3, "streams": 4, "max_inflight": 8, "warmup_requests": len(seen),
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "mean": float(lat.mean()), "max": float(lat.max())},
-        "poisson_run_s": wall, "saturated_run_s": wall_sat,
+        "served_shots_per_s": served, "poisson_run_s": wall, "saturated_run_s": wall_sat,
         "plain_shots_per_s": plain, "plain_sample": k, "served_vs_plain": served / plain,
+    }
+
+
+def bench_serving(args):
+    r = measure_serving(args)
+    out = {
+        "metric": "migrated shots/sec (2D & 3D) at <=1e-4 rel-L2 vs FP64; p50/p99 request latency",
+        "value": r["served_shots_per_s"], "unit": "shots/s", "n_gpus": 1, "steps": r["requests"],
+        "warmup": r["warmup_requests"], "ms_per_step": 1e3 / r["served_shots_per_s"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (per-request seeded velocity; random records)",
+        "config": {"workload": r["workload"], "streams": 4, "max_inflight": 8},
+        "latency_ms": r["latency_ms"], "poisson_run_s": r["poisson_run_s"], "saturated_run_s": r["saturated_run_s"],
+        "plain_shots_per_s": r["plain_shots_per_s"], "plain_sample": r["plain_sample"],
+        "served_vs_plain": r["served_vs_plain"],
     }
     print(json.dumps(out))
 
@@ -234,7 +268,7 @@ def bench_lsm(args):
     sx = synth.config5_shots()
     p = O.WemParams(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, fmin=g.fmin, fmax=g.fmax, padt=g.padt,
                     padx=g.padx, pspi=False)
-    cores = min(16, os.cpu_count() or 1)
+    cores = host_cores()
     t0 = time.perf_counter()
     d_obs = O.shots(p, vel, wav, sx, m=refl, adj=False, f64=True)
     t_oracle = time.perf_counter() - t0
@@ -348,7 +382,7 @@ def bench_angles(args):
     }
     if not args.no_cpu:
         import oracle as O
-        cores = min(16, os.cpu_count() or 1)
+        cores = host_cores()
         os.environ.setdefault("OMP_NUM_THREADS", str(cores))
         p = O.WemParams(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, fmin=76.0, fmax=80.0, pspi=False)
         t0 = time.perf_counter()
@@ -410,7 +444,7 @@ def bench_elastic(args):
     }
     if not args.no_cpu:
         import oracle as O
-        cores = min(16, os.cpu_count() or 1)
+        cores = host_cores()
         p = O.WemParams(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, fmin=79.0, fmax=80.0, pspi=False)
         sp = seiswem.WemGeometry(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nz=g.nz, dz=g.dz, fmin=79.0, fmax=80.0,
                                  pspi=False).sampling()
@@ -441,6 +475,9 @@ def main():
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="pspi2d: shots timed one per call on the plain and resident single-shot paths (0: skip)")
     ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm", "angles", "elastic"), default="pspi2d")
+    ap.add_argument("--no-3d", action="store_true", help="pspi2d: leave out the 3-D part of the line")
+    ap.add_argument("--no-serving", action="store_true", help="pspi2d: leave out the serving part of the line")
+    ap.add_argument("--steps-3d", type=int, default=3, help="pspi2d: 3-D shots timed for the line's 3-D part")
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--plain-sample", type=int, default=24, help="serving: requests timed on the plain path")
@@ -495,7 +532,7 @@ def main():
     flops_per_launch = flops / launches
     achieved = flops_per_launch / avg_launch_s / 1e12
     packed = args.variant in (0, 6)  # automatic selection takes the packed kernel for PSPI compat
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_per_launch(nshot, "march2d_adj_packed" if packed else "march2d_adj_pair"),
                 "kernel": "march2d_adj_packed<2048,16>" if packed else "march2d_adj (PSPI, nkx=2048)",
                 "compute_unit": "FP32 vector ALU (no stage is a matrix product); peak = MI355X FP32 157.3 TFLOP/s",
@@ -516,7 +553,7 @@ def main():
     if args.plain_shots > 0:
         out.update(plain_single_shot(args, g, vel, wav, sx, d, plan, value))
     if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
+        cores = host_cores()
         os.environ.setdefault("OMP_NUM_THREADS", str(cores))
 
         def m_gpu_fn(gs, dtr):
@@ -526,6 +563,14 @@ def main():
         out["cpu_baseline"] = cb
         out["rel_l2_vs_fp64"] = err
         out["rel_l2_sample"] = f"shot 0, {nf_samp} frequency bins, full nx/nz, PSPI compat"
+    plan.close()
+    del d, m, m_true
+    # the metric's other two parts: 3-D shots/s (the 3-D configuration, one shot per step) and
+    # the serving loop's p50 / p99 latency beside its throughput
+    if not args.no_3d:
+        out["ssf3d"] = measure_ssf3d(args, steps=args.steps_3d, warmup=1)
+    if not args.no_serving:
+        out["serving"] = measure_serving(args)
     if args.copy_calibration:
         src = seiswem.DeviceArray((COPY_BYTES // 4,))
         dst = seiswem.DeviceArray((COPY_BYTES // 4,))

@@ -274,7 +274,8 @@ int swem_server_destroy(swem_server* server);
 int swem_batch_submit(swem_server* server, const swem_request* req, long long* id);
 int swem_batch_poll(swem_server* server, long long id, swem_request_status* status);
 /* Blocks until the request is done, copies its stacked image [nm][nz] into m (if not NULL),
- * fills status and forgets the request.  Returns the request's error code if it failed. */
+ * fills status and forgets the request.  Returns the request's error code if it failed.  One
+ * thread may wait for a given id: a second concurrent wait on it returns SWEM_EINVAL. */
 int swem_batch_wait(swem_server* server, long long id, float* m, swem_request_status* status);
 int swem_server_get_stats(swem_server* server, swem_server_stats* out);
 

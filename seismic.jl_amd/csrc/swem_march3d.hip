@@ -43,43 +43,56 @@ __device__ __forceinline__ double reduce_rev(double phase)
     return r - rint(r);
 }
 
+// kz phase from s = w^2 p^2 - kx^2 - ky^2 evaluated in FP64 (the cancellation near the
+// evanescent cutoff): square root, phase and its reduction to revolutions in FP64, rounded to
+// float only for the hardware sine / cosine of the reduced angle; the evanescent factor
+// exp(-0.2 sqrt(-s) |dz|) from the FP64 root (wem.c:341-343).  1/nk folded in.
+__device__ __forceinline__ float2 kz_phase_s64(const March3DArgs& a, double s)
+{
+    const double q = sqrt(fabs(s));
+    if (s >= 0.0) {
+        const float2 c = cis_rev(reduce_rev(q * a.dzd));
+        return make_float2(c.x * a.inv_nk, c.y * a.inv_nk);
+    }
+    return make_float2(expf((float)(-0.2 * q * fabs(a.dzd))) * a.inv_nk, 0.f);
+}
+
 // kz phase of one (kx, ky) point, +dz orientation, with 1/nk folded in (wem.c:336-344).
 __device__ __forceinline__ float2 kz_phase(const March3DArgs& a, double w, double p2, float p2f, int ikx, int iky)
 {
-    const int kxi = ikx <= a.nkx / 2 ? ikx : a.nkx - ikx;
-    const int kyi = iky <= a.nky / 2 ? iky : a.nky - iky;
     if (a.prec == SWEM_PREC_KZ64) {
-        // s in FP64 (the cancellation w^2 p^2 - k^2 near the evanescent cutoff), then the
-        // correctly-signed float s, whose square root and phase carry float relative error only
+        const int kxi = ikx <= a.nkx / 2 ? ikx : a.nkx - ikx;
+        const int kyi = iky <= a.nky / 2 ? iky : a.nky - iky;
         const double kx = kxi * a.dkx, ky = kyi * a.dky;
-        const float s = (float)(w * w * p2 - kx * kx - ky * ky);
-        const float dzf = (float)a.dzd;
+        return kz_phase_s64(a, w * w * p2 - kx * kx - ky * ky);
+    }
+    {
+        // the reference's float expressions (wem.c:336-343), mirrored wavenumbers included, with
+        // no contraction into fused multiply-adds
+#pragma clang fp contract(off)
+        const float wf = (float)w;
+        const float kx = ikx <= a.nkx / 2 ? a.dkxf * (float)ikx : -(a.dkxf * (float)a.nkx - a.dkxf * (float)ikx);
+        const float ky = iky <= a.nky / 2 ? a.dkyf * (float)iky : -(a.dkyf * (float)a.nky - a.dkyf * (float)iky);
+        const float s = (wf * wf) * p2f - kx * kx - ky * ky;
         if (s >= 0.f) {
-            float r = sqrtf(s) * dzf * (float)(1.0 / kTwoPi);
-            r -= rintf(r);
-            return make_float2(__builtin_amdgcn_cosf(r) * a.inv_nk, __builtin_amdgcn_sinf(r) * a.inv_nk);
+            float sn, cs;
+            sincosf(sqrtf(s) * (float)a.dzd, &sn, &cs);
+            return make_float2(cs * a.inv_nk, sn * a.inv_nk);
         }
-        return make_float2(expf(-0.2f * sqrtf(-s) * fabsf(dzf)) * a.inv_nk, 0.f);
+        return make_float2(expf(-0.2f * sqrtf(fabsf(s)) * fabsf((float)a.dzd)) * a.inv_nk, 0.f);
     }
-    // the reference's float expressions (wem.c:336-343)
-    const float wf = (float)w;
-    const float kx = a.dkxf * (float)kxi, ky = a.dkyf * (float)kyi;
-    const float s = (wf * wf) * p2f - kx * kx - ky * ky;
-    if (s >= 0.f) {
-        float sn, cs;
-        sincosf(sqrtf(s) * (float)a.dzd, &sn, &cs);
-        return make_float2(cs * a.inv_nk, sn * a.inv_nk);
-    }
-    return make_float2(expf(-0.2f * sqrtf(fabsf(s)) * fabsf((float)a.dzd)) * a.inv_nk, 0.f);
 }
 
 // split-step phase exp(i w pd dz) (+dz orientation, wem.c:326,353)
 __device__ __forceinline__ float2 ss_phase(const March3DArgs& a, double w, size_t ixy)
 {
     if (a.prec == SWEM_PREC_KZ64) return cis_rev(reduce_rev(w * a.pd[(size_t)a.iz * a.nm + ixy] * a.dzd));
-    float sn, cs;
-    sincosf((float)w * a.pdf[(size_t)a.iz * a.nm + ixy] * (float)a.dzd, &sn, &cs);
-    return make_float2(cs, sn);
+    {
+#pragma clang fp contract(off)
+        float sn, cs;
+        sincosf((float)w * a.pdf[(size_t)a.iz * a.nm + ixy] * (float)a.dzd, &sn, &cs);
+        return make_float2(cs, sn);
+    }
 }
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -195,14 +208,7 @@ __global__ __launch_bounds__(256) void march3d_init(March3DArgs a)
 __device__ __forceinline__ float2 kz_phase64(const March3DArgs& a, double c0, int ikx)
 {
     const int kxi = ikx <= a.nkx / 2 ? ikx : a.nkx - ikx;
-    const float s = (float)fma(-(double)(kxi * kxi), a.dkx2, c0);
-    const float dzf = (float)a.dzd;
-    if (s >= 0.f) {
-        float r = sqrtf(s) * dzf * (float)(1.0 / kTwoPi);
-        r -= rintf(r);
-        return make_float2(__builtin_amdgcn_cosf(r) * a.inv_nk, __builtin_amdgcn_sinf(r) * a.inv_nk);
-    }
-    return make_float2(expf(-0.2f * sqrtf(-s) * fabsf(dzf)) * a.inv_nk, 0.f);
+    return kz_phase_s64(a, fma(-(double)(kxi * kxi), a.dkx2, c0));
 }
 
 // x-pass.  Workgroup: C waves, wave w owns column ky0 + w of every stepped wavefield (NWF = 2:

@@ -44,7 +44,11 @@ __global__ __launch_bounds__(256) void build_kz_table(PhaseTableArgs a)
                 row[kk] = make_float2((float)(re * a.inv_nk), (float)(im * a.inv_nk));
             }
         } else {
-            // wem.c:336-343 in float: kx = (float)dkx*ikx, s = (w*w)*p2 - kx*kx
+            // wem.c:336-343 in float: kx = (float)dkx*ikx, s = (w*w)*p2 - kx*kx, no contraction.
+            // The row holds kk = 0..nkx/2 and serves ikx = nkx - kk by symmetry, so the mirrored
+            // wavenumbers are (float)dkx*kk rather than the reference's -((float)dkx*nkx - dkx*ikx)
+            // (they differ in the last bit of kx).
+#pragma clang fp contract(off)
             const float w = (float)iw * a.dwf;
             const float p2 = a.p2f[(size_t)iz * a.nref + ir];
             for (int kk = threadIdx.x; kk < a.nkh; kk += blockDim.x) {
@@ -79,6 +83,7 @@ __global__ __launch_bounds__(256) void build_ss_table(PhaseTableArgs a)
                 sincos((double)iw * a.dw * a.pd[(size_t)iz * a.nmx + x] * a.dz, &sn, &cs);
                 S = make_float2((float)cs, (float)sn);
             } else {
+#pragma clang fp contract(off)
                 float sn, cs;
                 sincosf((float)iw * a.dwf * a.pdf[(size_t)iz * a.nmx + x] * (float)a.dz, &sn, &cs);
                 S = make_float2(cs, sn);

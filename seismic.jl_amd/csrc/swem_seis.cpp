@@ -153,6 +153,16 @@ int swem_seis_read_header(const char* name, swem_seis_extent* fh)
         else if (key == "unit5") copy_str(fh->unit5, sval);
     }
     std::fclose(fp);
+    // Every extent must be at least 1 (callers size buffers and index the first and last trace
+    // from them), and a file of more than 2^36 samples is taken as a corrupt header.
+    const int ns[5] = {fh->n1, fh->n2, fh->n3, fh->n4, fh->n5};
+    double total = 1.0;
+    for (int i = 0; i < 5; i++) {
+        if (ns[i] < 1) return set_error(SWEM_EINVAL, "seis header %s: n%d=%d (extents must be >= 1)", name, i + 1, ns[i]);
+        total *= ns[i];
+    }
+    if (total > 68719476736.0)
+        return set_error(SWEM_EINVAL, "seis header %s: %g samples (n1..n5 product above 2^36)", name, total);
     return SWEM_OK;
 }
 
@@ -187,7 +197,9 @@ int swem_seis_write(const char* name, const float* data, const void* headers, sw
     const char* dp = std::getenv("DATAPATH");
     std::string base = dp ? std::string(dp) + name : std::string(name);
     std::string dn = base + "@data@", hn = base + "@headers@";
-    if (dn.size() >= sizeof fh->dname) return set_error(SWEM_EINVAL, "seis file name too long: %s", dn.c_str());
+    // hn ("@headers@") is the longer of the two names; both fields hold 100 bytes
+    if (dn.size() >= sizeof fh->dname || hn.size() >= sizeof fh->hname)
+        return set_error(SWEM_EINVAL, "seis file name too long: %s", hn.c_str());
     std::strcpy(fh->dname, dn.c_str());
     std::strcpy(fh->hname, hn.c_str());
     FILE* fp = std::fopen(name, "w");

@@ -19,6 +19,7 @@
 // loop itself allocates no device memory except the band bases (once per distinct band).
 #include <algorithm>
 #include <chrono>
+#include <atomic>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -143,6 +144,7 @@ struct Request {
     int nshot = 0;
     int state = SWEM_REQ_PENDING;
     int error = SWEM_OK;
+    bool claimed = false;  // a thread waits in swem_batch_wait (one waiter per request)
     std::string message;
     double t_submit = 0, t_admit = 0, t_done = 0;
     // resources while admitted
@@ -159,7 +161,7 @@ struct Request {
 // Fourier bases per band, resident for the server's lifetime (a handful of distinct bands).
 struct ServerBases : BasisCache {
     std::map<std::vector<long long>, std::unique_ptr<Bases>> entries;
-    size_t bytes = 0;
+    std::atomic<size_t> bytes{0};  // written by the worker, read by swem_server_get_stats
     const Bases* find_or_build(const swem_geometry& g, const Sampling& s, int nf) override
     {
         long long dtbits = 0;
@@ -578,6 +580,10 @@ int swem_batch_wait(swem_server* s, long long id, float* m, swem_request_status*
     auto it = s->all.find(id);
     if (it == s->all.end()) return set_error(SWEM_EINVAL, "batch_wait: unknown request %lld", id);
     Request* r = it->second.get();
+    // the waiter forgets the request when it wakes, so a second waiter would be left holding a
+    // freed request: refuse it
+    if (r->claimed) return set_error(SWEM_EINVAL, "batch_wait: request %lld already has a waiter", id);
+    r->claimed = true;
     s->cv_done.wait(lk, [r] { return r->state == SWEM_REQ_DONE || r->state == SWEM_REQ_FAILED; });
     fill_status(r, st);
     int rc = SWEM_OK;
@@ -604,7 +610,7 @@ int swem_server_get_stats(swem_server* s, swem_server_stats* out)
     out->device_bytes_used = s->dev.used();
     out->device_bytes_peak = s->dev.peak();
     out->busy_ms = s->busy_ms + (s->busy_since >= 0 ? now_ms(s->t0) - s->busy_since : 0.0);
-    out->basis_bytes = s->bases.bytes;
+    out->basis_bytes = s->bases.bytes.load();
     return SWEM_OK;
 }
 

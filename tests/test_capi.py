@@ -197,6 +197,43 @@ def test_seis_datapath_prefix(tmp_path, monkeypatch):
     assert (tmp_path / "dp" / "x.seis@data@").exists() and (tmp_path / "x.seis").exists()
 
 
+@pytest.mark.parametrize("extra", [0, 1, 2])
+def test_seis_write_rejects_header_name_past_its_field(tmp_path, monkeypatch, extra):
+    """The trace-header file name (base + "@headers@") is 3 bytes longer than the data file name
+    (base + "@data@"): a base of 91..93 bytes fits the 100-byte data field but not the header
+    field, and is refused instead of overrunning it."""
+    lib = seiswem.load()
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("DATAPATH", raising=False)
+    name = "n" * (91 + extra)
+    assert len(name + "@data@") < 100 <= len(name + "@headers@")
+    fh = _lib.SeisExtent()
+    lib.swem_seis_extent_init(ctypes.byref(fh))
+    fh.n1, fh.n2 = 2, 1
+    data = np.ones(2, np.float32)
+    rc = lib.swem_seis_write(name.encode(), data.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), None,
+                             ctypes.byref(fh))
+    assert rc == _lib.SWEM_EINVAL and "too long" in lib.swem_last_error().decode()
+    assert not os.path.exists(name)
+
+
+@pytest.mark.parametrize("key,val", [("n2", "0"), ("n1", "-3"), ("n3", "0")])
+def test_seis_header_with_empty_or_negative_extent_is_refused(tmp_path, monkeypatch, key, val):
+    """A velocity header with n2 = 0 (no traces) or a negative extent is refused with EINVAL by the
+    header reader, before any buffer is sized from it or any trace header is indexed."""
+    lib = seiswem.load()
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("DATAPATH", raising=False)
+    write_seis(lib, "vel.seis", np.full((2, 4), 2000.0), header_array(2, imx=[0, 1], mx=[0.0, 10.0]), 4, 2)
+    write_seis(lib, "wav.seis", np.zeros((1, 8)), header_array(1), 8, 1, d1=0.004)
+    txt = open("vel.seis").read().replace(f"\t{key}=", f"\t{key}={val}\n\told_{key}=")
+    open("vel.seis", "w").write(txt)
+    fh = _lib.SeisExtent()
+    assert lib.swem_seis_read_header(b"vel.seis", ctypes.byref(fh)) == _lib.SWEM_EINVAL
+    assert call_main(["adj=y", "d=d.seis", "m=m.seis", "vel=vel.seis", "wav=wav.seis"]) == _lib.SWEM_EINVAL
+    assert "extents must be >= 1" in lib.swem_last_error().decode()
+
+
 def call_main(args):
     lib = seiswem.load()
     argv = [b"0"] + [a.encode() for a in args]

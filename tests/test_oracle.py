@@ -156,6 +156,135 @@ def test_ssf_plane_wave_evanescent_and_taper():
     assert np.max(np.abs(out - expect)) <= 1e-12
 
 
+def lateral_case(nmx=64, pspi=False, mode=0, nref=4, f_bin=20, iz=1):
+    """padx = 1, nmx <= 100 (no taper, nkx = nmx): a laterally constant wavefield is the kx = 0
+    mode alone, so one step multiplies it by closed-form phases even over a laterally varying
+    velocity.  v(x) runs 2000 -> 2600 m/s, chosen off the reference-velocity ladder's steps."""
+    p = O.WemParams(nt=64, dt=0.004, nmx=nmx, dmx=10.0, nz=3, dz=10.0, fmin=0, fmax=125, padx=1, pspi=pspi,
+                    pspi_mode=mode, nref=nref)
+    x = np.arange(nmx)
+    v = (2000.0 + 600.0 * x / (nmx - 1)).astype(np.float32)
+    vel = np.repeat(v[:, None], 3, 1)
+    s = O.sampling(p)
+    w = 2 * np.pi * f_bin / (s.ntfft * float(np.float32(p.dt)))
+    u = np.full(nmx, 0.7 - 0.4j)
+    return p, vel, v.astype(np.float64), u, w
+
+
+@pytest.mark.parametrize("dzs", [10.0, -10.0])
+def test_ssf_split_step_phase_closed_form(dzs):
+    """ssop over v(x) (wem.c:313-359).  Adjoint branch: FFT, kz phase at kx = 0 (exp(i w po dz)),
+    IFFT, then the split-step factor exp(i w pd dz): a constant u leaves as u exp(i w dz / v(x)).
+    Forward branch (adj = 0, wem.c:321-326): the split-step factor comes first, so the input
+    c exp(-i w pd dz) is constant after it and leaves as c exp(i w po dz).  A flipped sign of the
+    split-step phase, a dropped pd, or po taken from another depth fails both."""
+    p, vel, v, u, w = lateral_case()
+    out = O.step(p, vel, u, 20, 1, dzs, adj=True)
+    assert np.max(np.abs(out - u * np.exp(1j * w * dzs / v))) <= 1e-12
+    po = 1.0 / np.mean(v)
+    pd = 1.0 / v - po
+    uf = u * np.exp(-1j * w * pd * dzs)
+    outf = O.step(p, vel, uf, 20, 1, dzs, adj=False)
+    assert np.max(np.abs(outf - u * np.exp(1j * w * po * dzs))) <= 1e-12
+    # the naive FP32 restatement follows within float rounding of the phase
+    out32 = O.step(p, vel, u, 20, 1, dzs, adj=True, f64=False)
+    assert np.max(np.abs(out32 - u * np.exp(1j * w * dzs / v))) <= 2e-5
+
+
+def vref_ladder(v, nref):
+    """vref[i] = vmin + i (vmax - vmin) / (nref - 1) and iref = trunc((nref - 1)(v - vmin) /
+    (vmax - vmin)), both clamped to nref - 1 above nref - 2 (wem.c:39-70)."""
+    vmin, vmax = float(v.min()), float(v.max())
+    vref = vmin + np.arange(nref) * (vmax - vmin) / (nref - 1)
+    iref = np.floor((nref - 1) * (v - vmin) / (vmax - vmin)).astype(int)
+    i1 = np.where(iref > nref - 2, nref - 1, iref)
+    i2 = np.where(iref > nref - 2, nref - 1, iref + 1)
+    return vref, i1, i2
+
+
+@pytest.mark.parametrize("dzs", [5.0, -5.0])
+def test_pspi_compat_selection_closed_form(dzs):
+    """pspiop compat (wem.c:428-459, linear_interp wem.c:473-478) on the kx = 0 mode over v(x):
+    each vref propagates the constant u to u exp(i w dz / vref), and point x keeps the result of
+    vref[iref1(x)] -- only its real part where vref2 - vref1 > 10 (every point below vmax), the
+    full complex value where iref1 == iref2 (v = vmax).  Taking iref2, Im instead of Re, or
+    skipping the cut fails."""
+    nref = 4
+    p, vel, v, u, w = lateral_case(pspi=True, mode=0, nref=nref)
+    vref, i1, i2 = vref_ladder(v, nref)
+    cut = (vref[i2] - vref[i1]) > 10
+    assert cut.sum() == v.size - 1 and not cut[-1] and np.all(i2[cut] == i1[cut] + 1)
+    y1 = u * np.exp(1j * w * dzs / vref[i1])
+    expect = np.where(cut, y1.real, y1)
+    for adj in (True, False):  # pspiop ignores adj (wem.c:380-471)
+        out = O.step(p, vel, u, 20, 1, dzs, adj=adj)
+        assert np.max(np.abs(out - expect)) <= 1e-12
+    out32 = O.step(p, vel, u, 20, 1, dzs, adj=True, f64=False)
+    assert np.max(np.abs(out32 - expect)) <= 2e-5
+
+
+def test_pspi_linear_weights_closed_form():
+    """pspi_mode 1, select after propagation (the adjoint branch): cut points take
+    (1 - w2) y1 + w2 y2 with w2 = (v - vref1) / (vref2 - vref1), the interpolation of
+    wem.c:476.  The forward branch (weight per x, propagate, sum) is pinned as the transpose of
+    the adjoint one: L is even in kx, so F^-1 L F = F L F^-1 and every factor is symmetric."""
+    nref = 4
+    p, vel, v, u, w = lateral_case(pspi=True, mode=1, nref=nref)
+    vref, i1, i2 = vref_ladder(v, nref)
+    cut = (vref[i2] - vref[i1]) > 10
+    w2 = np.where(cut, (v - vref[i1]) / np.where(cut, vref[i2] - vref[i1], 1.0), 0.0)
+    assert np.all((w2 >= 0) & (w2 < 1)) and w2.max() > 0.5
+    y1 = u * np.exp(1j * w * 10.0 / vref[i1])
+    y2 = u * np.exp(1j * w * 10.0 / vref[i2])
+    expect = np.where(cut, (1 - w2) * y1 + w2 * y2, y1)
+    out = O.step(p, vel, u, 20, 1, 10.0, adj=True)
+    assert np.max(np.abs(out - expect)) <= 1e-12
+    # forward branch = the transpose of the adjoint one: <P_fwd a, b> = <a, P_adj^T b> for the
+    # complex bilinear form, since every factor (weights, phases, unitary FFTs) is symmetric
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal(v.size) + 1j * rng.standard_normal(v.size)
+    b = rng.standard_normal(v.size) + 1j * rng.standard_normal(v.size)
+    lhs = np.sum(O.step(p, vel, a, 20, 1, 10.0, adj=False) * b)
+    rhs = np.sum(a * O.step(p, vel, b, 20, 1, 10.0, adj=True))
+    assert abs(lhs - rhs) <= 1e-12 * abs(rhs)
+
+
+def pspi_lateral_images(amp, nrefs, nx=48, nz=20, nt=128):
+    base = O.WemParams(nt=nt, dt=0.004, nmx=nx, dmx=10.0, nz=nz, dz=10.0, fmin=2, fmax=30, pspi=False)
+    z = np.arange(nz) * base.dz
+    x = np.arange(nx)[:, None] * base.dmx
+    vel = (2000 + 0.5 * z[None, :] + amp * np.sin(2 * np.pi * x / 470.0)).astype(np.float32)
+    wav = np.zeros(nt, np.float32)
+    wav[:25] = np.hanning(25)
+    d = np.random.default_rng(8).standard_normal((nx, nt))
+    m_ssf = O.wem(base, vel, wav, 240.0, d=d, adj=True)
+    ms = {n: O.wem(O.WemParams(**{**base.__dict__, "pspi": True, "pspi_mode": 1, "nref": n}), vel, wav, 240.0,
+                   d=d, adj=True) for n in nrefs}
+    return m_ssf, ms
+
+
+def test_pspi_linear_ladder_convergence():
+    """Linear PSPI (wem.c:476's interpolation) converges at second order in the ladder step while
+    the steps stay above the 10 m/s cut threshold (contrast 480 m/s, nref 3 -> 33): halving the
+    step divides the distance to the nref = 33 image by more than 2.5.  Weights swapped between
+    vref1 and vref2, or nearest-vref selection, converge at first order (factor 2) or not at all."""
+    _, ms = pspi_lateral_images(240.0, (3, 5, 9, 17, 33))
+    errs = [rel_l2(ms[n], ms[33]) for n in (3, 5, 9)]
+    assert errs[0] / errs[1] > 2.5 and errs[1] / errs[2] > 2.5
+
+
+def test_pspi_and_ssf_agree_to_first_order_in_contrast():
+    """PSPI and split-step share the phase-shift operator of v(z) and differ at first order in the
+    lateral contrast: quartering the contrast quarters their distance (ratio 3.5..4.5; the
+    cut threshold stays inactive: nref 4 over a 120+ m/s range).  A split-step correction of the
+    wrong sign or size, or a wrong PSPI reference velocity, leaves a contrast-independent part."""
+    d = []
+    for amp in (240.0, 60.0):
+        m_ssf, ms = pspi_lateral_images(amp, (4,))
+        d.append(rel_l2(ms[4], m_ssf))
+    assert 3.5 <= d[0] / d[1] <= 4.5
+
+
 def small(pspi, mode=0, lateral=True, seed=0, nx=48, nz=24, nt=128):
     p = O.WemParams(nt=nt, dt=0.004, nmx=nx, dmx=10.0, nz=nz, dz=10.0, fmin=2, fmax=50, nref=4, pspi=pspi,
                     pspi_mode=mode)
