@@ -201,6 +201,18 @@ int swem_plan_migrate(swem_plan* plan, const float* d_dev, const float* sx, cons
 /* As swem_plan_migrate but one image per shot: m_dev [nshot][nm][nz], no stacking. */
 int swem_plan_migrate_each(swem_plan* plan, const float* d_dev, const float* sx, const float* sy,
                            int nshot, float* m_dev, void* stream);
+/* No counterpart in the reference: swem_plan_migrate for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and result as swem_plan_migrate (within FP32 rounding of the frequency-chunk
+ * sums), 2-D plans only; calls no hipMalloc / hipFree.  Its work memory (source indices,
+ * receiver spectra, partial images) is a fixed 128 MiB scratch array in device memory, one per
+ * GPU, held until the process ends; shots and frequencies are launched in chunks that fit it.
+ * SWEM_ENOMEM when one shot's spectra plus one partial image exceed it (nm * nf * 8 +
+ * nz * nmx * 4 bytes above 128 MiB - 64 KiB), SWEM_EUNSUPPORTED for a 3-D plan.  Every call on
+ * a GPU shares that GPU's scratch: calls that overlap on one GPU (two host threads, or two
+ * streams in flight at once) race and must be serialised by the caller. */
+int swem_plan_migrate_noalloc(swem_plan* plan, const float* d_dev, const float* sx, const float* sy,
+                              int nshot, float* m_dev, int accumulate, void* stream);
 /* Demigration (modelling) of the image m_dev [nm][nz] into d_dev [nshot][nm][nt]. */
 int swem_plan_model(swem_plan* plan, const float* m_dev, const float* sx, const float* sy,
                     int nshot, float* d_dev, void* stream);

@@ -1103,6 +1103,89 @@ int swem_plan_migrate_each(swem_plan* p, const float* d_dev, const float* sx, co
     return migrate_impl(p, d_dev, sx, sy, nshot, m_dev, 0, 1, (hipStream_t)stream);
 }
 
+// The 2-D migration of migrate_impl with every work buffer carved out of the fixed device
+// scratch (swem_noalloc.hip) instead of allocated: source indices, receiver spectra and partial
+// images.  Shots run in chunks and the frequencies of a table chunk in sub-chunks sized so that
+// the spectra of a shot chunk and the partial images of a sub-chunk fit the scratch; each
+// sub-chunk's partial images are reduced into the image before the next launch reuses them.
+// The kernels and the reduction are migrate_impl's, so the image equals swem_plan_migrate's up
+// to the order of the frequency-chunk sums.  No timing events are recorded.
+int swem_plan_migrate_noalloc(swem_plan* p, const float* d_dev, const float* sx, const float* sy, int nshot,
+                              float* m_dev, int accumulate, void* stream)
+{
+    if (!p || !d_dev || !sx || !m_dev || nshot < 1)
+        return set_error(SWEM_EINVAL, "migrate_noalloc: null argument or nshot < 1");
+    if (p->three_d) return set_error(SWEM_EUNSUPPORTED, "migrate_noalloc: 2-D plans only (this plan is 3-D)");
+    SWEM_HIP(hipSetDevice(p->opt.device));
+    const hipStream_t st = (hipStream_t)stream;
+    void* base = nullptr;
+    size_t cap = 0;
+    SWEM_TRY(noalloc_scratch(p->opt.device, &base, &cap));
+    const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf, nmx = p->g.nmx;
+    const int F = p->F;
+    constexpr size_t kIgxBytes = size_t(64) << 10;  // source indices of one shot chunk
+    constexpr int kIgxMax = (int)(kIgxBytes / sizeof(int));
+    const auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t spec_shot = up(nm * nf * sizeof(float2));
+    const size_t part_group = nz * nmx * sizeof(float);
+    const size_t avail = cap - kIgxBytes;
+    if (spec_shot + part_group > avail)
+        return set_error(SWEM_ENOMEM,
+                         "migrate_noalloc: one shot needs %zu bytes of work memory (spectra %zu, one partial image "
+                         "%zu), above the fixed scratch of %zu bytes",
+                         spec_shot + part_group, spec_shot, part_group, avail);
+    // shots per chunk and groups per launch: the most workgroups per launch the scratch allows
+    const int gtot = (p->nfc + F - 1) / F;
+    int ns = 1, ngr = 1;
+    long long best = 0;
+    const int ns_max = (int)std::min<size_t>({(size_t)nshot, (size_t)kIgxMax, avail / (spec_shot + part_group)});
+    for (int n = 1; n <= ns_max; n++) {
+        const int g = (int)std::min<size_t>(gtot, (avail / n - spec_shot) / part_group);
+        if (g >= 1 && (long long)n * g >= best) best = (long long)n * g, ns = n, ngr = g;
+    }
+    char* sc = static_cast<char*>(base);
+    int* igx_dev = reinterpret_cast<int*>(sc);
+    float2* spec = reinterpret_cast<float2*>(sc + kIgxBytes);
+    float* part = reinterpret_cast<float*>(sc + kIgxBytes + (size_t)ns * spec_shot);
+    int host_igx[kIntBlock];
+    for (int s0 = 0; s0 < nshot; s0 += ns) {
+        const int nsc = std::min(ns, nshot - s0);
+        // source indices, written by a kernel whose arguments carry them (stream-ordered after the
+        // previous chunk's kernels, which read the same scratch)
+        for (int i0 = 0; i0 < nsc; i0 += kIntBlock) {
+            const int n = std::min(kIntBlock, nsc - i0);
+            for (int i = 0; i < n; i++) {
+                int ix, iy;
+                SWEM_TRY(source_index(p->g, sx[s0 + i0 + i], sy ? sy[s0 + i0 + i] : 0.f, ix, iy));
+                host_igx[i] = ix * p->g.nmy + iy;
+            }
+            SWEM_TRY(launch_fill_ints(igx_dev + i0, host_igx, n, st));
+        }
+        SWEM_TRY(launch_sgemm(nsc * (int)nm, 2 * (int)nf, (int)nt, d_dev + (size_t)s0 * nm * nt, (int)nt, p->basis_adj,
+                              2 * (int)nf, reinterpret_cast<float*>(spec), 2 * (int)nf, st));
+        for (int c = 0; c < p->nchunks; c++) {
+            if (p->resident != c) SWEM_TRY(ensure_chunk(p, c, st));
+            const March2DArgs a0 = base_args(p, c, F);
+            for (int j0 = 0; j0 < a0.nfc; j0 += ngr * F) {
+                March2DArgs a = a0;
+                a.nfc = std::min(ngr * F, a0.nfc - j0);
+                a.f0 = a0.f0 + j0;
+                a.ngroups = (a.nfc + F - 1) / F;
+                a.ktab = a0.ktab + (size_t)j0 * nz * a.nref * a.nkh;
+                if (a0.stab) a.stab = a0.stab + (size_t)j0 * nz * nmx;
+                a.nshot = nsc;
+                a.igx = igx_dev;
+                a.spec_r = spec;
+                a.partial = part;
+                SWEM_TRY(launch_march2d(a, p->nkx, F, mode_of(p), 1, adj_variant(p), st));
+                const int acc = accumulate || s0 > 0 || c > 0 || j0 > 0;
+                SWEM_TRY(launch_reduce_image(part, nsc, a.ngroups, (int)nz, (int)nmx, m_dev, acc, 0, st));
+            }
+        }
+    }
+    return SWEM_OK;
+}
+
 int swem_plan_model(swem_plan* p, const float* m_dev, const float* sx, const float* sy, int nshot, float* d_dev,
                     void* stream)
 {

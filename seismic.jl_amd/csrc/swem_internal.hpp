@@ -205,6 +205,15 @@ int launch_sgemm(int M, int Nc, int K, const float* A, int lda, const float* B, 
 // badj, bsrc [nt][2 nf], bfwd [2 nf][nt] (f_op with the w^2 and 1/sqrt(ntfft) factors).
 int launch_build_bases(int nt, int nf, int n, int ifmin, double rnt, const double* tab, float* badj, float* bsrc,
                        float* bfwd, hipStream_t st);
+// Fixed device scratch of the allocation-free migration (swem_noalloc.hip): one array per device,
+// kNoallocScratchBytes long, whose device address for `device` is returned in *ptr (taken once
+// per device with hipGetSymbolAddress).
+constexpr size_t kNoallocScratchBytes = size_t(128) << 20;
+constexpr int kMaxDevices = 64;
+int noalloc_scratch(int device, void** ptr, size_t* bytes);
+// dst[0..n) = vals[0..n) on stream st through kernel arguments (kIntBlock values per launch).
+constexpr int kIntBlock = 256;
+int launch_fill_ints(int* dst, const int* vals, int n, hipStream_t st);
 int launch_reduce_image(const float* partial, int nshot, int ngroups, int nz, int nmx, float* out, int accumulate,
                         int per_shot, hipStream_t st);
 // Solver vector operations: m1 = a m1 + b m2; <x, y> in double into part[kDotBlocks] (part has

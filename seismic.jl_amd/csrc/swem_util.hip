@@ -286,4 +286,27 @@ int launch_build_bases(int nt, int nf, int n, int ifmin, double rnt, const doubl
     return SWEM_OK;
 }
 
+// Integers written to device memory from a kernel argument (the values travel by value with
+// the launch, so the host array may be reused as soon as the launch returns).
+struct IntBlock {
+    int v[kIntBlock];
+};
+
+__global__ __launch_bounds__(kIntBlock) void fill_ints(int* dst, int n, IntBlock b)
+{
+    if ((int)threadIdx.x < n) dst[threadIdx.x] = b.v[threadIdx.x];
+}
+
+int launch_fill_ints(int* dst, const int* vals, int n, hipStream_t st)
+{
+    for (int i0 = 0; i0 < n; i0 += kIntBlock) {
+        IntBlock b{};
+        const int m = n - i0 < kIntBlock ? n - i0 : kIntBlock;
+        for (int i = 0; i < m; i++) b.v[i] = vals[i0 + i];
+        hipLaunchKernelGGL(fill_ints, dim3(1), dim3(kIntBlock), 0, st, dst + i0, m, b);
+        SWEM_HIP(hipGetLastError());
+    }
+    return SWEM_OK;
+}
+
 }  // namespace swem

@@ -24,7 +24,7 @@ VARIANT_AUTO, VARIANT_SUBGROUP, VARIANT_WAVE, VARIANT_SUBGROUP16, VARIANT_PAIR, 
 EXPORTS = (
     "swem_version", "swem_last_error", "swem_options_init", "swem_device_count",
     "swem_shot_main", "swem_angles_main", "swem_eshot_main", "swem_wem", "swem_wem_ex", "swem_shots", "swem_compute_angles", "swem_compute_angles_ex", "swem_ewem", "swem_eshots",
-    "swem_plan_create", "swem_plan_destroy", "swem_plan_migrate", "swem_plan_migrate_each",
+    "swem_plan_create", "swem_plan_destroy", "swem_plan_migrate", "swem_plan_migrate_each", "swem_plan_migrate_noalloc",
     "swem_plan_model", "swem_plan_timing", "swem_plan_autotune", "swem_plan_bytes",
     "swem_malloc", "swem_free", "swem_memcpy_htod", "swem_memcpy_dtoh", "swem_memcpy_dtod", "swem_memset", "swem_synchronize",
     "swem_sampling", "swem_source_index", "swem_pspi_tables", "swem_taper",
@@ -163,6 +163,7 @@ def load():
         "swem_plan_create": ([gp, f32p, f32p, op_, P(vp)], ctypes.c_int),
         "swem_plan_destroy": ([vp], ctypes.c_int),
         "swem_plan_migrate": ([vp, vp, f32p, f32p, ctypes.c_int, vp, ctypes.c_int, vp], ctypes.c_int),
+        "swem_plan_migrate_noalloc": ([vp, vp, f32p, f32p, ctypes.c_int, vp, ctypes.c_int, vp], ctypes.c_int),
         "swem_plan_migrate_each": ([vp, vp, f32p, f32p, ctypes.c_int, vp, vp], ctypes.c_int),
         "swem_plan_model": ([vp, vp, f32p, f32p, ctypes.c_int, vp, vp], ctypes.c_int),
         "swem_plan_timing": ([vp, P(Timing)], ctypes.c_int),
