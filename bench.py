@@ -93,7 +93,7 @@ def traffic_per_launch(nshot, kernel):
     return (fetch + write) * 1024
 
 
-def measure_ssf3d(args, steps, warmup, grid=256, nz=0, dot=True):
+def measure_ssf3d(args, steps, warmup, grid=256, nz=0, dot=True, graphs=True):
     """3-D split-step migration of one shot (nx=ny=256, dx=dy=12.5 m; nz=300, dz=10 m; nt=1500,
     dt=4 ms; 3-50 Hz = 565 bins; nkx=nky=512).  HBM-bound: algorithmic bytes per (w, iz) are
     64 nk (two wavefields x two passes x read+write of the padded complex slice).  Returns the
@@ -106,6 +106,7 @@ def measure_ssf3d(args, steps, warmup, grid=256, nz=0, dot=True):
         g = dataclasses.replace(g, nz=nz)
         vel, refl = np.ascontiguousarray(vel[:, :nz]), np.ascontiguousarray(refl[:, :nz])
     plan = seiswem.Plan(g, vel, wav)
+    plan.set_graphs(graphs)
     mm = seiswem.DeviceArray(refl.shape, refl)
     d = plan.model(mm, sx, sy)
     seiswem.synchronize()
@@ -124,7 +125,23 @@ def measure_ssf3d(args, steps, warmup, grid=256, nz=0, dot=True):
     nf, nk = s["ifmax"] - s["ifmin"], s["nkx"] * s["nky"]
     byts = 64.0 * nk * nf * g.nz
     achieved = byts / (march_ms / steps / 1e3) / 1e9
+    # with graphs, the same shots again with the kernels launched one by one (without graphs no
+    # graph is captured at all: profilers run the one-by-one path alone)
+    elapsed_eager = elapsed
+    if graphs:
+        plan.set_graphs(False)
+        plan.migrate(d, sx, sy, m=m)
+        seiswem.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            plan.migrate(d, sx, sy, m=m)
+        seiswem.synchronize()
+        elapsed_eager = time.perf_counter() - t1
+        plan.set_graphs(True)
     out = {"shots_per_s": steps / elapsed, "ms_per_shot": elapsed / steps * 1e3, "steps": steps, "warmup": warmup,
+           "launch": "one captured graph per shot" if graphs else "kernels launched one by one",
+           "ms_per_shot_graph": elapsed / steps * 1e3 if graphs else None,
+           "ms_per_shot_launched_one_by_one": elapsed_eager / steps * 1e3,
            "workload": f"3-D SSF migration, 1 shot, nx=ny={g.nmx} nz={g.nz} nt=1500 3-50 Hz",
            "nkx": s["nkx"], "nky": s["nky"], "nf": nf, "nz": g.nz,
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -145,14 +162,17 @@ def measure_ssf3d(args, steps, warmup, grid=256, nz=0, dot=True):
 
 
 def bench_ssf3d(args):
-    r = measure_ssf3d(args, args.steps, args.warmup, grid=args.grid, nz=args.nz, dot=not args.no_dot)
+    r = measure_ssf3d(args, args.steps, args.warmup, grid=args.grid, nz=args.nz, dot=not args.no_dot,
+                      graphs=not args.no_graphs)
     out = {
         "metric": "migrated shots/sec (3D split-step) at <=1e-4 rel-L2 vs FP64", "value": r["shots_per_s"],
         "unit": "shots/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": r["ms_per_shot"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (3-D generator; record by the library's own demigration)",
         "config": {"workload": r["workload"], "nkx": r["nkx"], "nky": r["nky"], "nf": r["nf"], "nz": r["nz"]},
-        "roofline": r["roofline"], "dot_test_rel": r["dot_test_rel"],
+        "roofline": r["roofline"], "dot_test_rel": r["dot_test_rel"], "launch": r["launch"],
+        "ms_per_shot_launched_one_by_one": r["ms_per_shot_launched_one_by_one"],
+        "ms_per_shot_graph": r["ms_per_shot_graph"],
     }
     if args.nz:
         out["config"]["workload"] += f" (first {args.nz} depths only)"
@@ -469,6 +489,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nz", type=int, default=0, help="ssf3d: march only the first NZ depths (profiling)")
     ap.add_argument("--no-dot", action="store_true", help="ssf3d: skip the full-size dot test")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="ssf3d: launch each shot's kernels one by one (the timed steps) instead of one captured graph")
     ap.add_argument("--grid", type=int, default=256,
                     help="ssf3d: nx = ny of the 3-D grid (256: the 3-D configuration; others, e.g. 250, take "
                          "the Bluestein kernels)")

@@ -152,8 +152,9 @@ int swem_compute_angles_ex(float* angx, float* angy, const float* wav, const swe
  * source, the P / S separation of the receiver components at every depth and split-step
  * propagation with vp and vs (ewem.c:255-741).  2-D (nmy = 1); as in the reference, uy is not
  * used (migration) and returned zero (modelling), and mps1 is returned zero and not used.
- * Returns SWEM_EUNSUPPORTED when ewem.c's frequency band (its fmax / fmin truncation,
- * ewem.c:46-57) differs from wem.c's for the geometry. */
+ * The band is sampled as ewem.c samples it (ewem.c:43-52: an even ntfft = 2 truncf(padt nt / 2),
+ * ifmax = trunc(fmax dt ntfft) + 1), which differs from wem.c's rule for odd padt nt or a
+ * fractional fmax / fmin.  3-D (nmy > 1) returns SWEM_EUNSUPPORTED. */
 int swem_ewem(float* ux, float* uy, float* uz, float* mpp, float* mps1, float* mps2, const float* wav,
               int nt, float ot, float dt,
               int nmx, float omx, float dmx,
@@ -219,12 +220,38 @@ int swem_plan_model(swem_plan* plan, const float* m_dev, const float* sx, const 
 /* Timing of the last migrate/model call on this plan (synchronises its stream). */
 int swem_plan_timing(swem_plan* plan, swem_timing* out);
 
-/* Measured kernel choice for a 2-D plan (no reference counterpart): every migration kernel variant
- * that fits the plan's nkx is timed on nshot shots of zero records (the work does not depend on the
- * data), and the fastest becomes the plan's variant for later migrations; *best receives it and
- * times[0..SWEM_VARIANT_PACKED] each variant's march time in ms (0: not a distinct candidate; may
- * be NULL).  Takes two migrations per candidate; synchronous.  3-D plans: SWEM_EUNSUPPORTED. */
+/* Measured launch shapes of a plan (no reference counterpart; the reference plans FFTW with
+ * FFTW_ESTIMATE, wem.c:103-104).  2-D: every migration kernel variant that fits the plan's nkx,
+ * each with every frequency group size (frequencies per workgroup, 1..16) its workgroup and LDS
+ * allow, is timed on nshot shots of zero records (the work does not depend on the data).  3-D:
+ * frequencies per y-pass workgroup (2..16, 8 at most for PSPI or a Bluestein y axis) times the step
+ * state of one launch (64, 128, 256 MiB, i.e. frequencies marched together against the 256 MB
+ * last-level cache), timed on the plan's first 16 depths.  The fastest candidate becomes the
+ * plan's shape for later migrations (a freqs_per_block given at plan creation is kept).  Each
+ * candidate takes two migrations; synchronous. */
+#define SWEM_TUNE_MAX 48
+typedef struct swem_tune_result {
+    int variant;            /* 2-D: chosen SWEM_VARIANT_*; 3-D: SWEM_VARIANT_AUTO */
+    int freqs_per_block;    /* chosen frequencies per workgroup (3-D: per y-pass workgroup) */
+    size_t chunk_bytes;     /* 3-D: chosen step state of one launch; 2-D: 0 */
+    float best_ms;          /* march time of the choice (3-D: of the first 16 depths) */
+    int ncand;              /* candidates timed */
+    int cand_variant[SWEM_TUNE_MAX], cand_fpb[SWEM_TUNE_MAX];
+    size_t cand_chunk_bytes[SWEM_TUNE_MAX];
+    float cand_ms[SWEM_TUNE_MAX];
+} swem_tune_result;
+int swem_plan_autotune_ex(swem_plan* plan, int nshot, swem_tune_result* out);
+/* The 2-D variant choice of swem_plan_autotune_ex: *best receives the chosen variant and
+ * times[0..SWEM_VARIANT_PACKED] each variant's march time in ms at its best group size (0: not a
+ * distinct candidate; may be NULL).  A 3-D plan is tuned as by swem_plan_autotune_ex, *best =
+ * SWEM_VARIANT_AUTO and times all 0. */
 int swem_plan_autotune(swem_plan* plan, int nshot, int* best, float* times);
+/* No reference counterpart.  enable != 0 (the default): a 3-D migration launches each shot as one
+ * captured graph (every frequency chunk and depth: x-pass, y-pass, depth image), captured on the
+ * plan's first 3-D migration and again whenever its work buffers or launch shapes change, replayed
+ * for every later shot; the shot's source index and output are read from device memory.
+ * enable = 0 launches the same kernels one by one.  2-D plans ignore it. */
+int swem_plan_set_graphs(swem_plan* plan, int enable);
 /* Device bytes held by the plan's tables and its largest work chunk. */
 int swem_plan_bytes(const swem_plan* plan, size_t* table_bytes, size_t* work_bytes);
 

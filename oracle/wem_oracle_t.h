@@ -816,38 +816,46 @@ void SFX(esep_k)(RT kx, RT w, RT vp, RT vs, int sep, int adj, const RT* in1, con
     o2[0] = MCREAL(r2); o2[1] = MCREAL(-r2 * (RT _Complex)_Complex_I);
 }
 
-/* Separation / combination of a wavefield pair on the x grid (2-D): transform both (unitary),
- * apply esep_k per kx (kx = dkx ikx for ikx < nkx/2, else -(dkx nkx - dkx ikx), ewem.c:593), and
- * transform back; entries at x >= nmx are not written. */
+/* Separation / combination of a wavefield pair (elastic_separate_3d, ewem.c:553-741): pad both
+ * to the nkx x nky grid, transform (2-D when nmy > 1, unitary), apply esep_k per wavenumber with
+ * the matrix of kx alone (kx = dkx ikx for ikx < nkx/2, else -(dkx nkx - dkx ikx), ewem.c:593; the
+ * reference's matrix does not depend on ky), transform back and truncate to nmx x nmy. */
 static void SFX(esep)(const SFX(ctx)* c, CT* i1, CT* i2, CT* o1, CT* o2, RT w, RT vp, RT vs, int sep, int adj,
                       SFX(work)* wk, CT* b1, CT* b2)
 {
-    int nmx = c->g->nmx, nkx = c->s.nkx;
+    int nmx = c->g->nmx, nmy = c->g->nmy, nkx = c->s.nkx, nky = c->s.nky;
 #if ORC_IS_F64
-    RT rnk = 1.0 / sqrt((double)nkx), dkx = 2.0 * M_PI / ((double)nkx * c->g->dmx);
+    RT rnk = 1.0 / sqrt((double)nkx * nky), dkx = 2.0 * M_PI / ((double)nkx * c->g->dmx);
 #else
-    RT rnk = 1.0f / sqrtf((float)nkx), dkx = c->s.dkx;
+    RT rnk = 1.0f / sqrtf((float)nkx * nky), dkx = c->s.dkx;
 #endif
-    for (int ix = 0; ix < nkx; ix++) {
-        b1[ix] = ix < nmx ? i1[ix] : 0;
-        b2[ix] = ix < nmx ? i2[ix] : 0;
-    }
-    SFX(fft1)(b1, nkx, c->twx, -1, wk->col);
-    SFX(fft1)(b2, nkx, c->twx, -1, wk->col);
+    for (int ix = 0; ix < nkx; ix++)
+        for (int iy = 0; iy < nky; iy++) {
+            size_t k = (size_t)ix * nky + iy, x = (size_t)ix * nmy + iy;
+            b1[k] = (ix < nmx && iy < nmy) ? i1[x] : 0;
+            b2[k] = (ix < nmx && iy < nmy) ? i2[x] : 0;
+        }
+    SFX(fft2)(c, b1, -1, wk);
+    SFX(fft2)(c, b2, -1, wk);
     for (int ik = 0; ik < nkx; ik++) {
         RT kx = ik < nkx / 2.0 ? dkx * ik : -(dkx * nkx - dkx * ik);
-        CT a1 = b1[ik] * rnk, a2 = b2[ik] * rnk;
-        RT r1[2], r2[2];
-        SFX(esep_k)(kx, w, vp, vs, sep, adj, (RT*)&a1, (RT*)&a2, r1, r2);
-        b1[ik] = r1[0] + r1[1] * (RT _Complex)_Complex_I;
-        b2[ik] = r2[0] + r2[1] * (RT _Complex)_Complex_I;
+        for (int iy = 0; iy < nky; iy++) {
+            size_t k = (size_t)ik * nky + iy;
+            CT a1 = b1[k] * rnk, a2 = b2[k] * rnk;
+            RT r1[2], r2[2];
+            SFX(esep_k)(kx, w, vp, vs, sep, adj, (RT*)&a1, (RT*)&a2, r1, r2);
+            b1[k] = r1[0] + r1[1] * (RT _Complex)_Complex_I;
+            b2[k] = r2[0] + r2[1] * (RT _Complex)_Complex_I;
+        }
     }
-    SFX(fft1)(b1, nkx, c->twx, +1, wk->col);
-    SFX(fft1)(b2, nkx, c->twx, +1, wk->col);
-    for (int ix = 0; ix < nmx; ix++) {
-        o1[ix] = b1[ix] * rnk;
-        o2[ix] = b2[ix] * rnk;
-    }
+    SFX(fft2)(c, b1, +1, wk);
+    SFX(fft2)(c, b2, +1, wk);
+    for (int ix = 0; ix < nmx; ix++)
+        for (int iy = 0; iy < nmy; iy++) {
+            size_t k = (size_t)ix * nky + iy, x = (size_t)ix * nmy + iy;
+            o1[x] = b1[k] * rnk;
+            o2[x] = b2[k] * rnk;
+        }
 }
 
 /* The sampling of ewem (ewem.c:46-57): ntfft = 2 trunc(padt nt / 2), ifmax = trunc(fmax dt ntfft)
@@ -865,17 +873,17 @@ static void SFX(ewem_sampling)(const orc_geom* g, orc_sampling* s)
     s->ifmin = t + 1 < (float)s->ifmax ? (int)trunc(t) : 0;
 }
 
-/* ewem (ewem.c:3-253) and elastic_extrap1f (ewem.c:255-366), 2-D: adj maps the components
- * ux, uz [nm][nt] to the images mpp, mps2 [nm][nz] (mps1 stays zero: the reference never forms
- * the first S potential); !adj maps mpp, mps2 to ux, uz (uy stays zero).  The source (vp) is
- * normalised by max|wav| / sqrt(ntfft) (ewem.c:146); the receiver spectra carry no w^2. */
+/* ewem (ewem.c:3-253) and elastic_extrap1f (ewem.c:255-366), 2-D and 3-D: adj maps the
+ * components ux, uz [nm][nt] to the images mpp, mps2 [nm][nz] (mps1 stays zero: the reference
+ * never forms the first S potential, and uy is not used); !adj maps mpp, mps2 to ux, uz (uy stays
+ * zero).  The source (vp) is injected at igx nmy + igy (ewem.c:99-108) and normalised by
+ * max|wav| / sqrt(ntfft) (ewem.c:146); the receiver spectra carry no w^2. */
 int SFX(ewem)(RT* ux, RT* uz, RT* mpp, RT* mps2, const float* wav, const float* vp, const float* vs,
               const orc_geom* g0, float sx, float sy, int adj)
 {
     orc_geom ga = *g0;
     ga.pspi = 0;
     const orc_geom* g = &ga;
-    if (g->nmy != 1) return ORC_EINVAL;
     SFX(ctx) cp, cs;
     int rc = SFX(ctx_init)(&cp, g, vp);
     if (rc) return rc;
@@ -884,12 +892,13 @@ int SFX(ewem)(RT* ux, RT* uz, RT* mpp, RT* mps2, const float* wav, const float* 
     SFX(ewem_sampling)(g, &cp.s);
     SFX(ewem_sampling)(g, &cs.s);
     const orc_sampling* s = &cp.s;
-    const int nmx = g->nmx, nz = g->nz, nt = g->nt, nw = s->nw, nkx = s->nkx;
+    const int nmx = g->nmx, nmy = g->nmy, nm = nmx * nmy, nz = g->nz, nt = g->nt, nw = s->nw, nk = s->nkx * s->nky;
     const int ifmin = s->ifmin, ifmax = s->ifmax;
     const int igx = (int)truncf((sx - g->omx) / g->dmx);
-    if (igx < 0 || igx >= nmx || s->fop_n < nt) { SFX(ctx_free)(&cp); SFX(ctx_free)(&cs); return ORC_EINVAL; }
-    if (adj) { memset(mpp, 0, sizeof(RT) * (size_t)nmx * nz); memset(mps2, 0, sizeof(RT) * (size_t)nmx * nz); }
-    else { memset(ux, 0, sizeof(RT) * (size_t)nmx * nt); memset(uz, 0, sizeof(RT) * (size_t)nmx * nt); }
+    const int igy = nmy > 1 ? (int)truncf((sy - g->omy) / g->dmy) : 0;  /* ewem.c:99-100 */
+    if (igx < 0 || igx >= nmx || igy < 0 || igy >= nmy || s->fop_n < nt) { SFX(ctx_free)(&cp); SFX(ctx_free)(&cs); return ORC_EINVAL; }
+    if (adj) { memset(mpp, 0, sizeof(RT) * (size_t)nm * nz); memset(mps2, 0, sizeof(RT) * (size_t)nm * nz); }
+    else { memset(ux, 0, sizeof(RT) * (size_t)nm * nt); memset(uz, 0, sizeof(RT) * (size_t)nm * nt); }
 #if ORC_IS_F64
     const RT rnt = 1.0 / sqrt((double)s->ntfft);
 #else
@@ -904,16 +913,16 @@ int SFX(ewem)(RT* ux, RT* uz, RT* mpp, RT* mps2, const float* wav, const float* 
     CT* ws = (CT*)malloc(sizeof(CT) * nw);
     CT* dw = (CT*)malloc(sizeof(CT) * nw);
     RT* tr = (RT*)malloc(sizeof(RT) * nt);
-    CT* gx = (CT*)calloc((size_t)nmx * (nf ? nf : 1), sizeof(CT));
-    CT* gz = (CT*)calloc((size_t)nmx * (nf ? nf : 1), sizeof(CT));
-    CT* us = (CT*)malloc(sizeof(CT) * nmx);
-    CT* rx = (CT*)malloc(sizeof(CT) * nmx);
-    CT* rz = (CT*)malloc(sizeof(CT) * nmx);
-    CT* up = (CT*)malloc(sizeof(CT) * nmx);
-    CT* u2 = (CT*)malloc(sizeof(CT) * nmx);
-    CT* smig = (CT*)malloc(sizeof(CT) * (size_t)nmx * nz);
-    CT* b1 = (CT*)malloc(sizeof(CT) * nkx);
-    CT* b2 = (CT*)malloc(sizeof(CT) * nkx);
+    CT* gx = (CT*)calloc((size_t)nm * (nf ? nf : 1), sizeof(CT));
+    CT* gz = (CT*)calloc((size_t)nm * (nf ? nf : 1), sizeof(CT));
+    CT* us = (CT*)malloc(sizeof(CT) * nm);
+    CT* rx = (CT*)malloc(sizeof(CT) * nm);
+    CT* rz = (CT*)malloc(sizeof(CT) * nm);
+    CT* up = (CT*)malloc(sizeof(CT) * nm);
+    CT* u2 = (CT*)malloc(sizeof(CT) * nm);
+    CT* smig = (CT*)malloc(sizeof(CT) * (size_t)nm * nz);
+    CT* b1 = (CT*)malloc(sizeof(CT) * nk);
+    CT* b2 = (CT*)malloc(sizeof(CT) * nk);
     RT* vp0 = (RT*)malloc(sizeof(RT) * nz);
     RT* vs0 = (RT*)malloc(sizeof(RT) * nz);
     SFX(work) wk;
@@ -925,7 +934,7 @@ int SFX(ewem)(RT* ux, RT* uz, RT* mpp, RT* mps2, const float* wav, const float* 
         for (int it = 0; it < nt; it++) tr[it] = wav[it];
         SFX(fop)((RT*)ws, tr, nw, nt, 1);
         if (adj)
-            for (int ix = 0; ix < nmx; ix++) {
+            for (int ix = 0; ix < nm; ix++) {
                 for (int it = 0; it < nt; it++) tr[it] = ux[(size_t)ix * nt + it];
                 SFX(fop)((RT*)dw, tr, nw, nt, 1);
                 for (int iw = ifmin; iw < ifmax; iw++) gx[(size_t)ix * nf + iw - ifmin] = dw[iw];
@@ -935,18 +944,18 @@ int SFX(ewem)(RT* ux, RT* uz, RT* mpp, RT* mps2, const float* wav, const float* 
             }
         for (int iw = ifmin; iw < ifmax; iw++) {
             const RT w = SFX(omega)(&cp, iw), factor = iw == 0 ? 1 : 2;
-            for (int x = 0; x < nmx; x++) us[x] = 0;
-            us[igx] = ws[iw] * rnt;
+            for (int x = 0; x < nm; x++) us[x] = 0;
+            us[(size_t)igx * nmy + igy] = ws[iw] * rnt;
             for (int iz = 0; iz < nz; iz++) {  /* source, ewem.c:302-310 */
                 float z = g->oz + g->dz * iz;
                 if (z >= g->sz) {
                     SFX(ssop)(&cp, us, w, iz, (RT)(-g->dz), 1, &wk);
-                    for (int x = 0; x < nmx; x++) smig[(size_t)x * nz + iz] = us[x] / maxs;
+                    for (int x = 0; x < nm; x++) smig[(size_t)x * nz + iz] = us[x] / maxs;
                 } else
-                    for (int x = 0; x < nmx; x++) smig[(size_t)x * nz + iz] = 0;
+                    for (int x = 0; x < nm; x++) smig[(size_t)x * nz + iz] = 0;
             }
             if (adj) {  /* ewem.c:311-327 */
-                for (int x = 0; x < nmx; x++) {
+                for (int x = 0; x < nm; x++) {
                     rx[x] = gx[(size_t)x * nf + iw - ifmin] * rnt;
                     rz[x] = gz[(size_t)x * nf + iw - ifmin] * rnt;
                 }
@@ -957,19 +966,19 @@ int SFX(ewem)(RT* ux, RT* uz, RT* mpp, RT* mps2, const float* wav, const float* 
                     SFX(ssop)(&cp, up, w, iz, (RT)g->dz, 1, &wk);
                     SFX(ssop)(&cs, u2, w, iz, (RT)g->dz, 1, &wk);
                     SFX(esep)(&cp, up, u2, rx, rz, w, vp0[iz], vs0[iz], 0, 1, &wk, b1, b2);
-                    for (int x = 0; x < nmx; x++) {
+                    for (int x = 0; x < nm; x++) {
                         CT sm = MCONJ(smig[(size_t)x * nz + iz]);
                         mpp[(size_t)x * nz + iz] += factor * MCREAL(sm * up[x]);
                         mps2[(size_t)x * nz + iz] += factor * MCREAL(sm * u2[x]);
                     }
                 }
             } else {  /* ewem.c:328-346 */
-                for (int x = 0; x < nmx; x++) rx[x] = rz[x] = 0;
+                for (int x = 0; x < nm; x++) rx[x] = rz[x] = 0;
                 for (int iz = nz - 1; iz >= 0; iz--) {
                     float z = g->oz + g->dz * iz;
                     if (!(z >= g->gz)) continue;
                     SFX(esep)(&cp, rx, rz, up, u2, w, vp0[iz], vs0[iz], 1, 0, &wk, b1, b2);
-                    for (int x = 0; x < nmx; x++) {
+                    for (int x = 0; x < nm; x++) {
                         up[x] += smig[(size_t)x * nz + iz] * mpp[(size_t)x * nz + iz];
                         u2[x] += smig[(size_t)x * nz + iz] * mps2[(size_t)x * nz + iz];
                     }
@@ -977,14 +986,14 @@ int SFX(ewem)(RT* ux, RT* uz, RT* mpp, RT* mps2, const float* wav, const float* 
                     SFX(ssop)(&cs, u2, w, iz, (RT)(-g->dz), 0, &wk);
                     SFX(esep)(&cp, up, u2, rx, rz, w, vp0[iz], vs0[iz], 0, 0, &wk, b1, b2);
                 }
-                for (int x = 0; x < nmx; x++) {
+                for (int x = 0; x < nm; x++) {
                     gx[(size_t)x * nf + iw - ifmin] = rx[x] * rnt;
                     gz[(size_t)x * nf + iw - ifmin] = rz[x] * rnt;
                 }
             }
         }
         if (!adj)  /* ewem.c:219-241: c2r of the band, no w^2 */
-            for (int ix = 0; ix < nmx; ix++)
+            for (int ix = 0; ix < nm; ix++)
                 for (int k = 0; k < 2; k++) {
                     CT* src = k == 0 ? gx : gz;
                     RT* out = k == 0 ? ux : uz;

@@ -654,6 +654,9 @@ static March3DArgs base3(const swem_plan* p)
     a.sel = p->sel.p;
     a.used = p->used.p;
     a.R = p->R.p;
+    a.linear = p->g.pspi && p->opt.pspi_mode == SWEM_PSPI_LINEAR;
+    a.sel2 = p->sel2.p;
+    a.w2 = p->w2.p;
     return a;
 }
 
@@ -669,21 +672,89 @@ static bool stepped_at(const swem_plan* p, int iz, float level)
     return iz >= 0 && iz < p->g.nz && z >= level;
 }
 
-// Step state of one 3-D launch: at most this many bytes, so that a launch's wavefields stay in
-// the 256 MB last-level (Infinity) cache between the x-pass and the y-pass of a depth step.
-constexpr size_t kState3DBytes = size_t(128) << 20;
-
-// Frequencies per 3-D chunk: a multiple of the y-pass group whose step state fits the cache
-// bound (measured: 64 frequencies of a 512 x 512 shot run 6-11 % faster per frequency than all
-// 565 at once) and the work budget.
+// Step state of one 3-D launch: at most p->state3d_bytes (default 128 MiB), so that a launch's
+// wavefields stay in the 256 MB last-level (Infinity) cache between the x-pass and the y-pass of
+// a depth step.
 static int chunk3(const swem_plan* p, size_t per_item, int F)
 {
     const size_t state_item = 2 * (size_t)p->g.nmx * p->s.nky * sizeof(float2);
     size_t n = std::max<size_t>(1, p->budget / std::max<size_t>(per_item, 1));
-    n = std::min(n, std::max<size_t>(1, kState3DBytes / state_item));
+    n = std::min(n, std::max<size_t>(1, p->state3d_bytes / state_item));
     if (n >= (size_t)p->nf) return p->nf;
     n = std::max<size_t>(F, (n / F) * F);
     return (int)std::min<size_t>(n, p->nf);
+}
+
+// The launches of one shot's 3-D migration: every frequency chunk of nfc frequencies, the
+// initial wavefields, and per depth the x-pass, the y-pass (imaging) and the depth's image summed
+// over the chunk's frequency groups.  The shot's source index, output image and accumulate flag
+// are read from p->shot3d, so the sequence is the same for every shot.
+static int enqueue_shot3d(swem_plan* p, int nfc, int nz_run, hipStream_t st)
+{
+    const int nf = p->nf, nm = p->nm, nz = p->g.nz, F = p->F3;
+    for (int f0 = 0; f0 < nf; f0 += nfc) {
+        March3DArgs a = base3(p);
+        a.f0 = f0;
+        a.nitem = std::min(nfc, nf - f0);
+        a.igx = -1;
+        a.shot = p->shot3d.p;
+        a.adj = 1;
+        a.spec_r = p->spec_r.p;
+        a.W = p->W.p;
+        a.partial = p->partial.p;
+        a.R = p->R.p;
+        a.conj_src = 1;
+        a.conj_rec = 0;
+        SWEM_TRY(launch_march3d(0, a, F, 0, st));
+        for (int iz = 0; iz < nz_run; iz++) {
+            a.iz = iz;
+            a.ds = stepped_at(p, iz, p->g.sz);
+            a.dr = stepped_at(p, iz, p->g.gz);
+            a.next_s = stepped_at(p, iz + 1, p->g.sz);
+            a.next_r = stepped_at(p, iz + 1, p->g.gz);
+            a.wmask = (a.ds ? 1 : 0) | (a.dr ? 2 : 0);
+            a.iz_sel = iz;
+            if (p->g.pspi) {
+                if (a.wmask) SWEM_TRY(launch_march3d(4, a, F, 0, st));
+                SWEM_TRY(launch_march3d(5, a, F, 0, st));
+            } else {
+                if (a.wmask) SWEM_TRY(launch_march3d(1, a, F, 0, st));
+                SWEM_TRY(launch_march3d(2, a, F, 0, st));
+            }
+            // this depth's image, summed over the chunk's frequency groups
+            SWEM_TRY(launch_reduce_depth_shot(p->partial.p, (a.nitem + F - 1) / F, nm, p->shot3d.p, iz, nz, f0 > 0, st));
+        }
+    }
+    return SWEM_OK;
+}
+
+// Captures enqueue_shot3d as a graph of kernel nodes on the plan's capture stream and
+// instantiates it (replacing an earlier one).
+static int capture_shot3d(swem_plan* p, int nfc, int nz_run)
+{
+    if (p->graph3d_exec) (void)hipGraphExecDestroy(p->graph3d_exec);
+    if (p->graph3d) (void)hipGraphDestroy(p->graph3d);
+    p->graph3d_exec = nullptr;
+    p->graph3d = nullptr;
+    p->graph3d_key.clear();
+    if (!p->cap_stream) SWEM_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+    SWEM_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue_shot3d(p, nfc, nz_run, p->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    const hipError_t ei = hipGraphInstantiate(&p->graph3d_exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        p->graph3d_exec = nullptr;
+        return hip_error(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    p->graph3d = g;
+    return SWEM_OK;
 }
 
 static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, const float* sy, int nshot, float* m_dev,
@@ -691,6 +762,7 @@ static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, con
 {
     const size_t nm = p->nm, nz = p->g.nz, nt = p->g.nt, nf = p->nf;
     const int F = p->F3;
+    const int nz_run = p->tune_nz > 0 ? std::min(p->tune_nz, (int)nz) : (int)nz;
     const size_t per_item =
         2 * (size_t)p->g.nmx * p->s.nky * sizeof(float2) + nm * sizeof(float) / F + 1 + pspi3_slice_bytes(p);
     const int nfc = chunk3(p, per_item, F);
@@ -699,53 +771,37 @@ static int migrate3d_impl(swem_plan* p, const float* d_dev, const float* sx, con
     SWEM_TRY(p->partial.alloc(gmax * nm));
     if (p->g.pspi) SWEM_TRY(p->R.alloc((size_t)nfc * pspi3_slice_bytes(p) / sizeof(float2)));
     SWEM_TRY(p->spec_r.alloc(nm * nf));
+    SWEM_TRY(p->shot3d.alloc(1));
     SWEM_TRY(set_igx(p, sx, sy, nshot, st));
     p->nev = 0;
     p->last_launches = 0;
     p->last_flops = 0;
     p->last_stream = st;
     SWEM_TRY(record(p, -1, st));
+    const uintptr_t key[] = {(uintptr_t)p->W.p, (uintptr_t)p->partial.p, (uintptr_t)p->R.p, (uintptr_t)p->spec_r.p,
+                             (uintptr_t)p->shot3d.p, (uintptr_t)nfc, (uintptr_t)F, (uintptr_t)nz_run, (uintptr_t)nf};
+    const bool graphs = p->use_graphs && !p->served && p->tune_nz == 0;
+    if (graphs && p->graph3d_key != std::vector<uintptr_t>(std::begin(key), std::end(key))) {
+        SWEM_TRY(capture_shot3d(p, nfc, nz_run));
+        p->graph3d_key.assign(std::begin(key), std::end(key));
+    }
+    const int nchunks = ((int)nf + nfc - 1) / nfc;
     for (int s = 0; s < nshot; s++) {
         SWEM_TRY(launch_sgemm((int)nm, 2 * (int)nf, (int)nt, d_dev + (size_t)s * nm * nt, (int)nt, p->basis_adj,
                               2 * (int)nf, reinterpret_cast<float*>(p->spec_r.p), 2 * (int)nf, st));
         SWEM_TRY(record(p, 0, st));
-        for (int f0 = 0; f0 < (int)nf; f0 += nfc) {
-            March3DArgs a = base3(p);
-            a.f0 = f0;
-            a.nitem = std::min(nfc, (int)nf - f0);
-            a.igx = p->igx_host[s];
-            a.adj = 1;
-            a.spec_r = p->spec_r.p;
-            a.W = p->W.p;
-            a.partial = p->partial.p;
-            a.R = p->R.p;
-            a.conj_src = 1;
-            a.conj_rec = 0;
-            float* out = per_shot ? m_dev + (size_t)s * nm * nz : m_dev;
-            const int acc = f0 > 0 || (!per_shot && (accumulate || s > 0));
-            SWEM_TRY(launch_march3d(0, a, F, 0, st));
-            for (int iz = 0; iz < (int)nz; iz++) {
-                a.iz = iz;
-                a.ds = stepped_at(p, iz, p->g.sz);
-                a.dr = stepped_at(p, iz, p->g.gz);
-                a.next_s = stepped_at(p, iz + 1, p->g.sz);
-                a.next_r = stepped_at(p, iz + 1, p->g.gz);
-                a.wmask = (a.ds ? 1 : 0) | (a.dr ? 2 : 0);
-                a.iz_sel = iz;
-                if (p->g.pspi) {
-                    if (a.wmask) SWEM_TRY(launch_march3d(4, a, F, 0, st));
-                    SWEM_TRY(launch_march3d(5, a, F, 0, st));
-                } else {
-                    if (a.wmask) SWEM_TRY(launch_march3d(1, a, F, 0, st));
-                    SWEM_TRY(launch_march3d(2, a, F, 0, st));
-                }
-                // this depth's image, summed over the chunk's frequency groups
-                SWEM_TRY(launch_reduce_depth(p->partial.p, (a.nitem + F - 1) / F, (int)nm, out + iz, (int)nz, acc, st));
-            }
-            SWEM_TRY(record(p, 1, st));
-            p->last_launches++;
-            p->last_flops += step_flops(p) * (double)a.nitem * nz;
-        }
+        Shot3D v;
+        v.igx = p->igx_host[s];
+        v.acc = !per_shot && (accumulate || s > 0);
+        v.out = per_shot ? m_dev + (size_t)s * nm * nz : m_dev;
+        SWEM_TRY(launch_set_shot3d(p->shot3d.p, v, st));
+        if (graphs)
+            SWEM_HIP(hipGraphLaunch(p->graph3d_exec, st));
+        else
+            SWEM_TRY(enqueue_shot3d(p, nfc, nz_run, st));
+        SWEM_TRY(record(p, 1, st));
+        p->last_launches += nchunks;
+        p->last_flops += step_flops(p) * (double)nf * nz;
     }
     return SWEM_OK;
 }
@@ -766,6 +822,21 @@ static int model3d_pspi(swem_plan* p, March3DArgs a, int F, hipStream_t st)
         SWEM_TRY(launch_march3d(5, a, F, 1, st));
     }
     a.first = 1;
+    if (a.linear) {
+        // the exact adjoint of the migration's selection (wem.c:476 weights before propagation):
+        // per depth, upward, the y-pass adds the injection and writes the weighted wavefield of
+        // every vref in use, and the x-pass sums their propagated spectra into the state
+        for (int iz = nz - 1; iz >= 0; iz--) {
+            if (!stepped_at(p, iz, p->g.gz)) continue;
+            a.iz = iz;
+            a.iz_sel = iz;
+            a.wmask = 2;
+            SWEM_TRY(launch_march3d(5, a, F, 4, st));
+            SWEM_TRY(launch_march3d(6, a, F, 0, st));
+            a.first = 0;
+        }
+        return launch_march3d(5, a, F, 5, st);
+    }
     int last = -1;  // depth of the last receiver x-pass
     for (int iz = nz - 1; iz >= 0; iz--) {
         if (!stepped_at(p, iz, p->g.gz)) continue;
@@ -935,6 +1006,19 @@ int swem_device_count(int* count)
 
 namespace swem {
 
+// The sampling of ewem (ewem.c:43-52): ntfft = 2 truncf(padt nt / 2) (even), ifmax =
+// trunc(fmax dt ntfft) + 1, ifmin = trunc(fmin dt ntfft), without wem.c's integer casts of fmax
+// and fmin (wem.c:79-89).
+static void ewem_sampling(const swem_geometry& g, int& ntfft, int& ifmin, int& ifmax)
+{
+    ntfft = (int)(2.f * std::trunc((float)g.padt * (float)g.nt / 2.f));
+    const int nw = ntfft / 2 + 1;
+    float t = g.fmax * g.dt * (float)ntfft;
+    ifmax = t + 1 < (float)nw ? (int)std::trunc(t) + 1 : nw;
+    t = g.fmin * g.dt * (float)ntfft;
+    ifmin = t + 1 < (float)ifmax ? (int)std::trunc(t) : 0;
+}
+
 int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swem_options* opt)
 {
     swem_options o;
@@ -948,6 +1032,14 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
     p->g = *g;
     p->opt = o;
     SWEM_TRY(compute_sampling(p->g, p->s));
+    if (p->band_rule == 1) {
+        ewem_sampling(p->g, p->s.ntfft, p->s.ifmin, p->s.ifmax);
+        p->s.nw = p->s.ntfft / 2 + 1;
+        p->s.fop_n = (p->s.nw - 1) * 2;
+        if (p->s.fop_n < g->nt || p->s.fop_n < 2)
+            return set_error(SWEM_EINVAL, "ewem: ntfft=%d is too short for nt=%d", p->s.ntfft, g->nt);
+        p->s.dw = (float)(2 * kPi / ((double)(float)p->s.ntfft) / (double)g->dt);  // ewem.c:48
+    }
     p->nm = g->nmx * g->nmy;
     p->nf = p->s.ifmax - p->s.ifmin;
     p->nkx = p->s.nkx;
@@ -955,8 +1047,6 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
     p->narr = p->nkx;
     if (!p->three_d && march2d_blue_len(p->nkx)) p->narr = march2d_blue_len(p->nkx);
     if (p->three_d) {
-        if (g->pspi && o.pspi_mode != SWEM_PSPI_COMPAT)
-            return set_error(SWEM_EUNSUPPORTED, "3-D PSPI: the linear mode is not available in this build (compat is)");
         if (!march3d_supported(p->s.nkx, p->s.nky))
             return set_error(SWEM_EUNSUPPORTED, "3-D: nkx=%d, nky=%d: this build handles 1..512 and powers of two to 1024",
                              p->s.nkx, p->s.nky);
@@ -1014,7 +1104,10 @@ void plan_free(swem_plan* p)
     p->own_bases.adj.release(); p->own_bases.fwd.release(); p->own_bases.src.release();
     p->p2.release(); p->sel.release(); p->sel2.release();
     p->used.release(); p->cref.release(); p->R.release(); p->spec_r.release(); p->smig.release(); p->dgw.release(); p->partial.release();
-    p->mT.release(); p->igx.release();
+    p->mT.release(); p->igx.release(); p->shot3d.release();
+    if (p->graph3d_exec) (void)hipGraphExecDestroy(p->graph3d_exec);
+    if (p->graph3d) (void)hipGraphDestroy(p->graph3d);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     delete p;
 }
 
@@ -1068,6 +1161,23 @@ size_t plan_migrate_bytes(const swem_plan* p, int nshot)
 }  // namespace swem
 
 extern "C" {
+
+static int plan_create_rule(const swem_geometry* g, const float* vel, const float* wav, const swem_options* opt,
+                            int band_rule, swem_plan** out)
+{
+    if (!g || !vel || !wav || !out) return set_error(SWEM_EINVAL, "plan_create: null argument");
+    *out = nullptr;
+    swem_plan* p = new swem_plan();
+    p->band_rule = band_rule;
+    int rc = plan_setup(p, g, vel, opt);
+    if (!rc) rc = plan_build(p, wav);
+    if (rc) {
+        plan_free(p);
+        return rc;
+    }
+    *out = p;
+    return SWEM_OK;
+}
 
 int swem_plan_create(const swem_geometry* g, const float* vel, const float* wav, const swem_options* opt,
                      swem_plan** out)
@@ -1217,58 +1327,159 @@ int swem_plan_timing(swem_plan* p, swem_timing* out)
 // timed on nshot zero records (the march does the same work whatever the data) and the fastest
 // becomes the plan's variant.  times[SWEM_VARIANT_PACKED + 1] receives each variant's march time in
 // ms (0: not a distinct candidate).
-int swem_plan_autotune(swem_plan* p, int nshot, int* best, float* times)
-{
-    if (!p || nshot < 1) return set_error(SWEM_EINVAL, "autotune: null plan or nshot < 1");
-    if (p->three_d) return set_error(SWEM_EUNSUPPORTED, "autotune: 3-D plans have one kernel shape");
-    SWEM_HIP(hipSetDevice(p->opt.device));
-    const size_t nm = p->nm, nt = p->g.nt, nz = p->g.nz;
-    float *d = nullptr, *m = nullptr;
-    SWEM_HIP(hipMalloc((void**)&d, (size_t)nshot * nm * nt * sizeof(float)));
-    hipError_t e = hipMalloc((void**)&m, nm * nz * sizeof(float));
-    if (e != hipSuccess) {
-        (void)hipFree(d);
-        return hip_error(e, "hipMalloc", __FILE__, __LINE__);
+// Zero records of nshot shots spread over the line (the march's work does not depend on the
+// record values) and an image buffer, for timing candidate launch shapes.
+struct TuneBuffers {
+    float* d = nullptr;
+    float* m = nullptr;
+    std::vector<float> sx, sy;
+    ~TuneBuffers()
+    {
+        (void)hipDeviceSynchronize();
+        if (d) (void)hipFree(d);
+        if (m) (void)hipFree(m);
     }
-    std::vector<float> sx(nshot);
-    for (int i = 0; i < nshot; i++) sx[i] = p->g.omx + p->g.dmx * (float)((i * (p->g.nmx - 1)) / std::max(1, nshot - 1));
+    int init(const swem_plan* p, int nshot)
+    {
+        const size_t nm = p->nm, nt = p->g.nt, nz = p->g.nz;
+        SWEM_HIP(hipMalloc((void**)&d, (size_t)nshot * nm * nt * sizeof(float)));
+        SWEM_HIP(hipMalloc((void**)&m, nm * nz * sizeof(float)));
+        SWEM_HIP(hipMemset(d, 0, (size_t)nshot * nm * nt * sizeof(float)));
+        sx.resize(nshot);
+        sy.assign(nshot, p->g.omy + p->g.dmy * (float)(p->g.nmy / 2));
+        for (int i = 0; i < nshot; i++)
+            sx[i] = p->g.omx + p->g.dmx * (float)((i * (p->g.nmx - 1)) / std::max(1, nshot - 1));
+        return SWEM_OK;
+    }
+};
+
+// Time one candidate: a warm-up migration, then the timed one (march kernels only).
+static int tune_time(swem_plan* p, TuneBuffers& b, int nshot, float& ms)
+{
+    swem_timing t{};
+    for (int rep = 0; rep < 2; rep++) {
+        SWEM_TRY(migrate_impl(p, b.d, b.sx.data(), b.sy.data(), nshot, b.m, 0, 0, nullptr));
+        SWEM_TRY(swem_plan_timing(p, &t));
+    }
+    ms = (float)t.march_ms;
+    return SWEM_OK;
+}
+
+static void tune_add(swem_tune_result* r, int variant, int fpb, size_t chunk, float ms)
+{
+    if (r->ncand >= SWEM_TUNE_MAX) return;
+    r->cand_variant[r->ncand] = variant;
+    r->cand_fpb[r->ncand] = fpb;
+    r->cand_chunk_bytes[r->ncand] = chunk;
+    r->cand_ms[r->ncand] = ms;
+    r->ncand++;
+}
+
+// 2-D: every migration kernel variant that fits nkx, each with every frequency group size that
+// fits its workgroup and LDS (1, 2, 4, 8, 16 frequencies per workgroup).
+static int autotune2d(swem_plan* p, int nshot, swem_tune_result* r)
+{
+    TuneBuffers b;
+    SWEM_TRY(b.init(p, nshot));
     const swem_options keep = p->opt;
     const int keepF = p->F;
     int rc = SWEM_OK;
-    if (hipMemset(d, 0, (size_t)nshot * nm * nt * sizeof(float)) != hipSuccess)
-        rc = set_error(SWEM_EDEVICE, "autotune: memset");
-    int bestv = -1;
-    float bestms = 0.f;
     int seen[SWEM_VARIANT_PACKED + 1] = {};
     for (int v = SWEM_VARIANT_SUBGROUP; v <= SWEM_VARIANT_PACKED && !rc; v++) {
-        if (times) times[v] = 0.f;
         if (v == SWEM_VARIANT_PACKED && !p->packed_ok) continue;
         const March2DShape sh = march2d_shape(p->nkx, v, 1);
         if (sh.T <= 0 || seen[sh.variant]) continue;  // not available, or the same kernel as an earlier one
         seen[sh.variant] = 1;
         p->opt.variant = v;
-        p->F = choose_group(p, 1, keep.freqs_per_block);
-        if (sh.per_freq * p->F * sh.T > sh.max_block || march2d_lds_bytes(p->nkx, sh, p->F) > 160 * 1024) continue;
-        swem_timing t{};
-        for (int rep = 0; rep < 2 && !rc; rep++) {  // a warm-up, then the timed call
-            rc = migrate_impl(p, d, sx.data(), nullptr, nshot, m, 0, 0, nullptr);
-            if (!rc) rc = swem_plan_timing(p, &t);
+        for (int F = 1; F <= 16 && !rc; F *= 2) {
+            if (v == SWEM_VARIANT_PACKED && F > 1) break;  // one frequency per packed workgroup
+            if (keep.freqs_per_block > 0 && F != keep.freqs_per_block) continue;
+            if (sh.per_freq * F * sh.T > sh.max_block || march2d_lds_bytes(p->nkx, sh, F) > 160 * 1024) break;
+            if (F > 1 && (p->nfc + F - 1) / F < 1) break;
+            p->F = F;
+            float ms = 0.f;
+            rc = tune_time(p, b, nshot, ms);
+            if (!rc) tune_add(r, v, F, 0, ms);
         }
-        if (rc) break;
-        const float ms = (float)t.march_ms;
-        if (times) times[v] = ms;
-        if (bestv < 0 || ms < bestms) bestv = v, bestms = ms;
     }
-    (void)hipDeviceSynchronize();
-    (void)hipFree(d);
-    (void)hipFree(m);
     p->opt = keep;
     p->F = keepF;
-    if (rc) return rc;
-    if (bestv < 0) return set_error(SWEM_EUNSUPPORTED, "autotune: no 2-D kernel fits nkx=%d", p->nkx);
-    p->opt.variant = bestv;
-    p->F = choose_group(p, 1, keep.freqs_per_block);
-    if (best) *best = bestv;
+    return rc;
+}
+
+// 3-D: frequencies per y-pass workgroup (2, 4, 8 and 16 where the y-pass kernel allows it) and the
+// step state of one launch (64, 128, 256 MiB: the frequencies marched together, against the
+// 256 MB last-level cache), timed on the first 16 depths of the plan (every depth is the same
+// work).
+static int autotune3d(swem_plan* p, int nshot, swem_tune_result* r)
+{
+    TuneBuffers b;
+    SWEM_TRY(b.init(p, nshot));
+    const int keepF = p->F3;
+    const size_t keepS = p->state3d_bytes;
+    const int fmax = (p->g.pspi || p->narr_y != p->s.nky) ? 8 : 16;
+    p->tune_nz = std::min(16, p->g.nz);
+    int rc = SWEM_OK;
+    for (int F = 2; F <= fmax && !rc; F *= 2) {
+        if (p->opt.freqs_per_block > 0 && F != p->opt.freqs_per_block) continue;
+        for (size_t S = size_t(64) << 20; S <= (size_t(256) << 20) && !rc; S *= 2) {
+            p->F3 = F;
+            p->state3d_bytes = S;
+            float ms = 0.f;
+            rc = tune_time(p, b, nshot, ms);
+            if (!rc) tune_add(r, SWEM_VARIANT_AUTO, F, S, ms);
+        }
+    }
+    p->tune_nz = 0;
+    p->F3 = keepF;
+    p->state3d_bytes = keepS;
+    return rc;
+}
+
+int swem_plan_autotune_ex(swem_plan* p, int nshot, swem_tune_result* out)
+{
+    if (!p || nshot < 1 || !out) return set_error(SWEM_EINVAL, "autotune: null plan, null result or nshot < 1");
+    SWEM_HIP(hipSetDevice(p->opt.device));
+    std::memset(out, 0, sizeof(*out));
+    SWEM_TRY(p->three_d ? autotune3d(p, nshot, out) : autotune2d(p, nshot, out));
+    int best = -1;
+    for (int i = 0; i < out->ncand; i++)
+        if (best < 0 || out->cand_ms[i] < out->cand_ms[best]) best = i;
+    if (best < 0) return set_error(SWEM_EUNSUPPORTED, "autotune: no kernel shape fits this plan (nkx=%d)", p->nkx);
+    out->variant = out->cand_variant[best];
+    out->freqs_per_block = out->cand_fpb[best];
+    out->chunk_bytes = out->cand_chunk_bytes[best];
+    out->best_ms = out->cand_ms[best];
+    if (p->three_d) {
+        p->F3 = out->freqs_per_block;
+        p->state3d_bytes = out->chunk_bytes;
+    } else {
+        p->opt.variant = out->variant;
+        p->F = out->freqs_per_block;
+    }
+    return SWEM_OK;
+}
+
+int swem_plan_autotune(swem_plan* p, int nshot, int* best, float* times)
+{
+    if (!p || nshot < 1) return set_error(SWEM_EINVAL, "autotune: null plan or nshot < 1");
+    swem_tune_result r;
+    SWEM_TRY(swem_plan_autotune_ex(p, nshot, &r));
+    if (times) {
+        for (int v = 0; v <= SWEM_VARIANT_PACKED; v++) times[v] = 0.f;
+        for (int i = 0; i < r.ncand; i++) {  // each variant's best group size
+            float& t = times[r.cand_variant[i]];
+            if (t == 0.f || r.cand_ms[i] < t) t = r.cand_ms[i];
+        }
+    }
+    if (best) *best = r.variant;
+    return SWEM_OK;
+}
+
+int swem_plan_set_graphs(swem_plan* p, int enable)
+{
+    if (!p) return set_error(SWEM_EINVAL, "plan_set_graphs: null plan");
+    p->use_graphs = enable ? 1 : 0;
     return SWEM_OK;
 }
 
@@ -1602,15 +1813,6 @@ int swem_compute_angles(float* angx, float* angy, const float* wav, int nt, floa
 
 // ------------------------------------------------------------------------ elastic WEM (2-D)
 // The sampling of ewem (ewem.c:46-57), which truncates fmax dt ntfft where wem.c casts fmax.
-static void ewem_sampling(const swem_geometry& g, int& ntfft, int& ifmin, int& ifmax)
-{
-    ntfft = (int)(2.f * std::trunc((float)g.padt * (float)g.nt / 2.f));
-    const int nw = ntfft / 2 + 1;
-    float t = g.fmax * g.dt * (float)ntfft;
-    ifmax = t + 1 < (float)nw ? (int)std::trunc(t) + 1 : nw;
-    t = g.fmin * g.dt * (float)ntfft;
-    ifmin = t + 1 < (float)ifmax ? (int)std::trunc(t) : 0;
-}
 
 // nshot shots of elastic migration (records [nshot][nm][nt] -> images [nm][nz] summed over shots)
 // or modelling (images -> records of every shot), all shots of a chunk in one launch.
@@ -1777,12 +1979,12 @@ int swem_eshots(const swem_geometry* g, const float* vp, const float* vs, const 
     if (opt) o = *opt;
     else swem_options_init(&o);
     swem_plan *pp = nullptr, *ps = nullptr;
-    SWEM_TRY(swem_plan_create(&ge, vp, wav, &o, &pp));
-    int rc = swem_plan_create(&ge, vs, wav, &o, &ps);
+    // both plans sample the band as ewem.c does (ewem.c:43-52)
+    SWEM_TRY(plan_create_rule(&ge, vp, wav, &o, 1, &pp));
+    int rc = plan_create_rule(&ge, vs, wav, &o, 1, &ps);
     if (!rc && (pp->s.ntfft != ntfft || pp->s.ifmin != ifmin || pp->s.ifmax != ifmax))
-        rc = set_error(SWEM_EUNSUPPORTED, "ewem: this geometry's band differs between ewem.c (ntfft %d, [%d, %d)) and "
-                       "wem.c (ntfft %d, [%d, %d)), whose sampling the plans use", ntfft, ifmin, ifmax, pp->s.ntfft,
-                       pp->s.ifmin, pp->s.ifmax);
+        rc = set_error(SWEM_EDEVICE, "ewem: internal band mismatch (ntfft %d, [%d, %d)) against (%d, [%d, %d))", ntfft,
+                       ifmin, ifmax, pp->s.ntfft, pp->s.ifmin, pp->s.ifmax);
     if (!rc) rc = eshots_impl(pp, ps, nshot, igx.data(), ux, uy, uz, mpp, mps1, mps2, wav, adj);
     if (ps) swem_plan_destroy(ps);
     swem_plan_destroy(pp);

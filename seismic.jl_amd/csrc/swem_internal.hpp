@@ -152,6 +152,14 @@ size_t march2d_lds_bytes(int nkx, const March2DShape& sh, int F);
 
 // Arguments of the 3-D split-step kernels (swem_march3d.hip).  A launch covers the items
 // (frequencies f0 .. f0 + nitem of one shot) of a frequency chunk at depth iz.
+// Per-shot values of a 3-D migration read by the kernels from device memory, so that the launch
+// sequence of a shot does not depend on the shot (it is captured once as a graph and replayed).
+struct Shot3D {
+    int igx;     // source trace ix * nmy + iy
+    int acc;     // add this shot's image to out (else overwrite)
+    float* out;  // image [nm][nz] the shot's depths are reduced into
+};
+
 struct March3DArgs {
     int nmx, nmy, nm, nkx, nky, nz, nf, ifmin, f0, nitem, iz, prec;
     int igx;                    // source trace ix * nmy + iy
@@ -187,12 +195,18 @@ struct March3DArgs {
     const uint8_t* sel;
     const uint32_t* used;
     float2* R;
+    const Shot3D* shot;         // when set, igx is read from it (migration)
+    // PSPI linear mode: iref2 per point ([nz][nm]) and the interpolation weight of iref2
+    int linear;
+    const uint8_t* sel2;
+    const float* w2;
 };
 int march3d_len(int n);
 bool march3d_supported(int nkx, int nky);
 // which: 0 init, 1 x-pass, 2 migration y-pass, 3 modelling y-pass (phase 0 source,
 // 1 receiver, 2 output), 4 PSPI x-pass, 5 PSPI y-pass (phase 0 migration, 1 modelling source,
-// 2 modelling receiver, 3 modelling output).  F: frequencies per y-pass workgroup.
+// 2 modelling receiver, 3 modelling output, 4 linear-mode modelling receiver, 5 its output),
+// 6 linear-mode modelling receiver x-pass.  F: frequencies per y-pass workgroup.
 int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_t st);
 
 // C[M x Nc] = A[M x K] * B[K x Nc], row-major with leading dimensions lda, ldb, ldc.
@@ -214,6 +228,12 @@ int noalloc_scratch(int device, void** ptr, size_t* bytes);
 // dst[0..n) = vals[0..n) on stream st through kernel arguments (kIntBlock values per launch).
 constexpr int kIntBlock = 256;
 int launch_fill_ints(int* dst, const int* vals, int n, hipStream_t st);
+// Image of depth iz of a 3-D chunk: shot->out[x * ldo + iz] (+)= compensated sum over the chunk's
+// frequency groups, adding when acc_chunk or shot->acc is set.
+int launch_reduce_depth_shot(const float* partial, int ngroups, int n, const Shot3D* shot, int iz, int ldo,
+                             int acc_chunk, hipStream_t st);
+// *dst = v on stream st (the value travels as a kernel argument).
+int launch_set_shot3d(Shot3D* dst, const Shot3D& v, hipStream_t st);
 int launch_reduce_image(const float* partial, int nshot, int ngroups, int nz, int nmx, float* out, int accumulate,
                         int per_shot, hipStream_t st);
 // Solver vector operations: m1 = a m1 + b m2; <x, y> in double into part[kDotBlocks] (part has

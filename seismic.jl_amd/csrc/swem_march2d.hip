@@ -74,6 +74,14 @@ __device__ __forceinline__ void buf_store1(__amdgpu_buffer_rsrc_t r, uint32_t of
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
 }
+// Partial-image rows are written once and read once by the reduction: stored non-temporal (cache
+// policy NT), so they stream through L2 instead of evicting the phase-table rows that the shots
+// of one frequency share there.
+constexpr int kCachePolicyNT = 2;
+__device__ __forceinline__ void buf_store1_nt(__amdgpu_buffer_rsrc_t r, uint32_t off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, kCachePolicyNT);
+}
 __device__ __forceinline__ void buf_store2(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 v)
 {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
@@ -473,7 +481,7 @@ __device__ __forceinline__ void adj_body(const March2DArgs& a)
             if (!src) {
                 const auto pr = buf_rsrc(part + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
 #pragma unroll
-                for (int q = 0; q < P; ++q) buf_store1(pr, (uint32_t)(t + q * T) * 4u, cc[q]);
+                for (int q = 0; q < P; ++q) buf_store1_nt(pr, (uint32_t)(t + q * T) * 4u, cc[q]);
             }
         } else {
             if constexpr (WAVE) __syncthreads();  // single buffer: the source copy is read
@@ -616,7 +624,7 @@ __device__ __forceinline__ void adj_body_pair(const March2DArgs& a)
         const auto pr = buf_rsrc(part + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
 #pragma unroll
         for (int q = 0; q < P; ++q)
-            buf_store1(pr, (uint32_t)(t + q * T) * 4u, dr ? factor * (ur[q].x * us[q].x + ur[q].y * us[q].y) : 0.f);
+            buf_store1_nt(pr, (uint32_t)(t + q * T) * 4u, dr ? factor * (ur[q].x * us[q].x + ur[q].y * us[q].y) : 0.f);
     }
 }
 
@@ -861,7 +869,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
                 z[q] = make_float2(0.f, 0.f);
                 continue;  // x >= nmx: nothing to image
             }
-            buf_store1(pr, (uint32_t)(t + q * T) * 4u, factor * (z[q].x * z[q].y - e[q].x * e[q].y) * inv_alpha);
+            buf_store1_nt(pr, (uint32_t)(t + q * T) * 4u, factor * (z[q].x * z[q].y - e[q].x * e[q].y) * inv_alpha);
             n2s += z[q].x * z[q].x + e[q].x * e[q].x;
             n2r += z[q].y * z[q].y + e[q].y * e[q].y;
         }

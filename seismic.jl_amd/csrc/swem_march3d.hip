@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void march3d_init(March3DArgs a)
         if (y < a.nmy) {
             const size_t ixy = (size_t)x * a.nmy + y;
             if (wf == 0)
-                u = (int)ixy == a.igx ? a.spec_s[fl] : u;
+                u = (int)ixy == (a.shot ? a.shot->igx : a.igx) ? a.spec_s[fl] : u;
             else if (a.adj)
                 u = a.spec_r[ixy * a.nf + fl];
         }
@@ -422,6 +422,94 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 51
     }
 }
 
+// x-pass of the linear-mode modelling receiver (the transpose of march3d_cols_pspi): for every
+// vref in use at the depth, the slice R[ir] (the weighted wavefield, y transformed) is x
+// transformed and multiplied by its kz phase (conj for the -dz step), the products are summed,
+// and one inverse x-FFT gives the stepped receiver (y transformed) in the state.
+template <int NX, int C, bool BX>
+__global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 512 ? 2 : 4))) void march3d_cols_gather(
+    March3DArgs a)
+{
+    constexpr int P = NX / 64;
+    constexpr int L = LdsLen<NX>::value;
+    extern __shared__ float2 smem[];
+    const int tiles = (a.nky + C - 1) / C;
+    const int nblk = a.nitem * tiles;
+    const int full = nblk & ~7;
+    int b = blockIdx.x;
+    if (b < full) b = (b & 7) * (full >> 3) + (b >> 3);
+    const int item = b / tiles;
+    const int ky0 = (b - item * tiles) * C;
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const int c = threadIdx.x % C, x0 = threadIdx.x / C;
+    const bool kin = ky0 + c < a.nky;
+    const int iw = a.ifmin + a.f0 + item;
+    const double wd = (double)iw * a.dw;
+    const int iky = ky0 + w;
+    const int kyi = iky <= a.nky / 2 ? iky : a.nky - iky;
+    const double ky = kyi * a.dky;
+    const uint32_t used = a.used[a.iz];
+    float2* buf = smem + w * L;
+    float2 acc[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) acc[q] = make_float2(0.f, 0.f);
+    for (int ir = 0; ir < a.nref; ++ir) {
+        if (!((used >> ir) & 1u)) continue;
+        // stage the tile's columns of slice ir (row segments of C complex)
+        {
+            const float2* src = rslice(a, item, 1, ir) + ky0 + c;
+            float2* dst = smem + c * L;
+            float2 st[P];
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int x = x0 + 64 * j;
+                st[j] = x < a.nmx && kin ? src[(size_t)x * a.nky] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int x = x0 + 64 * j;
+                if (x < a.nmx) dst[lds_pad(x)] = st[j];
+            }
+        }
+        __syncthreads();
+        float2 v[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int x = t + 64 * q;
+            v[q] = x < a.nmx ? buf[lds_pad(x)] : make_float2(0.f, 0.f);
+        }
+        wave_sync_lds();
+        xfft<NX, -1, BX>(v, buf, a, t);
+        const double p2 = a.p2[(size_t)a.iz * a.nref + ir];
+        const float p2f = a.p2f[(size_t)a.iz * a.nref + ir];
+        const double c0 = wd * wd * p2 - ky * ky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const float2 ph = a.prec == SWEM_PREC_KZ64 ? kz_phase64(a, c0, t + 64 * q)
+                                                       : kz_phase(a, wd, p2, p2f, t + 64 * q, iky);
+            acc[q] = cadd(acc[q], cmul(v[q], make_float2(ph.x, a.conj_rec ? -ph.y : ph.y)));
+        }
+        __syncthreads();  // the tile is restaged for the next vref
+    }
+    xfft<NX, +1, BX>(acc, buf, a, t);
+    wave_sync_lds();
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int x = t + 64 * q;
+        if (x < a.nmx) buf[lds_pad(x)] = acc[q];
+    }
+    __syncthreads();
+    int c1 = c, x1 = x0;
+    asm volatile("" : "+v"(c1), "+v"(x1));
+    float2* dst = state(a, item, 1) + ky0 + c1;
+    const float2* src = smem + c1 * L;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        const int x = x1 + 64 * j;
+        if (x < a.nmx && ky0 + c1 < a.nky) dst[(size_t)x * a.nky] = src[lds_pad(x)];
+    }
+}
+
 // u = the pspiop selection of row x of wavefield wf from the slices of depth iz_sel: inverse
 // y-FFT of each slice in use, per point the slice of its iref1, real part where cut.
 template <int NY, bool BY>
@@ -438,6 +526,20 @@ __device__ __forceinline__ void pspi_select_row(const March3DArgs& a, int item, 
         s1[q] = y < a.nmy ? sr[y] : (uint8_t)0x7f;
         u[q] = make_float2(0.f, 0.f);
     }
+    // linear mode: the weights of wem.c:476's interpolation, (1 - w2) for iref1 and w2 for iref2
+    // at the cut points, 1 for iref1 elsewhere
+    uint8_t s2[P];
+    float wt2[P];
+    if (a.linear) {
+        const uint8_t* sr2 = a.sel2 + (size_t)a.iz_sel * a.nm + (size_t)x * a.nmy;
+        const float* wr = a.w2 + (size_t)a.iz_sel * a.nm + (size_t)x * a.nmy;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int y = t + 64 * q;
+            s2[q] = y < a.nmy ? sr2[y] : (uint8_t)0x7f;
+            wt2[q] = y < a.nmy ? wr[y] : 0.f;
+        }
+    }
     for (int ir = 0; ir < a.nref; ++ir) {
         if (!((used >> ir) & 1u)) continue;
         const float2* row = rslice(a, item, wf, ir) + (size_t)x * a.nky;
@@ -446,10 +548,30 @@ __device__ __forceinline__ void pspi_select_row(const March3DArgs& a, int item, 
         for (int q = 0; q < P; ++q) bq[q] = yin<BY>(a, t + 64 * q) ? row[t + 64 * q] : make_float2(0.f, 0.f);
         yfft<NY, +1, BY>(bq, buf, a, t);
         wave_sync_lds();
+        if (a.linear) {
 #pragma unroll
-        for (int q = 0; q < P; ++q)
-            if ((s1[q] & 0x7f) == ir) u[q] = (s1[q] >> 7) ? make_float2(bq[q].x, 0.f) : bq[q];
+            for (int q = 0; q < P; ++q) {
+                const bool cut = s1[q] >> 7;
+                float wt = (s1[q] & 0x7f) == ir ? (cut ? 1.f - wt2[q] : 1.f) : 0.f;
+                if (cut && s2[q] == ir) wt += wt2[q];
+                u[q] = cadd(u[q], cscale(bq[q], wt));
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+                if ((s1[q] & 0x7f) == ir) u[q] = (s1[q] >> 7) ? make_float2(bq[q].x, 0.f) : bq[q];
+        }
     }
+}
+
+// Weight of vref ir at a point in the linear mode: (1 - w2) for iref1 and w2 for iref2 where cut,
+// 1 for iref1 elsewhere (wem.c:476).
+__device__ __forceinline__ float pspi3_weight(uint8_t s1, uint8_t s2, float w2, int ir)
+{
+    const bool cut = s1 >> 7;
+    float wt = (s1 & 0x7f) == ir ? (cut ? 1.f - w2 : 1.f) : 0.f;
+    if (cut && s2 == ir) wt += w2;
+    return wt;
 }
 
 // y-pass of PSPI, F waves (frequencies g F + f) of one row x.
@@ -473,6 +595,65 @@ __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, i
     const int item = g * F + w;
     const bool active = item < a.nitem;
     float2* buf = smem + w * L;
+    if (phase >= 4) {
+        // linear-mode modelling receiver.  phase 4 at depth iz: u = the last step's result (the
+        // inverse y-FFT of the state; zero before the first step) plus the injection m smig of
+        // this depth, then per vref in use u weighted, forward y-FFT, slice R[ir].  phase 5: the
+        // last step's result out as the modelled spectra.
+        if (!active) return;  // no workgroup barriers below
+        const int fl = a.f0 + item;
+        float2 u[P];
+        if (a.first) {
+#pragma unroll
+            for (int q = 0; q < P; ++q) u[q] = make_float2(0.f, 0.f);
+        } else {
+            const float2* src = state(a, item, 1) + (size_t)x * a.nky;
+#pragma unroll
+            for (int q = 0; q < P; ++q) u[q] = yin<BY>(a, t + 64 * q) ? src[t + 64 * q] : make_float2(0.f, 0.f);
+            yfft<NY, +1, BY>(u, buf, a, t);
+            wave_sync_lds();
+        }
+        if (phase == 5) {
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int y = t + 64 * q;
+                if (y < a.nmy) a.dgw[((size_t)x * a.nmy + y) * a.nf + fl] = u[q];
+            }
+            return;
+        }
+        const float2* sm = a.smig + ((size_t)item * a.nz + a.iz) * a.nm + (size_t)x * a.nmy;
+        const float* mt = a.mT + (size_t)a.iz * a.nm + (size_t)x * a.nmy;
+        const size_t o = (size_t)a.iz * a.nm + (size_t)x * a.nmy;
+        uint8_t s1[P], s2[P];
+        float wt2[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int y = t + 64 * q;
+            if (y < a.nmy) {
+                u[q] = cadd(u[q], cscale(sm[y], mt[y]));
+                s1[q] = a.sel[o + y];
+                s2[q] = a.sel2[o + y];
+                wt2[q] = a.w2[o + y];
+            } else {
+                u[q] = make_float2(0.f, 0.f);
+                s1[q] = s2[q] = 0x7f;
+                wt2[q] = 0.f;
+            }
+        }
+        const uint32_t used = a.used[a.iz];
+        for (int ir = 0; ir < a.nref; ++ir) {
+            if (!((used >> ir) & 1u)) continue;
+            float2 v[P];
+#pragma unroll
+            for (int q = 0; q < P; ++q) v[q] = cscale(u[q], pspi3_weight(s1[q], s2[q], wt2[q], ir));
+            wave_sync_lds();
+            yfft<NY, -1, BY>(v, buf, a, t);
+            float2* dst = rslice(a, item, 1, ir) + (size_t)x * a.nky;
+#pragma unroll
+            for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) dst[t + 64 * q] = v[q];
+        }
+        return;
+    }
     if (phase != 0) {
         if (!active) return;  // no workgroup barriers below
         const int fl = a.f0 + item;
@@ -817,6 +998,12 @@ static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStrea
         return nwf == 2 ? launch3(march3d_cols_pspi<N, 2, C2, BL>, grid, block, lds, st, a)
                         : launch3(march3d_cols_pspi<N, 1, C1, BL>, grid, block, lds, st, a);
     }
+    if (which == 6) {  // linear-mode modelling receiver x-pass (gather over vrefs)
+        constexpr int CG = N >= 1024 ? 8 : 16;
+        const size_t lds = (size_t)CG * L * sizeof(float2);
+        const dim3 grid(a.nitem * ((a.nky + CG - 1) / CG)), block(64 * CG);
+        return launch3(march3d_cols_gather<N, CG, BL>, grid, block, lds, st, a);
+    }
     const int ngroups = (a.nitem + F - 1) / F;
     if (which == 5) {
         const size_t lds = (size_t)F * L * sizeof(float2);
@@ -838,7 +1025,7 @@ static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStrea
 // (phase 0 source, 1 receiver, 2 output).  F: frequencies per y-pass workgroup.
 int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_t st)
 {
-    const bool xpass = which == 1 || which == 4;
+    const bool xpass = which == 1 || which == 4 || which == 6;
     const int n = xpass ? a.nkx : a.nky, m = march3d_len(n);
     if (!march3d_supported(a.nkx, a.nky) || a.nmx > a.nkx || a.nmy > a.nky || a.nitem < 1)
         return set_error(SWEM_EINVAL, "3-D march: inconsistent launch (nkx=%d nky=%d)", a.nkx, a.nky);
@@ -849,8 +1036,10 @@ int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_
     const int fmax = which == 5 ? 8 : (!xpass && m != n ? 8 : 16);
     if (((which == 2 || which == 3 || which == 5) && F > fmax) || F < 1)
         return set_error(SWEM_EINVAL, "3-D march: %d frequencies per workgroup (at most %d here)", F, fmax);
-    if ((which == 4 || which == 5) && (!a.R || !a.sel || !a.used || a.nref < 1 || a.nref > 31))
+    if ((which == 4 || which == 5 || which == 6) && (!a.R || !a.sel || !a.used || a.nref < 1 || a.nref > 31))
         return set_error(SWEM_EINVAL, "3-D PSPI march: tables missing");
+    if ((a.linear || which == 6) && (!a.sel2 || !a.w2))
+        return set_error(SWEM_EINVAL, "3-D PSPI march: linear-mode tables missing");
     const bool blue = m != n;
     switch (m) {
     case 128: return blue ? launch3_n<128, true>(which, a, F, phase, st) : launch3_n<128, false>(which, a, F, phase, st);

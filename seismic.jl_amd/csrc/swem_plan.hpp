@@ -2,6 +2,7 @@
 // in swem_host.cpp and the serving loop in swem_server.cpp.
 #pragma once
 #include <cstddef>
+#include <cstdint>
 #include <vector>
 
 #include "swem_internal.hpp"
@@ -88,7 +89,21 @@ struct swem_plan {
     // frequency chunks: [c * nfc, min(nf, (c + 1) * nfc)); the phase tables hold one chunk
     int nfc = 0, nchunks = 1, resident = -1;
     bool three_d = false;
+    // band rule: 0 the sampling of wem.c:79-89, 1 that of ewem.c:43-52 (elastic plans)
+    int band_rule = 0;
     int F3 = 8, F3f = 8;  // frequencies per 3-D y-pass workgroup (migration, modelling)
+    // 3-D step state of one launch (frequencies per launch); the autotuner may change it
+    size_t state3d_bytes = size_t(128) << 20;
+    int tune_nz = 0;  // autotuning only: march the first tune_nz depths (0 = all)
+    // 3-D migration of one shot as a captured graph: the launches of every frequency chunk and
+    // depth read the shot's values from shot3d; the graph is captured once per buffer layout
+    // (graph3d_key) and replayed for every shot.  use_graphs = 0 launches them one by one.
+    int use_graphs = 1;
+    swem::DevBuf<swem::Shot3D> shot3d;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph3d = nullptr;
+    hipGraphExec_t graph3d_exec = nullptr;
+    std::vector<uintptr_t> graph3d_key;
     std::vector<float> vel;  // host copy (source of every table)
     // construction context: stream of the table builds, optional shared bases
     hipStream_t build_stream = nullptr;

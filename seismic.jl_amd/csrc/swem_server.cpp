@@ -260,6 +260,7 @@ AdmitResult admit(swem_server* s, Request* r, int& code, std::string& msg)
     code = SWEM_OK;
     swem_plan* p = new swem_plan();
     p->served = true;
+    p->use_graphs = 0;  // one plan per request: a captured graph would not be replayed
     p->basis_cache = &s->bases;
     swem_options o = r->opt;
     o.device = s->cfg.device;

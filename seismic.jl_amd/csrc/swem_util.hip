@@ -225,6 +225,44 @@ int launch_reduce_depth(const float* partial, int ngroups, int n, float* out, in
     return SWEM_OK;
 }
 
+__global__ __launch_bounds__(256) void reduce_depth_shot(const float* __restrict__ partial, int ngroups, int n,
+                                                         const Shot3D* __restrict__ shot, int iz, int ldo,
+                                                         int acc_chunk)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    float sum = 0.f, comp = 0.f;
+    for (int g = 0; g < ngroups; ++g) {
+        const float y = partial[(size_t)g * n + x] - comp;
+        const float t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
+    }
+    float* o = shot->out + (size_t)x * ldo + iz;
+    *o = (acc_chunk || shot->acc) ? *o + sum : sum;
+}
+
+int launch_reduce_depth_shot(const float* partial, int ngroups, int n, const Shot3D* shot, int iz, int ldo,
+                             int acc_chunk, hipStream_t st)
+{
+    hipLaunchKernelGGL(reduce_depth_shot, dim3((n + 255) / 256), dim3(256), 0, st, partial, ngroups, n, shot, iz, ldo,
+                       acc_chunk);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
+__global__ __launch_bounds__(64) void set_shot3d(Shot3D* dst, Shot3D v)
+{
+    if (threadIdx.x == 0) *dst = v;
+}
+
+int launch_set_shot3d(Shot3D* dst, const Shot3D& v, hipStream_t st)
+{
+    hipLaunchKernelGGL(set_shot3d, dim3(1), dim3(64), 0, st, dst, v);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
 __global__ __launch_bounds__(256) void transpose_k(const float* __restrict__ in, int rows, int cols,
                                                    float* __restrict__ out)
 {

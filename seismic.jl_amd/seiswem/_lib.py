@@ -25,7 +25,7 @@ EXPORTS = (
     "swem_version", "swem_last_error", "swem_options_init", "swem_device_count",
     "swem_shot_main", "swem_angles_main", "swem_eshot_main", "swem_wem", "swem_wem_ex", "swem_shots", "swem_compute_angles", "swem_compute_angles_ex", "swem_ewem", "swem_eshots",
     "swem_plan_create", "swem_plan_destroy", "swem_plan_migrate", "swem_plan_migrate_each", "swem_plan_migrate_noalloc",
-    "swem_plan_model", "swem_plan_timing", "swem_plan_autotune", "swem_plan_bytes",
+    "swem_plan_model", "swem_plan_timing", "swem_plan_autotune", "swem_plan_autotune_ex", "swem_plan_set_graphs", "swem_plan_bytes",
     "swem_malloc", "swem_free", "swem_memcpy_htod", "swem_memcpy_dtoh", "swem_memcpy_dtod", "swem_memset", "swem_synchronize",
     "swem_sampling", "swem_source_index", "swem_pspi_tables", "swem_taper",
     "swem_seis_extent_init", "swem_seis_read_header", "swem_seis_read", "swem_seis_write",
@@ -68,6 +68,20 @@ class Timing(ctypes.Structure):
     _fields_ = [
         ("total_ms", ctypes.c_double), ("spectra_ms", ctypes.c_double), ("march_ms", ctypes.c_double),
         ("reduce_ms", ctypes.c_double), ("march_launches", ctypes.c_int), ("march_flops", ctypes.c_double),
+    ]
+
+
+TUNE_MAX = 48
+
+
+class TuneResult(ctypes.Structure):
+    """swem_tune_result: measured launch shapes of a plan."""
+
+    _fields_ = [
+        ("variant", ctypes.c_int), ("freqs_per_block", ctypes.c_int), ("chunk_bytes", ctypes.c_size_t),
+        ("best_ms", ctypes.c_float), ("ncand", ctypes.c_int),
+        ("cand_variant", ctypes.c_int * TUNE_MAX), ("cand_fpb", ctypes.c_int * TUNE_MAX),
+        ("cand_chunk_bytes", ctypes.c_size_t * TUNE_MAX), ("cand_ms", ctypes.c_float * TUNE_MAX),
     ]
 
 
@@ -168,6 +182,8 @@ def load():
         "swem_plan_model": ([vp, vp, f32p, f32p, ctypes.c_int, vp, vp], ctypes.c_int),
         "swem_plan_timing": ([vp, P(Timing)], ctypes.c_int),
         "swem_plan_autotune": ([vp, ctypes.c_int, i32p, f32p], ctypes.c_int),
+        "swem_plan_autotune_ex": ([vp, ctypes.c_int, P(TuneResult)], ctypes.c_int),
+        "swem_plan_set_graphs": ([vp, ctypes.c_int], ctypes.c_int),
         "swem_plan_bytes": ([vp, P(ctypes.c_size_t), P(ctypes.c_size_t)], ctypes.c_int),
         "swem_malloc": ([P(vp), ctypes.c_size_t], ctypes.c_int),
         "swem_free": ([vp], ctypes.c_int),

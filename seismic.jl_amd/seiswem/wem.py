@@ -172,6 +172,22 @@ class Plan:
         check(load().swem_plan_autotune(self._p, int(nshot), ctypes.byref(best), times))
         return {"best": best.value, "ms": {v: float(times[v]) for v in range(len(times)) if times[v] > 0}}
 
+    def set_graphs(self, enable: bool):
+        """3-D migration: replay each shot's launches as one captured graph (default) or launch them
+        one by one (swem_plan_set_graphs)."""
+        check(load().swem_plan_set_graphs(self._p, int(bool(enable))))
+
+    def autotune_shapes(self, nshot: int = 1) -> dict:
+        """swem_plan_autotune_ex: times the plan's candidate launch shapes (2-D: kernel variant x
+        frequencies per workgroup; 3-D: frequencies per y-pass workgroup x step state per launch)
+        and keeps the fastest.  Returns the choice and every candidate's march time."""
+        r = _lib.TuneResult()
+        check(load().swem_plan_autotune_ex(self._p, int(nshot), ctypes.byref(r)))
+        cands = [{"variant": r.cand_variant[i], "freqs_per_block": r.cand_fpb[i],
+                  "chunk_bytes": r.cand_chunk_bytes[i], "ms": r.cand_ms[i]} for i in range(r.ncand)]
+        return {"variant": r.variant, "freqs_per_block": r.freqs_per_block, "chunk_bytes": r.chunk_bytes,
+                "best_ms": r.best_ms, "candidates": cands}
+
     def bytes(self):
         a, b = ctypes.c_size_t(), ctypes.c_size_t()
         check(load().swem_plan_bytes(self._p, ctypes.byref(a), ctypes.byref(b)))

@@ -90,13 +90,40 @@ def test_shot_profile_ewem_stacks_shots(gpu):
     assert rel_l2(mpp, alone[0][0] + alone[1][0]) <= 1e-6
 
 
-def test_ewem_refuses_3d_and_mismatched_band(gpu):
+@pytest.mark.parametrize("adj", [True, False])
+def test_ewem_own_band_matches_oracle(gpu, adj):
+    """ewem.c samples its band its own way (ewem.c:43-52): ntfft = 2 truncf(padt nt / 2) and
+    ifmax = trunc(fmax dt ntfft) + 1, where wem.c keeps padt nt and casts fmax to an integer first.
+    padt 3, nt 255 (ntfft 764 against wem.c's 765) and fmax 30.5 give another band under each rule;
+    the elastic plans follow ewem.c's, as the oracle does."""
+    nx, nz, nt = 64, 24, 255
+    g = seiswem.WemGeometry(nt=nt, dt=0.004, nmx=nx, dmx=10.0, nz=nz, dz=10.0, fmin=0.0, fmax=30.5, pspi=False,
+                            padt=3, padx=2)
+    p = O.WemParams(nt=nt, dt=0.004, nmx=nx, dmx=10.0, nz=nz, dz=10.0, fmin=0.0, fmax=30.5, padt=3, padx=2,
+                    pspi=False)
+    s = g.sampling()
+    assert s["ntfft"] == 765 and int(2 * np.trunc(np.float32(3 * nt) / 2)) == 764
+    rng = np.random.default_rng(9)
+    vp = synth.blob_velocity(nx, nz, g.dmx, g.dz, rng, nblobs=3, v0=2200.0)
+    vs = (vp / 1.8).astype(np.float32)
+    wav = synth.wavelet_trace(nt, g.dt, 15.0)
+    sx = 250.0
+    if adj:
+        ux, uz = rng.standard_normal((2, g.nm, g.nt)).astype(np.float32)
+        mpp, _, mps2 = run(g, vp, vs, wav, sx, True, ux, np.zeros_like(ux), uz)
+        rpp, rps2 = O.ewem(p, vp, vs, wav, sx, ux=ux, uz=uz, adj=True)
+        e1, e2 = rel_l2(mpp, rpp), rel_l2(mps2, rps2)
+    else:
+        mpp, mps2 = rng.standard_normal((2, g.nm, g.nz)).astype(np.float32)
+        ux, _, uz = run(g, vp, vs, wav, sx, False, mpp, np.zeros_like(mpp), mps2)
+        rx, rz = O.ewem(p, vp, vs, wav, sx, mpp=mpp, mps2=mps2, adj=False)
+        e1, e2 = rel_l2(ux, rx), rel_l2(uz, rz)
+    print(f"ewem band of ewem.c, adj={adj}: rel-L2 {e1:.3e}, {e2:.3e}", flush=True)
+    assert e1 <= TOL and e2 <= TOL
+
+
+def test_ewem_refuses_3d(gpu):
     g, p, vp, vs, wav, rng = case(64)
-    z = np.zeros((g.nm, g.nt), np.float32)
-    with pytest.raises(seiswem.SwemError) as e:  # fmax 2.5: ewem.c truncates 2.5 dt ntfft, wem.c casts 2.5 to 2
-        seiswem.ewem(z, z, z, None, None, None, wav, g.nt, 0.0, g.dt, g.nmx, 0.0, g.dmx, 1, 0.0, g.dmx, 100.0, 0.0,
-                     g.nz, 0.0, g.dz, 0.0, 0.0, vp, vs, 0.0, 2.5, 2, 2, True)
-    assert e.value.code == seiswem.SWEM_EUNSUPPORTED
     with pytest.raises(seiswem.SwemError) as e:
         seiswem.ewem(None, None, None, None, None, None, wav, g.nt, 0.0, g.dt, 16, 0.0, 10.0, 16, 0.0, 10.0, 50.0,
                      50.0, 8, 0.0, 10.0, 0.0, 0.0, np.full((256, 8), 2000, np.float32),

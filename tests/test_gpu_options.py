@@ -167,3 +167,45 @@ def test_concurrent_shotwem_main_from_threads(gpu, monkeypatch):
         print(f"{mname}: rel-L2 {err:.3e}")
         assert err <= 1e-4
     shutil.rmtree(tmp_path, ignore_errors=True)
+
+
+def near_cutoff_case():
+    """2-D split-step, v = 2000 m/s everywhere (pd = 0), nx 1024 (nkx 2048, dkx = 2 pi / 20480 m),
+    nz 200, 40-44 Hz (bins 0.125 Hz apart).  The record is a sum of plane waves cos(w t - kx x) at
+    the three bins whose grid wavenumbers come closest below the cutoff w / v: at 40.625 and
+    43.75 Hz, w / v is exactly 416 and 448 dkx, so s = w^2 p^2 - kx^2 is zero and its float
+    evaluation is rounding alone; at 42.875 Hz kx / (w / v) = 0.99991."""
+    g = seiswem.WemGeometry(nt=1000, dt=0.004, nmx=1024, dmx=10.0, nz=200, dz=10.0, fmin=40.0, fmax=44.0,
+                            pspi=False)
+    s = g.sampling()
+    vel = np.full((g.nm, g.nz), 2000.0, np.float32)
+    wav = synth.wavelet_trace(g.nt, g.dt, 40.0)
+    t = np.arange(g.nt) * g.dt
+    x = np.arange(g.nmx) * g.dmx
+    dkx = 2 * np.pi / (s["nkx"] * g.dmx)
+    fb = np.arange(s["ifmin"], s["ifmax"]) / (s["ntfft"] * g.dt)
+    r = 2 * np.pi * fb / 2000.0 / dkx  # w / v in units of dkx
+    d = np.zeros((g.nm, g.nt))
+    kxs = []
+    for i in np.argsort(r - np.floor(r), kind="stable")[:3]:
+        w, kx = 2 * np.pi * fb[i], np.floor(r[i]) * dkx
+        kxs.append(kx / (w / 2000.0))
+        d += np.cos(w * t[None, :] - kx * x[:, None])
+    return g, vel, wav, d.astype(np.float32), kxs
+
+
+def test_precision_near_evanescent_cutoff(gpu):
+    """Per-image error against the FP64 oracle of the two GPU variants on waves near the cutoff:
+    the FP64 kz tables keep the default within the 1e-4 bar, and the naive FP32 phases
+    (s = (w*w)*p2 - kx*kx in float, wem.c:341) carry the cancellation into the image."""
+    g, vel, wav, d, kxs = near_cutoff_case()
+    p = oparams(g)
+    m64 = O.wem(p, vel, wav, 5120.0, d=d, adj=True)
+    mk = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=[5120.0])
+    mf = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=[5120.0], precision=seiswem.PREC_FP32)
+    m32 = O.wem(p, vel, wav, 5120.0, d=d, adj=True, f64=False)
+    ek, ef, eo = rel_l2(mk, m64), rel_l2(mf, m64), rel_l2(m32, m64)
+    print(f"near cutoff (kx / (w/v) = {', '.join(f'{k:.5f}' for k in kxs)}): FP64 kz {ek:.3e}, naive FP32 {ef:.3e}, "
+          f"FP32 oracle {eo:.3e}")
+    assert ek <= 1e-4
+    assert ef > 3 * ek

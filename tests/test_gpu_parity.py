@@ -212,6 +212,40 @@ def test_packed_compat_migration(gpu, case):
     assert np.array_equal(m_auto, m_packed)
 
 
+
+def test_autotune_shapes_2d_times_group_sizes(gpu):
+    """swem_plan_autotune_ex on a 2-D SSF plan: variants with several frequency group sizes are
+    timed, the fastest is kept, and migration after tuning still matches the oracle."""
+    g, vel, wav, rng = small_case(False, nx=128, nz=30)
+    plan = seiswem.Plan(g, vel, wav)
+    r = plan.autotune_shapes(nshot=2)
+    fpbs = {c["freqs_per_block"] for c in r["candidates"]}
+    print(f"2-D shapes: best variant {r['variant']} x {r['freqs_per_block']} in {r['best_ms']:.3f} ms; "
+          f"{len(r['candidates'])} candidates, group sizes {sorted(fpbs)}")
+    assert len(r["candidates"]) >= 3 and len(fpbs) >= 2
+    assert r["best_ms"] == min(c["ms"] for c in r["candidates"])
+    d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
+    m = plan.migrate(seiswem.DeviceArray((1, g.nm, g.nt), d[None]), [300.0]).download()
+    m_ref = O.wem(oparams(g, 0), vel, wav, 300.0, d=d, adj=True)
+    assert rel_l2(m, m_ref) <= TOL
+
+
+def test_autotune_shapes_3d_and_parity(gpu):
+    """3-D: frequencies per y-pass workgroup x step state per launch, then parity with the oracle."""
+    g, vel, refl, wav, sx, sy = synth.config3(seed=2, nx=64, ny=64, nz=12, nt=128, fmax=25.0)
+    plan = seiswem.Plan(g, vel, wav)
+    r = plan.autotune_shapes(nshot=1)
+    print(f"3-D shapes: best F={r['freqs_per_block']} chunk {r['chunk_bytes'] >> 20} MiB in {r['best_ms']:.3f} ms; "
+          + ", ".join(f"{c['freqs_per_block']}/{c['chunk_bytes'] >> 20}:{c['ms']:.2f}" for c in r["candidates"]))
+    assert len(r["candidates"]) == 4 * 3 and r["chunk_bytes"] in (64 << 20, 128 << 20, 256 << 20)
+    d = np.random.default_rng(4).standard_normal((g.nm, g.nt)).astype(np.float32)
+    m = plan.migrate(seiswem.DeviceArray((1, g.nm, g.nt), d[None]), sx, sy).download()
+    po = O.WemParams(nt=g.nt, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nmy=g.nmy, dmy=g.dmy, nz=g.nz, dz=g.dz, fmin=g.fmin,
+                     fmax=g.fmax, padt=g.padt, padx=g.padx, pspi=False)
+    m_ref = O.wem(po, vel, wav, float(sx[0]), float(sy[0]), d=d, adj=True)
+    assert rel_l2(m, m_ref) <= TOL
+
+
 @pytest.mark.parametrize("pspi,mode", [(True, 0), (False, 0)])
 def test_autotune_picks_the_fastest_measured_variant_and_keeps_parity(gpu, pspi, mode):
     """swem_plan_autotune times the distinct kernel variants for the plan's nkx and keeps the
