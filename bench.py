@@ -1,7 +1,7 @@
 """Benchmark: migrated shots/s on one MI355X.
 
-Default workload (--workload pspi2d) is the headline one; --workload ssf3d and --workload
-serving measure the 3-D and the serving workloads (one JSON line each).
+Default workload (--workload pspi2d) is the headline one; --workload ssf3d, serving, lsm, angles,
+elastic and elastic3d measure the other workloads (one JSON line each).
 
 Workload (config 2): nx=1024, dx=10 m; nz=500, dz=5 m; nt=2000, dt=2 ms; 0-80 Hz (641 bins);
 4 reference velocities; 1800 m/s + 12 Gaussian blobs; 64 shots 160 m apart; 25 Hz Ricker.  The
@@ -478,6 +478,63 @@ def bench_elastic(args):
     print(json.dumps(out))
 
 
+def bench_elastic3d(args):
+    """3-D elastic migration (ewem.c with nmy > 1) on the 3-D SSF grid: nx=ny=256, dx=dy=12.5,
+    nz=300, dz=10, nt=1500, dt=4 ms, 3-50 Hz (ewem.c's band: ntfft 3000, bins 36-200, nkx=nky=512),
+    vp = the 3-D generator's velocity, vs = vp / 1.8, synthetic random ux, uz records; one shot per
+    call of swem_ewem (host arrays in and out).  CPU baseline: the FP32 oracle on one shot, one
+    frequency bin and the first depths at full nx, ny, scaled by bins and depths."""
+    import seiswem
+    from seiswem import synth
+
+    g, vp, _, wav, sxa, sya = synth.config3()
+    vs = (vp / 1.8).astype(np.float32)
+    rng = np.random.default_rng(0)
+    comps = rng.standard_normal((3, g.nm, g.nt)).astype(np.float32)
+    sx, sy = float(sxa[0]), float(sya[0])
+
+    def shot():
+        return seiswem.ewem(comps[0], comps[1], comps[2], None, None, None, wav, g.nt, 0.0, g.dt, g.nmx, 0.0, g.dmx,
+                            g.nmy, 0.0, g.dmy, sx, sy, g.nz, 0.0, g.dz, 0.0, 0.0, vp, vs, g.fmin, g.fmax, 2, 2, True)
+
+    for _ in range(args.warmup):
+        shot()
+    k = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        shot()
+    el = time.perf_counter() - t0
+    ntfft = int(2 * np.trunc(np.float32(2 * g.nt) / 2))
+    ifmin, ifmax = int(np.trunc(np.float32(g.fmin) * np.float32(g.dt) * ntfft)), \
+        int(np.trunc(np.float32(g.fmax) * np.float32(g.dt) * ntfft)) + 1
+    nf = ifmax - ifmin
+    out = {
+        "metric": "elastic migration: shots/sec (ewem, 3-D)", "value": k / el, "unit": "shots/s", "n_gpus": 1,
+        "steps": k, "warmup": args.warmup, "ms_per_step": el / k * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (3-D generator velocity, vs = vp / 1.8, random records)",
+        "config": {"workload": f"3-D ewem, nx=ny=256 nz=300 nt=1500 dt=4 ms, 3-50 Hz ({nf} bins), nkx=nky=512, "
+                               "one shot per call (swem_ewem, host arrays in and out)"},
+    }
+    if not args.no_cpu:
+        import oracle as O
+        nzs, nts = 1, 64  # one depth; 64-sample records (ntfft 128: the 50 Hz bin alone)
+        p = O.WemParams(nt=nts, dt=g.dt, nmx=g.nmx, dmx=g.dmx, nmy=g.nmy, dmy=g.dmy, nz=nzs, dz=g.dz, fmin=50.0,
+                        fmax=50.0, padt=2, padx=2, pspi=False)
+        nb = 1
+        t0 = time.perf_counter()
+        O.ewem(p, np.ascontiguousarray(vp[:, :nzs]), np.ascontiguousarray(vs[:, :nzs]), wav[:nts], sx, sy,
+               ux=np.ascontiguousarray(comps[0][:, :nts]), uz=np.ascontiguousarray(comps[2][:, :nts]), adj=True,
+               f64=False)
+        tc = time.perf_counter() - t0
+        scale = (nb / nf) * (nzs / g.nz)
+        out["cpu_baseline"] = {"value": scale / tc, "unit": "shots/s", "cores": 1, "kind": "port",
+                               "sample": f"1 shot, {nb} of {nf} frequency bins (50 Hz) and {nzs} of {g.nz} depths "
+                                         f"at full nx, ny (nkx=nky=512), records cut to {nts} samples so the record "
+                                         f"transforms stay out of the sample, {tc:.2f} s, scaled by bins and depths "
+                                         "(the FP32 oracle is serial)"}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -496,7 +553,8 @@ def main():
                          "the Bluestein kernels)")
     ap.add_argument("--plain-shots", type=int, default=4,
                     help="pspi2d: shots timed one per call on the plain and resident single-shot paths (0: skip)")
-    ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm", "angles", "elastic"), default="pspi2d")
+    ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm", "angles", "elastic", "elastic3d"),
+                    default="pspi2d")
     ap.add_argument("--no-3d", action="store_true", help="pspi2d: leave out the 3-D part of the line")
     ap.add_argument("--no-serving", action="store_true", help="pspi2d: leave out the serving part of the line")
     ap.add_argument("--steps-3d", type=int, default=3, help="pspi2d: 3-D shots timed for the line's 3-D part")
@@ -522,6 +580,8 @@ def main():
         return bench_angles(args)
     if args.workload == "elastic":
         return bench_elastic(args)
+    if args.workload == "elastic3d":
+        return bench_elastic3d(args)
 
     import seiswem
     from seiswem import synth

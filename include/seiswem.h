@@ -150,11 +150,14 @@ int swem_compute_angles_ex(float* angx, float* angy, const float* wav, const swe
 /* Replaces ewem() (ewem.c:3-15), elastic shot-profile migration (adj=1: components ux, uy, uz
  * [nm][nt] -> images mpp, mps1, mps2 [nm][nz]) and modelling (adj=0: the reverse), with the P
  * source, the P / S separation of the receiver components at every depth and split-step
- * propagation with vp and vs (ewem.c:255-741).  2-D (nmy = 1); as in the reference, uy is not
- * used (migration) and returned zero (modelling), and mps1 is returned zero and not used.
+ * propagation with vp and vs (ewem.c:255-741).  2-D (nmy = 1) and 3-D (nmy > 1: the source at
+ * igx nmy + igy, the separation matrix of kx alone applied on the 2-D transform, as
+ * elastic_separate_3d does, ewem.c:553-741); as in the reference, uy is not used (migration) and
+ * returned zero (modelling), and mps1 is returned zero and not used.
  * The band is sampled as ewem.c samples it (ewem.c:43-52: an even ntfft = 2 truncf(padt nt / 2),
  * ifmax = trunc(fmax dt ntfft) + 1), which differs from wem.c's rule for odd padt nt or a
- * fractional fmax / fmin.  3-D (nmy > 1) returns SWEM_EUNSUPPORTED. */
+ * fractional fmax / fmin.  3-D takes the 3-D march's transform lengths (nkx, nky up to 512, powers
+ * of two up to 1024); a source outside the grid returns SWEM_EINVAL. */
 int swem_ewem(float* ux, float* uy, float* uz, float* mpp, float* mps1, float* mps2, const float* wav,
               int nt, float ot, float dt,
               int nmx, float omx, float dmx,

@@ -1816,6 +1816,52 @@ int swem_compute_angles(float* angx, float* angy, const float* wav, int nt, floa
 
 // nshot shots of elastic migration (records [nshot][nm][nt] -> images [nm][nz] summed over shots)
 // or modelling (images -> records of every shot), all shots of a chunk in one launch.
+// Host tables of elastic shots, 2-D and 3-D: the Fourier bases without w^2 (ewem.c:116-140,
+// 219-241: R2C / sqrt(ntfft) and the c2r of the band), the layer velocities of the separation in
+// float as ewem.c:64-81 forms them (1 / po, po the inverse of the mean velocity over the nmx nmy
+// traces), and max|wav| / sqrt(ntfft) (ewem.c:146).
+struct EwemTables {
+    std::vector<float> badj, bfwd, vp0, vs0;
+    float maxs = 0.f;
+};
+
+static int ewem_tables(const swem_plan* pp, const swem_plan* ps, const float* wav, EwemTables& t)
+{
+    const int nt = pp->g.nt, nf = pp->nf, nz = pp->g.nz, nm = pp->g.nmx * pp->g.nmy;
+    const int n = pp->s.fop_n;
+    const double rnt = 1.0 / std::sqrt((double)pp->s.ntfft);
+    t.badj.assign((size_t)nt * 2 * nf, 0.f);
+    t.bfwd.assign((size_t)2 * nf * nt, 0.f);
+    for (int f = 0; f < nf; f++) {
+        const int iw = pp->s.ifmin + f;
+        const double cf = (iw == 0 || 2 * iw == n) ? 1.0 : 2.0;
+        for (int it = 0; it < nt; it++) {
+            const long long e = ((long long)iw * it) % n;
+            const double ang = 2.0 * kPi * (double)e / (double)n, c = std::cos(ang), sn = std::sin(ang);
+            t.badj[(size_t)it * 2 * nf + 2 * f] = (float)(rnt * c);
+            t.badj[(size_t)it * 2 * nf + 2 * f + 1] = (float)(-rnt * sn);
+            t.bfwd[(size_t)(2 * f) * nt + it] = (float)(cf * rnt * c);
+            t.bfwd[(size_t)(2 * f + 1) * nt + it] = (float)(-cf * rnt * sn);
+        }
+    }
+    t.vp0.assign(nz, 0.f);
+    t.vs0.assign(nz, 0.f);
+    for (int k = 0; k < 2; k++) {
+        const std::vector<float>& v = k == 0 ? pp->vel : ps->vel;
+        for (int iz = 0; iz < nz; iz++) {
+            float sum = 0.f;
+            for (int ix = 0; ix < nm; ix++) sum += v[(size_t)ix * nz + iz];
+            const float po = 1.f / (sum / (float)nm);
+            (k == 0 ? t.vp0 : t.vs0)[iz] = (float)(1.0 / (double)po);
+        }
+    }
+    const float rntf = std::sqrt((float)pp->s.ntfft);
+    t.maxs = 0.f;
+    for (int it = 0; it < nt; it++) t.maxs = std::max(t.maxs, std::fabs(wav[it]) / rntf);
+    if (!(t.maxs > 0.f)) return set_error(SWEM_EINVAL, "ewem: the wavelet is zero");
+    return SWEM_OK;
+}
+
 static int eshots_impl(swem_plan* pp, swem_plan* ps, int nshot, const int* igx, float* ux, float* uy, float* uz,
                        float* mpp, float* mps1, float* mps2, const float* wav, int adj)
 {
@@ -1834,37 +1880,10 @@ static int eshots_impl(swem_plan* pp, swem_plan* ps, int nshot, const int* igx, 
     if (nf < 1) return SWEM_OK;
     if (pp->nchunks != ps->nchunks || pp->nfc != ps->nfc)
         return set_error(SWEM_EINVAL, "ewem: the P and S plans chunk the band differently");
-    // bases without w^2 (ewem.c:116-140, 219-241): R2C / sqrt(ntfft) and the c2r of the band
-    const int n = pp->s.fop_n;
-    const double rnt = 1.0 / std::sqrt((double)pp->s.ntfft);
-    std::vector<float> badj((size_t)nt * 2 * nf), bfwd((size_t)2 * nf * nt);
-    for (int f = 0; f < nf; f++) {
-        const int iw = pp->s.ifmin + f;
-        const double cf = (iw == 0 || 2 * iw == n) ? 1.0 : 2.0;
-        for (int t = 0; t < nt; t++) {
-            const long long e = ((long long)iw * t) % n;
-            const double ang = 2.0 * kPi * (double)e / (double)n, c = std::cos(ang), sn = std::sin(ang);
-            badj[(size_t)t * 2 * nf + 2 * f] = (float)(rnt * c);
-            badj[(size_t)t * 2 * nf + 2 * f + 1] = (float)(-rnt * sn);
-            bfwd[(size_t)(2 * f) * nt + t] = (float)(cf * rnt * c);
-            bfwd[(size_t)(2 * f + 1) * nt + t] = (float)(-cf * rnt * sn);
-        }
-    }
-    // layer velocities of the separation, in float as ewem.c:64-81 forms them
-    std::vector<float> vp0(nz), vs0(nz);
-    for (int k = 0; k < 2; k++) {
-        const std::vector<float>& v = k == 0 ? pp->vel : ps->vel;
-        for (int iz = 0; iz < nz; iz++) {
-            float sum = 0.f;
-            for (int ix = 0; ix < nmx; ix++) sum += v[(size_t)ix * nz + iz];
-            const float po = 1.f / (sum / (float)nmx);
-            (k == 0 ? vp0 : vs0)[iz] = (float)(1.0 / (double)po);
-        }
-    }
-    float maxs = 0.f;
-    const float rntf = std::sqrt((float)pp->s.ntfft);
-    for (int t = 0; t < nt; t++) maxs = std::max(maxs, std::fabs(wav[t]) / rntf);
-    if (!(maxs > 0.f)) return set_error(SWEM_EINVAL, "ewem: the wavelet is zero");
+    EwemTables tb;
+    SWEM_TRY(ewem_tables(pp, ps, wav, tb));
+    const std::vector<float>&badj = tb.badj, &bfwd = tb.bfwd, &vp0 = tb.vp0, &vs0 = tb.vs0;
+    const float maxs = tb.maxs;
     DevBuf<float> dbadj, dbfwd, dvp0, dvs0, drx, drz, ppart, spart, dmpp, dmps, mppT, mpsT;
     DevBuf<float2> spx, spz, smig, dgx, dgz;
     DevBuf<int> dig;
@@ -1958,23 +1977,203 @@ static int eshots_impl(swem_plan* pp, swem_plan* ps, int nshot, const int* igx, 
     return SWEM_OK;
 }
 
+// 3-D elastic shots (ewem.c:3-366 with nmy > 1): per shot and frequency chunk, the P source and
+// the receiver pair marched through the two-pass depth step of the 3-D kernels (launch_ewem3d):
+// migration per depth: source kz x-pass, separation, truncation, kz x-pass of the potentials (vp,
+// vs), imaging y-pass, combination, truncation, and the depth's two images summed over the chunk's
+// frequency groups; modelling: the source over all depths first (stored), then the receiver upward:
+// separation, injection y-pass, kz x-pass, truncation, combination, truncation (or the output).
+static int eshots3d_impl(swem_plan* pp, swem_plan* ps, int nshot, const float* sx, const float* sy, float* ux,
+                         float* uy, float* uz, float* mpp, float* mps1, float* mps2, const float* wav, int adj)
+{
+    hipStream_t st = nullptr;
+    const swem_geometry& g = pp->g;
+    const int nmx = g.nmx, nmy = g.nmy, nm = nmx * nmy, nz = g.nz, nt = g.nt, nf = pp->nf;
+    const size_t nmz = (size_t)nm * nz, nmt = (size_t)nm * nt;
+    if (adj) {
+        std::fill(mpp, mpp + nmz, 0.f);
+        std::fill(mps1, mps1 + nmz, 0.f);
+        std::fill(mps2, mps2 + nmz, 0.f);
+    } else {
+        std::fill(ux, ux + (size_t)nshot * nmt, 0.f);
+        std::fill(uy, uy + (size_t)nshot * nmt, 0.f);
+        std::fill(uz, uz + (size_t)nshot * nmt, 0.f);
+    }
+    std::vector<int> isrc(nshot);
+    for (int i = 0; i < nshot; i++) {
+        const int ix = (int)std::trunc((sx[i] - g.omx) / g.dmx);  // ewem.c:99-100
+        const int iy = (int)std::trunc(((sy ? sy[i] : 0.f) - g.omy) / g.dmy);
+        if (ix < 0 || ix >= nmx || iy < 0 || iy >= nmy)
+            return set_error(SWEM_EINVAL, "ewem: source (%g, %g) outside the grid", (double)sx[i], sy ? (double)sy[i] : 0.0);
+        isrc[i] = ix * nmy + iy;
+    }
+    if (nf < 1) return SWEM_OK;
+    EwemTables tb;
+    SWEM_TRY(ewem_tables(pp, ps, wav, tb));
+    const std::vector<float>&badj = tb.badj, &bfwd = tb.bfwd, &vp0 = tb.vp0, &vs0 = tb.vs0;
+    const float maxs = tb.maxs;
+    // frequencies per launch: three slots of step state per frequency (plus the stored source when
+    // modelling) within the work budget, and the step state within the last-level cache bound
+    const int F = std::min(8, pp->F3);
+    const size_t slot = (size_t)nmx * pp->s.nky * sizeof(float2);
+    const size_t per_item = 3 * slot + (adj ? 2 * (size_t)nm * sizeof(float) / F + 1 : nmz * sizeof(float2));
+    size_t nfc = std::max<size_t>(1, pp->budget / per_item);
+    nfc = std::min(nfc, std::max<size_t>(1, pp->state3d_bytes / (3 * slot)));
+    if (nfc < (size_t)nf) nfc = std::max<size_t>(F, nfc / F * F);
+    nfc = std::min<size_t>(nfc, nf);
+    const size_t gmax = (nfc + F - 1) / F;
+    DevBuf<float> dbadj, dbfwd, dvp0, dvs0, drx, drz, ppart, spart, dmpp, dmps, mppT, mpsT;
+    DevBuf<float2> W, spx, spz, smig, dgx, dgz;
+    SWEM_TRY(dvp0.upload(vp0.data(), nz));
+    SWEM_TRY(dvs0.upload(vs0.data(), nz));
+    SWEM_TRY(W.alloc(nfc * 3 * (size_t)nmx * pp->s.nky));
+    SWEM_TRY(drx.alloc(nmt));
+    SWEM_TRY(drz.alloc(nmt));
+    if (adj) {
+        SWEM_TRY(dbadj.upload(badj.data(), badj.size()));
+        SWEM_TRY(spx.alloc((size_t)nm * nf));
+        SWEM_TRY(spz.alloc((size_t)nm * nf));
+        SWEM_TRY(ppart.alloc(gmax * nm));
+        SWEM_TRY(spart.alloc(gmax * nm));
+        SWEM_TRY(dmpp.alloc(nmz));
+        SWEM_TRY(dmps.alloc(nmz));
+    } else {
+        SWEM_TRY(dbfwd.upload(bfwd.data(), bfwd.size()));
+        std::vector<float> t1(nmz), t2(nmz);
+        for (int ix = 0; ix < nm; ix++)
+            for (int iz = 0; iz < nz; iz++) {
+                t1[(size_t)iz * nm + ix] = mpp[(size_t)ix * nz + iz];
+                t2[(size_t)iz * nm + ix] = mps2[(size_t)ix * nz + iz];
+            }
+        SWEM_TRY(mppT.upload(t1.data(), nmz));
+        SWEM_TRY(mpsT.upload(t2.data(), nmz));
+        SWEM_TRY(smig.alloc(nfc * nmz));
+        SWEM_TRY(dgx.alloc((size_t)nm * nf));
+        SWEM_TRY(dgz.alloc((size_t)nm * nf));
+    }
+    bool first_image = true;
+    for (int s = 0; s < nshot; s++) {
+        if (adj) {
+            SWEM_HIP(hipMemcpyAsync(drx.p, ux + (size_t)s * nmt, nmt * sizeof(float), hipMemcpyHostToDevice, st));
+            SWEM_HIP(hipMemcpyAsync(drz.p, uz + (size_t)s * nmt, nmt * sizeof(float), hipMemcpyHostToDevice, st));
+            SWEM_TRY(launch_sgemm(nm, 2 * nf, nt, drx.p, nt, dbadj.p, 2 * nf, reinterpret_cast<float*>(spx.p), 2 * nf, st));
+            SWEM_TRY(launch_sgemm(nm, 2 * nf, nt, drz.p, nt, dbadj.p, 2 * nf, reinterpret_cast<float*>(spz.p), 2 * nf, st));
+        }
+        for (int f0 = 0; f0 < nf; f0 += (int)nfc) {
+            March3DArgs ap = base3(pp), as = base3(ps);
+            ap.f0 = as.f0 = f0;
+            ap.nitem = as.nitem = std::min((int)nfc, nf - f0);
+            Ewem3DArgs e{};
+            e.W = W.p;
+            e.igx = isrc[s];
+            e.sep_adj = adj;
+            e.dwf = pp->s.dw;
+            e.dkxf = pp->s.dkx;
+            e.inv_maxs = 1.f / maxs;
+            e.vp0 = dvp0.p;
+            e.vs0 = dvs0.p;
+            e.spx = spx.p;
+            e.spz = spz.p;
+            e.ppart = ppart.p;
+            e.spart = spart.p;
+            e.smig = smig.p;
+            e.mppT = mppT.p;
+            e.mpsT = mpsT.p;
+            e.dgx = dgx.p;
+            e.dgz = dgz.p;
+            SWEM_TRY(launch_ewem3d(0, ap, as, e, F, 0, st));
+            auto at = [&](int iz) {
+                ap.iz = as.iz = iz;
+                e.ds = stepped_at(pp, iz, g.sz);
+                e.dr = stepped_at(pp, iz, g.gz);
+                e.next_s = stepped_at(pp, iz + 1, g.sz);
+            };
+            auto xpass = [&](int mode) {
+                e.mode = mode;
+                return launch_ewem3d(1, ap, as, e, F, 0, st);
+            };
+            const int ngr = (ap.nitem + F - 1) / F;
+            if (adj) {
+                e.conj_src = 1;
+                e.conj_rec = 0;
+                for (int iz = 0; iz < nz; iz++) {
+                    at(iz);
+                    if (e.ds) SWEM_TRY(xpass(3));
+                    if (e.dr) {
+                        SWEM_TRY(xpass(0));
+                        SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 1, st));
+                        SWEM_TRY(xpass(2));
+                    }
+                    SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 0, st));
+                    if (e.dr) {
+                        SWEM_TRY(xpass(1));
+                        SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 1, st));
+                    }
+                    const int acc = !first_image || f0 > 0;
+                    SWEM_TRY(launch_reduce_depth(ppart.p, ngr, nm, dmpp.p + iz, nz, acc, st));
+                    SWEM_TRY(launch_reduce_depth(spart.p, ngr, nm, dmps.p + iz, nz, acc, st));
+                }
+            } else {
+                e.conj_src = 1;
+                e.conj_rec = 1;
+                for (int iz = 0; iz < nz; iz++) {  // the source, stored per depth
+                    at(iz);
+                    if (e.ds) SWEM_TRY(xpass(3));
+                    SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 2, st));
+                }
+                int top = -1;  // the last (uppermost) receiver depth
+                for (int iz = 0; iz < nz; iz++)
+                    if (stepped_at(pp, iz, g.gz)) {
+                        top = iz;
+                        break;
+                    }
+                for (int iz = nz - 1; iz >= 0; iz--) {
+                    at(iz);
+                    if (!e.dr) continue;
+                    SWEM_TRY(xpass(0));
+                    SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 3, st));
+                    SWEM_TRY(xpass(2));
+                    SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 1, st));
+                    SWEM_TRY(xpass(1));
+                    if (iz != top) SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 1, st));
+                }
+                // the modelled spectra: the last combination's state (or the zero initial state)
+                SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 4, st));
+            }
+        }
+        if (adj) first_image = false;
+        if (!adj) {
+            SWEM_TRY(launch_sgemm(nm, nt, 2 * nf, reinterpret_cast<const float*>(dgx.p), 2 * nf, dbfwd.p, nt, drx.p, nt, st));
+            SWEM_TRY(launch_sgemm(nm, nt, 2 * nf, reinterpret_cast<const float*>(dgz.p), 2 * nf, dbfwd.p, nt, drz.p, nt, st));
+            SWEM_HIP(hipMemcpyAsync(ux + (size_t)s * nmt, drx.p, nmt * sizeof(float), hipMemcpyDeviceToHost, st));
+            SWEM_HIP(hipMemcpyAsync(uz + (size_t)s * nmt, drz.p, nmt * sizeof(float), hipMemcpyDeviceToHost, st));
+        }
+        SWEM_HIP(hipStreamSynchronize(st));  // the record buffers are reused by the next shot
+    }
+    if (adj) {
+        SWEM_HIP(hipMemcpyAsync(mpp, dmpp.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
+        SWEM_HIP(hipMemcpyAsync(mps2, dmps.p, nmz * sizeof(float), hipMemcpyDeviceToHost, st));
+    }
+    SWEM_HIP(hipStreamSynchronize(st));
+    return SWEM_OK;
+}
+
 int swem_eshots(const swem_geometry* g, const float* vp, const float* vs, const float* wav, const float* sx,
                 const float* sy, int nshot, float* ux, float* uy, float* uz, float* mpp, float* mps1, float* mps2,
                 int adj, const swem_options* opt)
 {
     if (!g || !vp || !vs || !wav || !sx || !ux || !uy || !uz || !mpp || !mps1 || !mps2 || nshot < 1)
         return set_error(SWEM_EINVAL, "ewem: null argument or nshot < 1");
-    if (g->nmy > 1) return set_error(SWEM_EUNSUPPORTED, "ewem: 3-D (nmy > 1) is not available in this build");
     swem_geometry ge = *g;
     ge.pspi = 0;
     int ntfft, ifmin, ifmax;
     ewem_sampling(ge, ntfft, ifmin, ifmax);
+    const bool three_d = ge.nmy > 1;
     std::vector<int> igx(nshot);
-    for (int i = 0; i < nshot; i++) {
+    for (int i = 0; i < nshot && !three_d; i++) {
         igx[i] = (int)std::trunc((sx[i] - ge.omx) / ge.dmx);  // ewem.c:99
         if (igx[i] < 0 || igx[i] >= ge.nmx) return set_error(SWEM_EINVAL, "ewem: source x %g outside the grid", (double)sx[i]);
     }
-    (void)sy;
     swem_options o;
     if (opt) o = *opt;
     else swem_options_init(&o);
@@ -1985,7 +2184,9 @@ int swem_eshots(const swem_geometry* g, const float* vp, const float* vs, const 
     if (!rc && (pp->s.ntfft != ntfft || pp->s.ifmin != ifmin || pp->s.ifmax != ifmax))
         rc = set_error(SWEM_EDEVICE, "ewem: internal band mismatch (ntfft %d, [%d, %d)) against (%d, [%d, %d))", ntfft,
                        ifmin, ifmax, pp->s.ntfft, pp->s.ifmin, pp->s.ifmax);
-    if (!rc) rc = eshots_impl(pp, ps, nshot, igx.data(), ux, uy, uz, mpp, mps1, mps2, wav, adj);
+    if (!rc)
+        rc = three_d ? eshots3d_impl(pp, ps, nshot, sx, sy, ux, uy, uz, mpp, mps1, mps2, wav, adj)
+                     : eshots_impl(pp, ps, nshot, igx.data(), ux, uy, uz, mpp, mps1, mps2, wav, adj);
     if (ps) swem_plan_destroy(ps);
     swem_plan_destroy(pp);
     return rc;

@@ -201,6 +201,35 @@ struct March3DArgs {
     const uint8_t* sel2;
     const float* w2;
 };
+// 3-D elastic WEM (ewem.c:3-366 with nmy > 1): three wavefield slots per frequency item,
+// W [nitem][3][nmx][nky] (y transformed, x in space): 0 the P source, 1 and 2 the receiver pair
+// (ux, uz before the separation and after the combination; up, us2 between).
+struct Ewem3DArgs {
+    float2* W;
+    int igx;                // source trace igx nmy + igy (ewem.c:99-108)
+    int sep_adj;            // matrix branch of the separation / combination (1: migration)
+    int ds, dr, next_s;     // source / receiver stepped at this depth; source stepped at the next
+    int conj_src, conj_rec; // x-pass: conj(L) for the -dz steps
+    int mode;               // x-pass: 0 separation, 1 combination, 2 kz phases of the pair, 3 of the source
+    float dwf, dkxf;        // float dw and dkx, as ewem.c forms w and the separation's kx
+    float inv_maxs;         // 1 / (max|wav| / sqrt(ntfft)) (ewem.c:146)
+    const float* vp0;       // [nz] 1 / po_p (separation)
+    const float* vs0;       // [nz] 1 / po_s
+    const float2* spx;      // [nm][nf] R2C(ux) / sqrt(ntfft) (migration)
+    const float2* spz;      // [nm][nf]
+    float* ppart;           // [ngroups][nm] partial P image of one depth (migration)
+    float* spart;           // [ngroups][nm] partial S image
+    float2* smig;           // [nitem][nz][nm] stored source / max|wav| (modelling)
+    const float* mppT;      // [nz][nm] images (modelling input)
+    const float* mpsT;
+    float2* dgx;            // [nm][nf] modelled spectra (modelling output)
+    float2* dgz;
+};
+// which: 0 initial wavefields, 1 x-pass (e.mode), 2 y-pass (phase 0 migration imaging, 1 receiver
+// truncation to the nmx x nmy grid, 2 modelling source, 3 modelling receiver injection, 4 modelling
+// output).  ap / as: the P and S plans' 3-D arguments (f0, nitem, iz set by the caller).
+int launch_ewem3d(int which, const March3DArgs& ap, const March3DArgs& as, const Ewem3DArgs& e, int F, int phase,
+                  hipStream_t st);
 int march3d_len(int n);
 bool march3d_supported(int nkx, int nky);
 // which: 0 init, 1 x-pass, 2 migration y-pass, 3 modelling y-pass (phase 0 source,

@@ -28,6 +28,7 @@
 //   pspiop             wem.c:380-471  FFT, per vref (phase, IFFT), per-x selection with the
 //                                     real-part cut of linear_interp (wem.c:473-478)
 // A step of -dz uses the complex conjugate of the +dz phases (the evanescent factor is real).
+#include "swem_esep.hpp"
 #include "swem_fft.hpp"
 #include "swem_internal.hpp"
 
@@ -1368,33 +1369,6 @@ int launch_angles(int which, const March2DArgs& ax, const March2DArgs& az, const
 // recombined; the P and S images correlate the potentials with the source.  The separation
 // matrix of wavenumber kx (kx = dkx ikx for ikx < nkx/2, else -(dkx nkx - dkx ikx)) uses the
 // depth's layer velocities vp0 = 1/po_p, vs0 = 1/po_s and is computed in place.
-__device__ __forceinline__ void esep_k(float kx, float w, float vp, float vs, bool sep, bool adj, float2 a1,
-                                       float2 a2, float2& o1, float2& o2)
-{
-    const float sp = w * w / (vp * vp) - kx * kx, ss = w * w / (vs * vs) - kx * kx;
-    float kzp = sp > 0.f ? sqrtf(sp) : 0.f, kzs = ss > 0.f ? sqrtf(ss) : 0.f;
-    const float np = sqrtf(kx * kx + kzp * kzp), ns = sqrtf(kx * kx + kzs * kzs);
-    const float kxp = kx / np, kxs = kx / ns;
-    kzp = kzp / np;
-    kzs = kzs / ns;
-    const float norm = kxp * kxs + kzp * kzs;
-    if (norm >= 0.1f && w >= 20.f) {
-        float m11, m12, m21, m22;
-        if (sep) {
-            if (!adj) m11 = kxs / norm, m12 = kzs / norm, m21 = -kzp / norm, m22 = kxp / norm;
-            else m11 = kxp, m12 = kzp, m21 = -kzs, m22 = kxs;
-        } else {
-            if (!adj) m11 = kxp, m12 = -kzs, m21 = kzp, m22 = kxs;
-            else m11 = kxs / norm, m12 = -kzp / norm, m21 = kzs / norm, m22 = kxp / norm;
-        }
-        o1 = cadd(cscale(a1, m11), cscale(a2, m12));
-        o2 = cadd(cscale(a1, m21), cscale(a2, m22));
-    } else {
-        o1 = a2;
-        o2 = a1;
-    }
-}
-
 // (i1, i2) -> (o1, o2) through the wavenumber domain; o1, o2 are zero at x >= nmx.
 template <int N, int P, class X>
 __device__ __forceinline__ void esep(const float2 (&i1)[P], const float2 (&i2)[P], float2 (&o1)[P], float2 (&o2)[P],

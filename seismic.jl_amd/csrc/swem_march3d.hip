@@ -21,6 +21,7 @@
 // Transforms are wave-level (one 64-lane wave per row / column, N/64 points per lane, no
 // workgroup barrier inside a transform).  An axis whose length n is not a power of two 128..1024
 // is transformed through Bluestein's identity on an array of N >= 2n - 1 points (line_fft).
+#include "swem_esep.hpp"
 #include "swem_fft.hpp"
 #include "swem_internal.hpp"
 
@@ -956,6 +957,360 @@ int march3d_len(int n)
 
 bool march3d_supported(int nkx, int nky) { return march3d_len(nkx) > 0 && march3d_len(nky) > 0; }
 
+// ------------------------------------------------------------------------ elastic WEM (3-D)
+// ewem (ewem.c:3-366) with nmy > 1 in the two passes of the 3-D march.  A depth step of the
+// receiver pair is: separation (x-pass: FFT_x, the 2 x 2 map of kx per point, IFFT_x), truncation
+// to the nmx x nmy grid (y-pass), the split-step steps of the potentials with vp and vs (x-pass kz
+// phase; y-pass split-step phase and taper), the imaging against the source, the combination
+// (x-pass) and truncation (y-pass).  Every x-pass applies one 1/(nkx nky): the state it writes, y
+// transformed, returns the step's result in space under the inverse y-FFT.
+
+__device__ __forceinline__ float2* eslot(const March3DArgs& a, const Ewem3DArgs& e, int item, int s)
+{
+    return e.W + ((size_t)item * 3 + s) * (size_t)a.nmx * a.nky;
+}
+
+// Initial state: the source delta at igx (slot 0) and the receiver spectra (slots 1 and 2, zero
+// when modelling), forward y-FFT.  One wave per row; grid: nitem * 3 * nmx rows.
+template <int NY, bool BY>
+__global__ __launch_bounds__(256) void ewem3d_init(March3DArgs a, Ewem3DArgs e)
+{
+    constexpr int P = NY / 64;
+    constexpr int L = LdsLen<NY>::value;
+    __shared__ float2 lds[4 * L];
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const size_t row = (size_t)blockIdx.x * 4 + w;
+    if (row >= (size_t)a.nitem * 3 * a.nmx) return;  // whole waves only
+    const int x = (int)(row % a.nmx);
+    const int s = (int)((row / a.nmx) % 3);
+    const int item = (int)(row / ((size_t)3 * a.nmx));
+    const int fl = a.f0 + item;
+    float2 v[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int y = t + 64 * q;
+        float2 u = make_float2(0.f, 0.f);
+        if (y < a.nmy) {
+            const size_t ixy = (size_t)x * a.nmy + y;
+            if (s == 0) u = (int)ixy == e.igx ? a.spec_s[fl] : u;
+            else if (e.sep_adj) u = (s == 1 ? e.spx : e.spz)[ixy * a.nf + fl];
+        }
+        v[q] = u;
+    }
+    yfft<NY, -1, BY>(v, lds + w * L, a, t);
+    float2* dst = eslot(a, e, item, s) + (size_t)x * a.nky;
+#pragma unroll
+    for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) dst[t + 64 * q] = v[q];
+}
+
+// x-pass.  NWF = 1 (e.mode 3): the source (slot 0), kz phase of vp (conj_src).  NWF = 2: the
+// receiver pair (slots 1, 2): e.mode 0 separation, 1 combination (the 2 x 2 map of kx,
+// ewem.c:593-630, 680-710), 2 kz phases of vp and vs (conj_rec).  Columns staged through LDS in
+// tiles of C.
+template <int NX, int NWF, int C, bool BX>
+__global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 1024 ? 2 : 4))) void ewem3d_cols(
+    March3DArgs ap, March3DArgs as, Ewem3DArgs e)
+{
+    constexpr int P = NX / 64;
+    constexpr int L = LdsLen<NX>::value;
+    extern __shared__ float2 smem[];
+    const March3DArgs& a = ap;
+    const int tiles = (a.nky + C - 1) / C;
+    const int nblk = a.nitem * tiles;
+    const int full = nblk & ~7;
+    int b = blockIdx.x;
+    if (b < full) b = (b & 7) * (full >> 3) + (b >> 3);
+    const int item = b / tiles;
+    const int ky0 = (b - item * tiles) * C;
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    auto slot_of = [&](int k) { return NWF == 1 ? 0 : 1 + k; };
+    const int c = threadIdx.x % C, x0 = threadIdx.x / C;
+    const bool kin = ky0 + c < a.nky;
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) {
+        const float2* src = eslot(a, e, item, slot_of(k)) + ky0 + c;
+        float2* dst = smem + (c * NWF + k) * L;
+        float2 st[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int x = x0 + 64 * j;
+            st[j] = x < a.nmx && kin ? src[(size_t)x * a.nky] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int x = x0 + 64 * j;
+            if (x < a.nmx) dst[lds_pad(x)] = st[j];
+        }
+    }
+    __syncthreads();
+    float2 v[NWF][P];
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) {
+        float2* buf = smem + (w * NWF + k) * L;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int x = t + 64 * q;
+            v[k][q] = x < a.nmx ? buf[lds_pad(x)] : make_float2(0.f, 0.f);
+        }
+        wave_sync_lds();
+        xfft<NX, -1, BX>(v[k], buf, a, t);
+    }
+    const int iw = a.ifmin + a.f0 + item;
+    const int iky = ky0 + w;
+    if (NWF == 2 && e.mode != 2) {
+        const float wf = (float)iw * e.dwf;
+        const float vp = e.vp0[a.iz], vs = e.vs0[a.iz];
+        const int n = a.nkx;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int k = t + 64 * q;
+            const float kx = (float)k < (float)n / 2.f ? e.dkxf * (float)k : -(e.dkxf * (float)n - e.dkxf * (float)k);
+            float2 r1, r2;
+            esep_k(kx, wf, vp, vs, e.mode == 0, e.sep_adj != 0, v[0][q], v[NWF - 1][q], r1, r2);
+            v[0][q] = k < n ? cscale(r1, a.inv_nk) : make_float2(0.f, 0.f);
+            v[NWF - 1][q] = k < n ? cscale(r2, a.inv_nk) : make_float2(0.f, 0.f);
+        }
+    } else {
+        const double wd = (double)iw * a.dw;
+        const int kyi = iky <= a.nky / 2 ? iky : a.nky - iky;
+        const double ky = kyi * a.dky;
+#pragma unroll
+        for (int k = 0; k < NWF; ++k) {
+            const March3DArgs& pl = (NWF == 2 && k == 1) ? as : ap;  // us2 steps with vs
+            const double p2 = pl.p2[a.iz];
+            const double c0 = wd * wd * p2 - ky * ky;
+            const bool cj = NWF == 1 ? e.conj_src : e.conj_rec;
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const float2 ph = a.prec == SWEM_PREC_KZ64 ? kz_phase64(pl, c0, t + 64 * q)
+                                                           : kz_phase(pl, wd, p2, pl.p2f[a.iz], t + 64 * q, iky);
+                v[k][q] = cmul(v[k][q], make_float2(ph.x, cj ? -ph.y : ph.y));
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) {
+        float2* buf = smem + (w * NWF + k) * L;
+        wave_sync_lds();
+        xfft<NX, +1, BX>(v[k], buf, a, t);
+        wave_sync_lds();
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int x = t + 64 * q;
+            if (x < a.nmx) buf[lds_pad(x)] = v[k][q];
+        }
+    }
+    __syncthreads();
+    int c1 = c, x1 = x0;
+    asm volatile("" : "+v"(c1), "+v"(x1));
+#pragma unroll
+    for (int k = 0; k < NWF; ++k) {
+        float2* dst = eslot(a, e, item, slot_of(k)) + ky0 + c1;
+        const float2* src = smem + (c1 * NWF + k) * L;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int x = x1 + 64 * j;
+            if (x < a.nmx && ky0 + c1 < a.nky) dst[(size_t)x * a.nky] = src[lds_pad(x)];
+        }
+    }
+}
+
+// y-pass, F waves (frequencies g F + f) of one row x.
+//   phase 0 migration: the stepped source (inverse y-FFT, conj split-step phase of vp, taper) and
+//           potentials (split-step phases of vp and vs, taper); images factor Re(conj(s / maxs) up)
+//           and factor Re(conj(s / maxs) us2) summed over the F frequencies (zero where the source is
+//           not stepped: ewem.c's smig is zero there); forward y-FFTs of what steps next
+//   phase 1 truncation of the receiver pair to y < nmy (between a map and a step)
+//   phase 2 modelling source: stepped source stored (divided by max|wav|) in smig, zero where
+//           z < sz; forward y-FFT for the next step
+//   phase 3 modelling receiver: truncation, injection up += smig mpp, us2 += smig mps2, taper,
+//           conj split-step phases (the forward branch of ssop, -dz), forward y-FFT
+//   phase 4 modelling output: the receiver pair's spectra
+template <int NY, bool BY>
+__global__ __launch_bounds__(512) void ewem3d_rows(March3DArgs ap, March3DArgs as, Ewem3DArgs e, int F, int phase)
+{
+    constexpr int P = NY / 64;
+    constexpr int L = LdsLen<NY>::value;
+    extern __shared__ float2 smem[];
+    const March3DArgs& a = ap;
+    const int ngroups = (a.nitem + F - 1) / F;
+    const int g = blockIdx.x % ngroups, x = blockIdx.x / ngroups;
+    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    const int item = g * F + w;
+    const bool active = item < a.nitem;
+    const int fl = a.f0 + (active ? item : 0);
+    const double wd = (double)(a.ifmin + fl) * a.dw;
+    float2* buf = smem + w * L;
+    const float tx = a.taper_x[x];
+    auto load_row = [&](int s, float2 (&v)[P]) {
+        const float2* src = eslot(a, e, item, s) + (size_t)x * a.nky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) v[q] = yin<BY>(a, t + 64 * q) ? src[t + 64 * q] : make_float2(0.f, 0.f);
+    };
+    auto store_row = [&](int s, float2 (&v)[P]) {
+        wave_sync_lds();
+        yfft<NY, -1, BY>(v, buf, a, t);
+        float2* dst = eslot(a, e, item, s) + (size_t)x * a.nky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) if (yin<BY>(a, t + 64 * q)) dst[t + 64 * q] = v[q];
+    };
+    if (phase == 0) {
+        const bool do_s = active && e.ds, do_r = active && e.dr;
+        float2 vs[P], v1[P], v2[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) vs[q] = v1[q] = v2[q] = make_float2(0.f, 0.f);
+        if (do_s) load_row(0, vs);
+        if (do_r) {
+            load_row(1, v1);
+            load_row(2, v2);
+        }
+        if (do_s) {
+            yfft<NY, +1, BY>(vs, buf, a, t);
+            wave_sync_lds();
+        }
+        if (do_r) {
+            yfft<NY, +1, BY>(v1, buf, a, t);
+            wave_sync_lds();
+            yfft<NY, +1, BY>(v2, buf, a, t);
+        }
+        float cp[P], cs[P];
+        const float factor = a.ifmin + fl == 0 ? 1.f : 2.f;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int y = t + 64 * q;
+            cp[q] = cs[q] = 0.f;
+            if (y >= a.nmy) {
+                vs[q] = v1[q] = v2[q] = make_float2(0.f, 0.f);
+                continue;
+            }
+            const size_t ixy = (size_t)x * a.nmy + y;
+            const float tp = tx * a.taper_y[y];
+            if (do_s) vs[q] = cscale(cmulc(vs[q], ss_phase(ap, wd, ixy)), tp);
+            if (do_r) {
+                v1[q] = cscale(cmul(v1[q], ss_phase(ap, wd, ixy)), tp);
+                v2[q] = cscale(cmul(v2[q], ss_phase(as, wd, ixy)), tp);
+                if (do_s) {
+                    const float2 sm = cscale(vs[q], e.inv_maxs);
+                    cp[q] = factor * (sm.x * v1[q].x + sm.y * v1[q].y);
+                    cs[q] = factor * (sm.x * v2[q].x + sm.y * v2[q].y);
+                }
+            }
+        }
+        // the two images of the F frequencies, each summed in a fixed order
+        for (int im = 0; im < 2; ++im) {
+            wave_sync_lds();
+            float* red = reinterpret_cast<float*>(buf);
+#pragma unroll
+            for (int q = 0; q < P; ++q) red[t + 64 * q] = im == 0 ? cp[q] : cs[q];
+            __syncthreads();
+            float* part = (im == 0 ? e.ppart : e.spart) + (size_t)g * a.nm + (size_t)x * a.nmy;
+            for (int y = threadIdx.x; y < a.nmy; y += blockDim.x) {
+                float sum = 0.f;
+                for (int ff = 0; ff < F; ++ff) sum += reinterpret_cast<const float*>(smem + ff * L)[y];
+                part[y] = sum;
+            }
+            __syncthreads();
+        }
+        if (do_s && e.next_s) store_row(0, vs);
+        if (do_r) {
+            store_row(1, v1);
+            store_row(2, v2);
+        }
+        return;
+    }
+    if (!active) return;  // phases 1-4 have no workgroup barriers
+    if (phase == 2) {
+        float2 vs[P];
+        if (e.ds) {
+            load_row(0, vs);
+            yfft<NY, +1, BY>(vs, buf, a, t);
+        }
+        float2* sm = e.smig + ((size_t)item * a.nz + a.iz) * a.nm + (size_t)x * a.nmy;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int y = t + 64 * q;
+            if (y >= a.nmy) {
+                vs[q] = make_float2(0.f, 0.f);
+                continue;
+            }
+            if (e.ds) vs[q] = cscale(cmulc(vs[q], ss_phase(ap, wd, (size_t)x * a.nmy + y)), tx * a.taper_y[y]);
+            sm[y] = e.ds ? cscale(vs[q], e.inv_maxs) : make_float2(0.f, 0.f);
+        }
+        if (e.ds && e.next_s) store_row(0, vs);
+        return;
+    }
+    float2 v1[P], v2[P];
+    load_row(1, v1);
+    load_row(2, v2);
+    yfft<NY, +1, BY>(v1, buf, a, t);
+    wave_sync_lds();
+    yfft<NY, +1, BY>(v2, buf, a, t);
+    if (phase == 4) {
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int y = t + 64 * q;
+            if (y < a.nmy) {
+                const size_t o = ((size_t)x * a.nmy + y) * a.nf + fl;
+                e.dgx[o] = v1[q];
+                e.dgz[o] = v2[q];
+            }
+        }
+        return;
+    }
+    const float2* sm = e.smig + ((size_t)item * a.nz + a.iz) * a.nm + (size_t)x * a.nmy;
+    const size_t mo = (size_t)a.iz * a.nm + (size_t)x * a.nmy;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int y = t + 64 * q;
+        if (y >= a.nmy) {
+            v1[q] = v2[q] = make_float2(0.f, 0.f);
+            continue;
+        }
+        if (phase == 3) {
+            const size_t ixy = (size_t)x * a.nmy + y;
+            const float tp = tx * a.taper_y[y];
+            const float2 u1 = cadd(v1[q], cscale(sm[y], e.mppT[mo + y]));
+            const float2 u2 = cadd(v2[q], cscale(sm[y], e.mpsT[mo + y]));
+            v1[q] = cmulc(cscale(u1, tp), ss_phase(ap, wd, ixy));
+            v2[q] = cmulc(cscale(u2, tp), ss_phase(as, wd, ixy));
+        }
+    }
+    store_row(1, v1);
+    store_row(2, v2);
+}
+
+template <int N, bool BL>
+static int launch_ewem3d_n(int which, const March3DArgs& ap, const March3DArgs& as, const Ewem3DArgs& e, int F,
+                           int phase, hipStream_t st)
+{
+    constexpr int L = LdsLen<N>::value;
+    if (which == 0) {
+        const size_t rows = (size_t)ap.nitem * 3 * ap.nmx;
+        hipLaunchKernelGGL((ewem3d_init<N, BL>), dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, st, ap, e);
+        SWEM_HIP(hipGetLastError());
+        return SWEM_OK;
+    }
+    if (which == 1) {
+        const bool src = e.mode == 3;  // the source's kz x-pass
+        constexpr int C2 = N >= 1024 ? 4 : 8, C1 = 2 * C2;
+        const int C = src ? C1 : C2, nwf = src ? 1 : 2;
+        const size_t lds = (size_t)C * nwf * L * sizeof(float2);
+        const dim3 grid(ap.nitem * ((ap.nky + C - 1) / C)), block(64 * C);
+        auto k = src ? ewem3d_cols<N, 1, C1, BL> : ewem3d_cols<N, 2, C2, BL>;
+        if (lds > 48 * 1024) SWEM_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k, grid, block, lds, st, ap, as, e);
+        SWEM_HIP(hipGetLastError());
+        return SWEM_OK;
+    }
+    const int ngroups = (ap.nitem + F - 1) / F;
+    const size_t lds = (size_t)F * L * sizeof(float2);
+    if (lds > 48 * 1024)
+        SWEM_HIP(hipFuncSetAttribute((const void*)ewem3d_rows<N, BL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((ewem3d_rows<N, BL>), dim3(ngroups * ap.nmx), dim3(64 * F), lds, st, ap, as, e, F, phase);
+    SWEM_HIP(hipGetLastError());
+    return SWEM_OK;
+}
+
 // Columns (waves) per x-pass workgroup: 16 buffers of staging per workgroup (8 columns of two
 // wavefields or 16 of one) below nkx = 1024, half that at 1024.
 static int march3d_cols_tile(int nwf, int nkx) { return (nkx >= 1024 ? 8 : 16) / nwf; }
@@ -1023,6 +1378,25 @@ static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStrea
 
 // which: 0 init (y-FFT of the initial wavefields), 1 x-pass, 2 adjoint y-pass, 3 forward y-pass
 // (phase 0 source, 1 receiver, 2 output).  F: frequencies per y-pass workgroup.
+int launch_ewem3d(int which, const March3DArgs& ap, const March3DArgs& as, const Ewem3DArgs& e, int F, int phase,
+                  hipStream_t st)
+{
+    const bool xpass = which == 1;
+    const int n = xpass ? ap.nkx : ap.nky, m = march3d_len(n);
+    if (!march3d_supported(ap.nkx, ap.nky) || ap.nmx > ap.nkx || ap.nmy > ap.nky || ap.nitem < 1 || !e.W)
+        return set_error(SWEM_EINVAL, "3-D ewem: inconsistent launch (nkx=%d nky=%d)", ap.nkx, ap.nky);
+    if (which == 2 && (F < 1 || F > 8))  // the y-pass holds three rows per wave: 512 threads
+        return set_error(SWEM_EINVAL, "3-D ewem: %d frequencies per workgroup (1..8)", F);
+    const bool blue = m != n;
+    switch (m) {
+    case 128: return blue ? launch_ewem3d_n<128, true>(which, ap, as, e, F, phase, st) : launch_ewem3d_n<128, false>(which, ap, as, e, F, phase, st);
+    case 256: return blue ? launch_ewem3d_n<256, true>(which, ap, as, e, F, phase, st) : launch_ewem3d_n<256, false>(which, ap, as, e, F, phase, st);
+    case 512: return blue ? launch_ewem3d_n<512, true>(which, ap, as, e, F, phase, st) : launch_ewem3d_n<512, false>(which, ap, as, e, F, phase, st);
+    case 1024: return blue ? launch_ewem3d_n<1024, true>(which, ap, as, e, F, phase, st) : launch_ewem3d_n<1024, false>(which, ap, as, e, F, phase, st);
+    }
+    return set_error(SWEM_EUNSUPPORTED, "3-D ewem: n=%d", n);
+}
+
 int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_t st)
 {
     const bool xpass = which == 1 || which == 4 || which == 6;
