@@ -122,6 +122,43 @@ struct Dft<16, SIGN> {
     }
 };
 
+template <int SIGN>
+struct Dft<3, SIGN> {
+    // X1,2 = x0 - (x1 + x2) / 2 +- SIGN i (sqrt(3) / 2) (x1 - x2)
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        const float h = 0.86602540378443864676f;
+        const float2 a = cadd(v[1], v[2]), b = cscale(mul_si<SIGN>(csub(v[1], v[2])), h);
+        const float2 c = csub(v[0], cscale(a, 0.5f));
+        v[0] = cadd(v[0], a);
+        v[1] = cadd(c, b);
+        v[2] = csub(c, b);
+    }
+};
+
+template <int SIGN>
+struct Dft<5, SIGN> {
+    // with a1 = x1 + x4, a2 = x2 + x3, b1 = x1 - x4, b2 = x2 - x3 and w = exp(SIGN 2 pi i / 5):
+    // X1,4 = x0 + c1 a1 + c2 a2 +- SIGN i (s1 b1 + s2 b2), X2,3 = x0 + c2 a1 + c1 a2 +- SIGN i (s2 b1 - s1 b2)
+    static __device__ __forceinline__ void run(float2* v)
+    {
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+        const float2 a1 = cadd(v[1], v[4]), a2 = cadd(v[2], v[3]);
+        const float2 b1 = csub(v[1], v[4]), b2 = csub(v[2], v[3]);
+        const float2 x0 = v[0];
+        const float2 p1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+        const float2 p2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+        const float2 q1 = mul_si<SIGN>(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+        const float2 q2 = mul_si<SIGN>(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+        v[0] = cadd(x0, cadd(a1, a2));
+        v[1] = cadd(p1, q1);
+        v[4] = csub(p1, q1);
+        v[2] = cadd(p2, q2);
+        v[3] = csub(p2, q2);
+    }
+};
+
 // Bank-conflict padding: one complex inserted every 2^SH.  The LDS has 64 banks of 4 bytes (32
 // complex per row).  A Stockham pass writes at lane stride R (first pass) or in groups of NS
 // lanes, then reads back at unit lane stride; SH is chosen per pass so both the writes and the
@@ -311,6 +348,96 @@ __device__ __forceinline__ void wfft(float2 (&v)[N / 64], float2* buf, const flo
 {
     const float2 none[1] = {make_float2(0.f, 0.f)};
     wfft_pass<N, SIGN, 1>(v, buf, tw, t, none);
+}
+
+// ------------------------------------------------------------------ mixed-radix transforms
+// A length n = 2^a 3^b 5^c held in an array of M >= n points (M a power of two; T = M / P
+// threads, point j = t + q T in v[q]; entries j >= n are zero on input and on output).  The
+// transform runs in LDS: the points are stored in natural order; every pass of radix R (in the
+// plan's order) reads butterfly b < n / R at b + r n / R, multiplies input r by
+// exp(SIGN 2 pi i r k / (NS R)) with k = b mod NS, runs the R-point DFT and writes the
+// self-sorting Stockham position (b - k) R + k + r NS, NS being the product of the radices before
+// the pass; the points are then loaded back.  Each thread holds at most IT = ceil(P / R)
+// butterflies of a pass.  PP: two buffers alternated per pass (one barrier per pass), parity `pp`
+// carried as in fft; otherwise one buffer with a barrier between a pass's loads and its stores.
+// Twiddles: w = tw[k n / (NS R)] from the table tw[j] = exp(-2 pi i j / n), its powers w^r by
+// products.  LDS indices are unpadded: the loads and the later passes' stores run at unit lane
+// stride, the first pass's stores at stride R.
+
+template <int R, int SIGN, int IT, bool PP, class Sync>
+__device__ __forceinline__ void mixed_pass(const float2* src, float2* dst, const float2* __restrict__ tw, int n,
+                                           int ns, int tws, uint32_t magic, int t, int T, Sync sync)
+{
+    const int nb = n / R;
+    float2 x[IT][R];
+    int base[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int b = t + i * T;
+        if (i == 0 || b < nb) {
+            const int k = ns > 1 ? b - (int)__umulhi((uint32_t)b, magic) * ns : 0;
+            base[i] = (b - k) * R + k;
+            float2 w1 = make_float2(1.f, 0.f);
+            if (ns > 1 && b < nb) w1 = tw[k * tws];
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[i][r] = b < nb ? src[b + r * nb] : make_float2(0.f, 0.f);
+            if (ns > 1) {
+                const float2 w = SIGN > 0 ? cconj(w1) : w1;
+                float2 wr = w;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    x[i][r] = cmul(x[i][r], wr);
+                    if (r + 1 < R) wr = cmul(wr, w);
+                }
+            }
+            Dft<R, SIGN>::run(x[i]);
+        }
+    }
+    if constexpr (!PP) sync();
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int b = t + i * T;
+        if (b < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[base[i] + r * ns] = x[i][r];
+        }
+    }
+    sync();
+}
+
+template <int M, int P, int SIGN, bool PP, class Plan, class Sync>
+__device__ __forceinline__ void mixed_fft(float2 (&v)[P], float2* buf0, float2* buf1, const float2* __restrict__ tw,
+                                          const Plan& mr, int t, int& pp, Sync sync)
+{
+    constexpr int T = M / P;
+    const int n = mr.n;
+    float2* cur = (PP && pp) ? buf1 : buf0;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int j = t + q * T;
+        if (j < n) cur[j] = v[q];
+    }
+    sync();
+    for (int p = 0; p < mr.np; ++p) {
+        float2* nxt = PP ? (cur == buf0 ? buf1 : buf0) : cur;
+        const int ns = mr.ns[p], tws = mr.tws[p];
+        const uint32_t mg = mr.magic[p];
+        switch (mr.r[p]) {
+        case 8: mixed_pass<8, SIGN, (P + 7) / 8, PP>(cur, nxt, tw, n, ns, tws, mg, t, T, sync); break;
+        case 4: mixed_pass<4, SIGN, (P + 3) / 4, PP>(cur, nxt, tw, n, ns, tws, mg, t, T, sync); break;
+        case 2: mixed_pass<2, SIGN, (P + 1) / 2, PP>(cur, nxt, tw, n, ns, tws, mg, t, T, sync); break;
+        case 3: mixed_pass<3, SIGN, (P + 2) / 3, PP>(cur, nxt, tw, n, ns, tws, mg, t, T, sync); break;
+        default: mixed_pass<5, SIGN, (P + 4) / 5, PP>(cur, nxt, tw, n, ns, tws, mg, t, T, sync); break;
+        }
+        cur = nxt;
+    }
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int j = t + q * T;
+        v[q] = j < n ? cur[j] : make_float2(0.f, 0.f);
+    }
+    if constexpr (PP) pp = cur == buf0 ? 1 : 0;  // the next transform writes the other buffer
+    else sync();
 }
 
 }  // namespace swem

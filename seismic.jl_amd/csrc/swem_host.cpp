@@ -240,42 +240,47 @@ static void bluestein_tables(int n, int m, std::vector<float2>& chirp, std::vect
     }
 }
 
+// Tables of a transform of n points on an array of m: the twiddles exp(-2 pi i j / len) in FP64
+// (len = m for power-of-two and Bluestein arrays, n for a mixed-radix length), Bluestein's chirps,
+// and the mixed-radix plan (n = 0 in mr for the other kinds).
+static int length_tables(int n, int m, int kind, DevBuf<float2>& tw, DevBuf<float2>& chirp, DevBuf<float2>& bchirp,
+                         MixedRadix& mr)
+{
+    const int len = kind == kLenMixed ? n : m;
+    std::vector<float2> t(len);
+    for (int j = 0; j < len; j++) {
+        const double a = -2.0 * kPi * (double)j / (double)len;
+        t[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    SWEM_TRY(tw.upload(t.data(), t.size()));
+    mr = MixedRadix{};
+    if (kind == kLenMixed && !mixed_radix_plan(n, &mr))
+        return set_error(SWEM_EDEVICE, "no mixed-radix plan for n=%d", n);
+    if (kind == kLenBluestein) {
+        std::vector<float2> c, bc;
+        bluestein_tables(n, m, c, bc);
+        SWEM_TRY(chirp.upload(c.data(), c.size()));
+        SWEM_TRY(bchirp.upload(bc.data(), bc.size()));
+    }
+    return SWEM_OK;
+}
+
 static int build_tables(swem_plan* p, const float* wav)
 {
     const swem_geometry& g = p->g;
     const float* bsrc = nullptr;
     const int nmx = g.nmx, nz = g.nz, nm = p->nm, nf = p->nf, nt = g.nt, nkx = p->nkx, narr = p->narr;
-    // twiddles exp(-2 pi i m / narr) of the transform array length in FP64
-    std::vector<float2> tw(narr);
-    for (int m = 0; m < narr; m++) {
-        double a = -2.0 * kPi * (double)m / (double)narr;
-        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    SWEM_TRY(p->tw.upload(tw.data(), tw.size()));
-    if (narr != nkx) {
-        std::vector<float2> chirp, bchirp;
-        bluestein_tables(nkx, narr, chirp, bchirp);
-        SWEM_TRY(p->chirp.upload(chirp.data(), chirp.size()));
-        SWEM_TRY(p->bchirp.upload(bchirp.data(), bchirp.size()));
-    }
+    // x transform tables: twiddles in FP64 of the array length (power of two, Bluestein) or of
+    // nkx itself (mixed radix), and the chirps of Bluestein's identity
+    SWEM_TRY(length_tables(nkx, narr, p->three_d ? march3d_kind(nkx) : march2d_kind(nkx), p->tw, p->chirp, p->bchirp,
+                           p->mrx));
     // taper (wem.c:546-564)
     std::vector<float> tap(nmx);
     taper_f32(nmx, p->s.lmx, tap.data());
     SWEM_TRY(p->taper.upload(tap.data(), tap.size()));
     if (p->three_d) {
         const int nky = p->s.nky, my = p->narr_y;
-        std::vector<float2> twy(my);
-        for (int m = 0; m < my; m++) {
-            double a = -2.0 * kPi * (double)m / (double)my;
-            twy[m] = make_float2((float)std::cos(a), (float)std::sin(a));
-        }
-        SWEM_TRY(p->twy.upload(twy.data(), twy.size()));
-        if (my != nky) {
-            std::vector<float2> chirp, bchirp;
-            bluestein_tables(nky, my, chirp, bchirp);
-            SWEM_TRY(p->chirp_y.upload(chirp.data(), chirp.size()));
-            SWEM_TRY(p->bchirp_y.upload(bchirp.data(), bchirp.size()));
-        }
+        SWEM_TRY(length_tables(nky, my, march3d_kind(nky), p->twy, p->chirp_y, p->bchirp_y, p->mry));
         std::vector<float> tapy(g.nmy);
         taper_f32(g.nmy, p->s.lmy, tapy.data());
         SWEM_TRY(p->taper_y.upload(tapy.data(), tapy.size()));
@@ -536,6 +541,7 @@ static March2DArgs base_args(const swem_plan* p, int c, int F)
     a.inv_nk = 1.0f / (float)p->s.nkx;
     a.nkx = p->nkx;
     a.tw = p->tw.p;
+    a.mr = p->mrx;
     a.chirp = p->chirp.p;
     a.bchirp = p->bchirp.p;
     a.ktab = p->ktab.p;
@@ -644,6 +650,8 @@ static March3DArgs base3(const swem_plan* p)
     a.taper_x = p->taper.p;
     a.taper_y = p->taper_y.p;
     a.twx = p->tw.p;
+    a.mrx = p->mrx;
+    a.mry = p->mry;
     a.twy = p->twy.p;
     a.chx = p->chirp.p;
     a.bchx = p->bchirp.p;
@@ -1048,13 +1056,14 @@ int plan_setup(swem_plan* p, const swem_geometry* g, const float* vel, const swe
     if (!p->three_d && march2d_blue_len(p->nkx)) p->narr = march2d_blue_len(p->nkx);
     if (p->three_d) {
         if (!march3d_supported(p->s.nkx, p->s.nky))
-            return set_error(SWEM_EUNSUPPORTED, "3-D: nkx=%d, nky=%d: this build handles 1..512 and powers of two to 1024",
+            return set_error(SWEM_EUNSUPPORTED,
+                             "3-D: nkx=%d, nky=%d: this build handles 1..512, 2^a 3^b 5^c and powers of two to 1024",
                              p->s.nkx, p->s.nky);
         p->narr = march3d_len(p->s.nkx);
         p->narr_y = march3d_len(p->s.nky);
     } else if (!march2d_supported(p->nkx)) {
-        return set_error(SWEM_EUNSUPPORTED, "nkx = padx*nmx = %d: this build handles 1..2048 and powers of two to 4096",
-                         p->nkx);
+        return set_error(SWEM_EUNSUPPORTED,
+                         "nkx = padx*nmx = %d: this build handles 1..2048, 2^a 3^b 5^c and powers of two to 4096", p->nkx);
     }
     if (p->nf < 1) return set_error(SWEM_EINVAL, "empty frequency band [ifmin=%d, ifmax=%d)", p->s.ifmin, p->s.ifmax);
     if (g->pspi && (g->nref < 2 || g->nref > 31)) return set_error(SWEM_EINVAL, "PSPI needs 2 <= nref <= 31 (nref=%d)", g->nref);
@@ -1130,8 +1139,10 @@ size_t plan_upload_bytes(const swem_plan* p, int nshot)
     }
     else
         b += pad256(nz * nm * 8) + pad256(nz * nm * 4);
-    if (p->narr != p->nkx) b += pad256(p->nkx * 8) + pad256(p->narr * 16);
-    if (p->three_d && p->narr_y != p->s.nky) b += pad256(p->narr_y * 8) + pad256(p->s.nky * 8) + pad256(p->narr_y * 16);
+    const int kx = p->three_d ? march3d_kind(p->nkx) : march2d_kind(p->nkx);
+    if (kx == kLenBluestein) b += pad256(p->nkx * 8) + pad256(p->narr * 16);
+    if (p->three_d && march3d_kind(p->s.nky) == kLenBluestein)
+        b += pad256(p->narr_y * 8) + pad256(p->s.nky * 8) + pad256(p->narr_y * 16);
     return b + pad256((size_t)nshot * 4);
 }
 
@@ -1645,23 +1656,12 @@ static int angles_impl(swem_plan* p, float sx, float sy, float* angx, float* ang
     return SWEM_OK;
 }
 
-// z tables of the angle passes: twiddles of the array length, Bluestein tables for other nz.
+// z tables of the angle passes: the transform tables of nz on its 2-D array length.
 static int angle_z_tables(int nz, DevBuf<float2>& twz, DevBuf<float2>& chz, DevBuf<float2>& bchz, March2DArgs& az)
 {
     const int mz = march2d_blue_len(nz), nza = mz ? mz : nz;
-    std::vector<float2> tw(nza);
-    for (int m = 0; m < nza; m++) {
-        const double a = -2.0 * kPi * (double)m / (double)nza;
-        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    SWEM_TRY(twz.upload(tw.data(), tw.size()));
-    if (mz) {
-        std::vector<float2> chirp, bchirp;
-        bluestein_tables(nz, mz, chirp, bchirp);
-        SWEM_TRY(chz.upload(chirp.data(), chirp.size()));
-        SWEM_TRY(bchz.upload(bchirp.data(), bchirp.size()));
-    }
     az = March2DArgs{};
+    SWEM_TRY(length_tables(nz, nza, mz ? march2d_kind(nz) : kLenPow2, twz, chz, bchz, az.mr));
     az.nkx = nz;
     az.tw = twz.p;
     az.chirp = chz.p;
@@ -1698,10 +1698,12 @@ static int angles3d_impl(swem_plan* p, float sx, float sy, float* angx, float* a
     March2DArgs ax{}, ay{};
     ax.nkx = nmx;  // nkx = nmx, nky = nmy (padx = 1); Bluestein tables where the march has them
     ax.tw = p->tw.p;
+    ax.mr = p->mrx;
     ax.chirp = p->chirp.p;
     ax.bchirp = p->bchirp.p;
     ay.nkx = nmy;
     ay.tw = p->twy.p;
+    ay.mr = p->mry;
     ay.chirp = p->chirp_y.p;
     ay.bchirp = p->bchirp_y.p;
     AngleArgs b{};

@@ -36,6 +36,24 @@ int pspi_tables(const swem_geometry& g, const float* vel, float* vref, int* iref
 void taper_f32(int n, int l, float* t);
 void slowness_f32(const swem_geometry& g, const float* vel, float* po, float* pd);
 
+// Mixed-radix transform plan of a length n = 2^a 3^b 5^c <= 4096 (mixed_fft in swem_fft.hpp):
+// the radix of every pass in order (8 while 8 divides the rest, then 4, 2, 5, 3), the product of
+// the radices before it, its twiddle step n / (ns r) and the multiplier that divides by ns
+// (b / ns = umulhi(b, magic) for b ns < 2^32; 0 where ns = 1).
+constexpr int kMixedMaxPass = 12;
+struct MixedRadix {
+    int n, np;  // length; passes (np = 0 for n = 1: the identity)
+    uint16_t ns[kMixedMaxPass];
+    uint16_t tws[kMixedMaxPass];
+    uint32_t magic[kMixedMaxPass];
+    uint8_t r[kMixedMaxPass];
+};
+// The plan of n (mr may be null); false when n is not 2^a 3^b 5^c or exceeds 4096.
+bool mixed_radix_plan(int n, MixedRadix* mr);
+// How a transform of length n runs: a power-of-two kernel on n itself, Bluestein's identity on a
+// longer power-of-two array, or the mixed-radix transform on the next power of two.
+enum { kLenPow2 = 0, kLenBluestein = 1, kLenMixed = 2 };
+
 // Arguments of the FP64 phase-table builders (swem_phase.hip).
 struct PhaseTableArgs {
     int nfc, f0, ifmin, nz, nref, nkh, nmx, prec;
@@ -67,7 +85,9 @@ struct March2DArgs {
     float oz, dz, sz, gz;
     float inv_nk;           // 1 / nkx (identity steps)
     int nkx;                // transform length
-    const float2* tw;       // [nkx] exp(-2 pi i m / nkx); Bluestein: [M] of the array length M
+    const float2* tw;       // [nkx] exp(-2 pi i m / nkx) (power-of-two and mixed-radix lengths);
+                            // Bluestein: [M] of the array length M
+    MixedRadix mr;          // mixed-radix lengths: the plan of nkx
     const float2* chirp;    // Bluestein: [nkx] exp(-i pi j^2 / nkx)
     const float2* bchirp;   // Bluestein: [2][M] FFT_M of the wrapped conj chirp / M (SIGN -1, +1)
     const float2* ktab;     // [nfc][nz][nref][nkh] kz phase / nk, +dz
@@ -141,13 +161,17 @@ struct March2DShape {
     int per_freq;   // subgroups per frequency
     int max_block;  // largest workgroup
     int bufs;       // LDS buffers per subgroup
-    int blue;       // Bluestein array length (0: nkx is a supported power of two)
+    int blue;       // array length of a length that is not a power of two 128..4096 (0: nkx is one)
+    int mixed;      // that length runs the mixed-radix transform (else Bluestein's)
 };
 March2DShape march2d_shape(int nkx, int variant, int adj);
 bool march2d_supported(int nkx);
-// Array length of the Bluestein transform for nkx (0: a power-of-two kernel takes nkx, or
-// nkx is out of range).
+// Array length of a 2-D transform length that is not a power of two 128..4096: the next power
+// of two (at least 128) for mixed-radix lengths up to 4096, Bluestein's 2 nkx - 1 rounded up for
+// the others up to 2048 (0: a power-of-two kernel takes nkx, or nkx is out of range).
 int march2d_blue_len(int nkx);
+// kLenPow2, kLenBluestein or kLenMixed for a supported 2-D length.
+int march2d_kind(int nkx);
 size_t march2d_lds_bytes(int nkx, const March2DShape& sh, int F);
 
 // Arguments of the 3-D split-step kernels (swem_march3d.hip).  A launch covers the items
@@ -176,8 +200,10 @@ struct March3DArgs {
     const float* pdf;           // [nz][nm] float counterpart
     const float* taper_x;       // [nmx]
     const float* taper_y;       // [nmy]
-    const float2* twx;          // [march3d_len(nkx)] twiddles of the x array length
-    const float2* twy;          // [march3d_len(nky)]
+    const float2* twx;          // [march3d_len(nkx)] twiddles of the x array length; [nkx] of nkx
+                                // for a mixed-radix length
+    const float2* twy;          // [march3d_len(nky)] (mixed radix: [nky])
+    MixedRadix mrx, mry;        // mixed-radix plans of nkx, nky (kLenMixed axes)
     // Bluestein tables of an axis whose array length differs from its nk: [nk] chirp
     // exp(-i pi j^2 / nk), [2][array length] kernel spectra (transform sign -1, +1)
     const float2 *chx, *bchx, *chy, *bchy;
@@ -230,7 +256,11 @@ struct Ewem3DArgs {
 // output).  ap / as: the P and S plans' 3-D arguments (f0, nitem, iz set by the caller).
 int launch_ewem3d(int which, const March3DArgs& ap, const March3DArgs& as, const Ewem3DArgs& e, int F, int phase,
                   hipStream_t st);
+// Array length of a 3-D line of n points: n (powers of two 128..1024), the next power of two (at
+// least 128) for mixed-radix n up to 1024, Bluestein's 2 n - 1 rounded up for other n up to 512;
+// 0 when unsupported.
 int march3d_len(int n);
+int march3d_kind(int n);
 bool march3d_supported(int nkx, int nky);
 // which: 0 init, 1 x-pass, 2 migration y-pass, 3 modelling y-pass (phase 0 source,
 // 1 receiver, 2 output), 4 PSPI x-pass, 5 PSPI y-pass (phase 0 migration, 1 modelling source,

@@ -192,6 +192,22 @@ struct XBlue {
     }
 };
 
+// XMixed: a mixed-radix length n = nkx (2^a 3^b 5^c) in the power-of-two array of M >= n points,
+// transformed in the subgroup's two LDS buffers (mixed_fft); entries j >= n stay zero.
+template <int M, int P>
+struct XMixed {
+    template <int SIGN, bool WAVE>
+    __device__ __forceinline__ static void fft(float2 (&v)[P], float2* lds, const March2DArgs& a, int t, int& pp)
+    {
+        static_assert(!WAVE, "mixed-radix transforms use the subgroup buffers");
+        mixed_fft<M, P, SIGN, true>(v, lds, lds + LdsLen<M>::value, a.tw, a.mr, t, pp, []() { __syncthreads(); });
+    }
+    __device__ __forceinline__ static void kload_row(float2 (&L)[P], const float2* __restrict__ row, const March2DArgs& a, int t)
+    {
+        XBlue<M, P>::kload_row(L, row, a, t);
+    }
+};
+
 struct StepCtx {
     int t, iz, fc;  // thread within the subgroup, depth index, frequency index within the chunk
                     // (fc and iz are wave-uniform: table rows are addressed from scalar registers)
@@ -955,6 +971,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void m
     fwd_body<M, P, MODE, false, XBlue<M, P>>(a);
 }
 
+template <int M, int MODE, int P>
+__global__ __launch_bounds__(M / P) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_mixed(March2DArgs a)
+{
+    adj_body_pair<M, P, MODE, false, XMixed<M, P>>(a);
+}
+
+template <int M, int MODE, int P>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void march2d_fwd_mixed(March2DArgs a)
+{
+    fwd_body<M, P, MODE, false, XMixed<M, P>>(a);
+}
+
 // ------------------------------------------------------------------------ dispatch
 // Kernel shape of a variant: threads per wavefield subgroup T, subgroups per frequency
 // (2 = source and receiver in separate subgroups, 1 = one subgroup marches both), the largest
@@ -972,10 +1000,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void m
 //                transform is the identity): the PAIR body (migration) or the
 //                subgroup body (modelling) over the array length M = blue (P = 8 from M = 1024,
 //                one wave below); the variant option does not apply.
+int march2d_kind(int nkx)
+{
+    if (nkx >= 128 && !(nkx & (nkx - 1))) return kLenPow2;
+    return mixed_radix_plan(nkx, nullptr) ? kLenMixed : kLenBluestein;
+}
+
 int march2d_blue_len(int nkx)
 {
-    if (nkx < 1 || nkx > 2048 || (nkx >= 128 && !(nkx & (nkx - 1)))) return 0;
+    if (nkx < 1 || (nkx >= 128 && !(nkx & (nkx - 1)))) return 0;
     int m = 128;
+    if (march2d_kind(nkx) == kLenMixed) {
+        while (m < nkx) m <<= 1;
+        return m <= 4096 ? m : 0;
+    }
+    if (nkx > 2048) return 0;
     while (m < 2 * nkx - 1) m <<= 1;
     return m;
 }
@@ -985,6 +1024,7 @@ March2DShape march2d_shape(int nkx, int variant, int adj)
     March2DShape sh{};
     if (const int m = march2d_blue_len(nkx)) {
         sh.blue = m;
+        sh.mixed = march2d_kind(nkx) == kLenMixed;
         sh.variant = adj ? SWEM_VARIANT_PAIR : SWEM_VARIANT_SUBGROUP;
         sh.T = m >= 1024 ? m / 8 : 64;
         sh.per_freq = 1;
@@ -1089,6 +1129,9 @@ static int launch_blue_mode(const March2DArgs& a, const March2DShape& sh, int F,
     constexpr int P = M >= 1024 ? 8 : M / 64;
     const size_t lds = march2d_lds_bytes(a.nkx, sh, F);
     dim3 block(sh.T * sh.per_freq * F);
+    if (sh.mixed)
+        return adj ? launch_kernel(march2d_adj_mixed<M, MODE, P>, a, block, lds, st)
+                   : launch_kernel(march2d_fwd_mixed<M, MODE, P>, a, block, lds, st);
     return adj ? launch_kernel(march2d_adj_blue<M, MODE, P>, a, block, lds, st)
                : launch_kernel(march2d_fwd_blue<M, MODE, P>, a, block, lds, st);
 }
@@ -1114,7 +1157,8 @@ int launch_march2d(const March2DArgs& a, int nkx, int F, int mode, int adj, int 
 {
     const March2DShape sh = march2d_shape(nkx, variant, adj);
     if (sh.T <= 0)
-        return set_error(SWEM_EUNSUPPORTED, "2-D march: nkx=%d is neither a power of two 128..4096 nor 2..2048", nkx);
+        return set_error(SWEM_EUNSUPPORTED,
+                         "2-D march: nkx=%d is neither a power of two 128..4096, 2^a 3^b 5^c up to 4096 nor 1..2048", nkx);
     if (F < 1 || sh.T * sh.per_freq * F > sh.max_block)
         return set_error(SWEM_EINVAL, "2-D march: %d frequencies of %d subgroups of %d threads exceed %d", F,
                          sh.per_freq, sh.T, sh.max_block);
@@ -1124,7 +1168,8 @@ int launch_march2d(const March2DArgs& a, int nkx, int F, int mode, int adj, int 
     if (march2d_lds_bytes(nkx, sh, F) > 160 * 1024)
         return set_error(SWEM_EINVAL, "2-D march: LDS for %d frequencies at nkx=%d exceeds 160 KiB", F, nkx);
     if (sh.blue) {
-        if (a.nkx != nkx || !a.chirp || !a.bchirp) return set_error(SWEM_EINVAL, "2-D march: Bluestein tables missing");
+        if (a.nkx != nkx || (sh.mixed ? a.mr.n != nkx : (!a.chirp || !a.bchirp)))
+            return set_error(SWEM_EINVAL, "2-D march: %s tables missing", sh.mixed ? "mixed-radix" : "Bluestein");
         switch (sh.blue) {
         case 128: return launch_blue<128>(a, sh, F, mode, adj, st);
         case 256: return launch_blue<256>(a, sh, F, mode, adj, st);
@@ -1329,33 +1374,40 @@ static int launch_angles_n(int which, const March2DArgs& ax, const March2DArgs& 
 }
 
 template <int N>
-static int launch_angles_len(int which, int blue, const March2DArgs& ax, const March2DArgs& az, const AngleArgs& b,
+static int launch_angles_len(int which, int kind, const March2DArgs& ax, const March2DArgs& az, const AngleArgs& b,
                              int F, hipStream_t st)
 {
     constexpr int P = N >= 1024 ? 8 : N / 64;
-    return blue ? launch_angles_n<N, XBlue<N, P>>(which, ax, az, b, F, st)
-                : launch_angles_n<N, XPow2<N, P>>(which, ax, az, b, F, st);
+    if (kind == kLenMixed) return launch_angles_n<N, XMixed<N, P>>(which, ax, az, b, F, st);
+    return kind == kLenBluestein ? launch_angles_n<N, XBlue<N, P>>(which, ax, az, b, F, st)
+                                 : launch_angles_n<N, XPow2<N, P>>(which, ax, az, b, F, st);
 }
 
 int angles_supported(int n) { return n >= 2 && (march2d_blue_len(n) || (n >= 128 && n <= 4096 && !(n & (n - 1)))); }
+
+// The tables a launch needs for its transform length are present.
+static bool len_tables(const March2DArgs& a, int kind)
+{
+    return kind == kLenPow2 || (kind == kLenMixed ? a.mr.n == a.nkx : a.chirp && a.bchirp);
+}
 
 int launch_angles(int which, const March2DArgs& ax, const March2DArgs& az, const AngleArgs& b, int F, hipStream_t st)
 {
     const int n = which == 1 ? az.nkx : ax.nkx;
     const int m = march2d_blue_len(n);
-    const int len = m ? m : n;
-    if (!angles_supported(n) || (m && !(which == 1 ? az : ax).chirp))
+    const int len = m ? m : n, kind = m ? march2d_kind(n) : kLenPow2;
+    if (!angles_supported(n) || !len_tables(which == 1 ? az : ax, kind))
         return set_error(SWEM_EUNSUPPORTED, "angles: transform length %d", n);
     if (b.len < 1 || b.nz < 1 || b.nfc < 1 || (which > 0 && (b.inner < 1 || b.lines < 1)) ||
         (which == 2 && b.len != n) || (which == 1 && b.nz != n))
         return set_error(SWEM_EINVAL, "angles: inconsistent launch (len=%d nz=%d n=%d)", b.len, b.nz, n);
     switch (len) {
-    case 128: return launch_angles_len<128>(which, m != 0, ax, az, b, F, st);
-    case 256: return launch_angles_len<256>(which, m != 0, ax, az, b, F, st);
-    case 512: return launch_angles_len<512>(which, m != 0, ax, az, b, F, st);
-    case 1024: return launch_angles_len<1024>(which, m != 0, ax, az, b, F, st);
-    case 2048: return launch_angles_len<2048>(which, m != 0, ax, az, b, F, st);
-    case 4096: return launch_angles_len<4096>(which, m != 0, ax, az, b, F, st);
+    case 128: return launch_angles_len<128>(which, kind, ax, az, b, F, st);
+    case 256: return launch_angles_len<256>(which, kind, ax, az, b, F, st);
+    case 512: return launch_angles_len<512>(which, kind, ax, az, b, F, st);
+    case 1024: return launch_angles_len<1024>(which, kind, ax, az, b, F, st);
+    case 2048: return launch_angles_len<2048>(which, kind, ax, az, b, F, st);
+    case 4096: return launch_angles_len<4096>(which, kind, ax, az, b, F, st);
     }
     return set_error(SWEM_EUNSUPPORTED, "angles: transform length %d", n);
 }
@@ -1531,26 +1583,28 @@ static int launch_ewem_n(const March2DArgs& ap, const March2DArgs& as, const Ewe
 }
 
 template <int N>
-static int launch_ewem_len(int blue, const March2DArgs& ap, const March2DArgs& as, const EwemArgs& e, int adj,
+static int launch_ewem_len(int kind, const March2DArgs& ap, const March2DArgs& as, const EwemArgs& e, int adj,
                            hipStream_t st)
 {
     constexpr int P = N >= 1024 ? 8 : N / 64;
-    return blue ? launch_ewem_n<N, XBlue<N, P>>(ap, as, e, adj, st) : launch_ewem_n<N, XPow2<N, P>>(ap, as, e, adj, st);
+    if (kind == kLenMixed) return launch_ewem_n<N, XMixed<N, P>>(ap, as, e, adj, st);
+    return kind == kLenBluestein ? launch_ewem_n<N, XBlue<N, P>>(ap, as, e, adj, st)
+                                 : launch_ewem_n<N, XPow2<N, P>>(ap, as, e, adj, st);
 }
 
 int launch_ewem(const March2DArgs& ap, const March2DArgs& as, const EwemArgs& e, int adj, hipStream_t st)
 {
-    const int n = ap.nkx, m = march2d_blue_len(n), len = m ? m : n;
-    if (!angles_supported(n) || as.nkx != n || (m && (!ap.chirp || !as.chirp)) || ap.nshot < 1 ||
+    const int n = ap.nkx, m = march2d_blue_len(n), len = m ? m : n, kind = m ? march2d_kind(n) : kLenPow2;
+    if (!angles_supported(n) || as.nkx != n || !len_tables(ap, kind) || !len_tables(as, kind) || ap.nshot < 1 ||
         ap.ngroups != ap.nfc || ap.nmx > n)
         return set_error(SWEM_EINVAL, "ewem: inconsistent launch (nkx=%d nmx=%d)", n, ap.nmx);
     switch (len) {
-    case 128: return launch_ewem_len<128>(m != 0, ap, as, e, adj, st);
-    case 256: return launch_ewem_len<256>(m != 0, ap, as, e, adj, st);
-    case 512: return launch_ewem_len<512>(m != 0, ap, as, e, adj, st);
-    case 1024: return launch_ewem_len<1024>(m != 0, ap, as, e, adj, st);
-    case 2048: return launch_ewem_len<2048>(m != 0, ap, as, e, adj, st);
-    case 4096: return launch_ewem_len<4096>(m != 0, ap, as, e, adj, st);
+    case 128: return launch_ewem_len<128>(kind, ap, as, e, adj, st);
+    case 256: return launch_ewem_len<256>(kind, ap, as, e, adj, st);
+    case 512: return launch_ewem_len<512>(kind, ap, as, e, adj, st);
+    case 1024: return launch_ewem_len<1024>(kind, ap, as, e, adj, st);
+    case 2048: return launch_ewem_len<2048>(kind, ap, as, e, adj, st);
+    case 4096: return launch_ewem_len<4096>(kind, ap, as, e, adj, st);
     }
     return set_error(SWEM_EUNSUPPORTED, "ewem: transform length %d", n);
 }

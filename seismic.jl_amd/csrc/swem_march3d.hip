@@ -109,24 +109,29 @@ __device__ __forceinline__ float2* state(const March3DArgs& a, int item, int wf)
 }
 
 // Point i of a y line is stored (lines hold nky points; a Bluestein array is longer).
-template <bool BY>
+template <int BY>
 __device__ __forceinline__ bool yin(const March3DArgs& a, int i)
 {
-    return !BY || i < a.nky;
+    return BY == kLenPow2 || i < a.nky;
 }
 
 // Transform of one line of n points held by a wave as N array points (lane t, slot q: point
-// t + 64 q).  BLUE = false: n = N, the wave FFT.  BLUE = true: Bluestein's chirp-z identity on the
-// power-of-two array length N >= 2n - 1, X_k = c_k IFFT_N(FFT_N(c x) B)_k with the chirp
-// c_j = exp(SIGN i pi j^2 / n) and B the spectrum of the wrapped conjugate chirp over N, built
-// in FP64 on the host (c = 0 for j >= n, so points beyond n enter and leave as zero).
-template <int N, int SIGN, bool BLUE>
+// t + 64 q).  KIND kLenPow2: n = N, the wave FFT.  kLenMixed: n = 2^a 3^b 5^c <= N, the
+// mixed-radix transform in the wave's LDS buffer (tw: [n] of n).  kLenBluestein: Bluestein's
+// chirp-z identity on the power-of-two array length N >= 2n - 1, X_k = c_k IFFT_N(FFT_N(c x) B)_k
+// with the chirp c_j = exp(SIGN i pi j^2 / n) and B the spectrum of the wrapped conjugate chirp
+// over N, built in FP64 on the host (c = 0 for j >= n, so points beyond n enter and leave as zero).
+template <int N, int SIGN, int KIND>
 __device__ __forceinline__ void line_fft(float2 (&v)[N / 64], float2* buf, const float2* __restrict__ tw,
-                                         const float2* __restrict__ chirp, const float2* __restrict__ bch, int n, int t)
+                                         const float2* __restrict__ chirp, const float2* __restrict__ bch,
+                                         const MixedRadix& mr, int n, int t)
 {
     constexpr int P = N / 64;
-    if constexpr (!BLUE) {
+    if constexpr (KIND == kLenPow2) {
         wfft<N, SIGN>(v, buf, tw, t);
+    } else if constexpr (KIND == kLenMixed) {
+        int pp = 0;
+        mixed_fft<N, P, SIGN, false>(v, buf, buf, tw, mr, t, pp, []() { wave_sync_lds(); });
     } else {
         auto chirp_mul = [&]() {
             float2 c[P];
@@ -153,16 +158,16 @@ __device__ __forceinline__ void line_fft(float2 (&v)[N / 64], float2* buf, const
     }
 }
 
-template <int N, int SIGN, bool BX>
+template <int N, int SIGN, int BX>
 __device__ __forceinline__ void xfft(float2 (&v)[N / 64], float2* buf, const March3DArgs& a, int t)
 {
-    line_fft<N, SIGN, BX>(v, buf, a.twx, a.chx, a.bchx, a.nkx, t);
+    line_fft<N, SIGN, BX>(v, buf, a.twx, a.chx, a.bchx, a.mrx, a.nkx, t);
 }
 
-template <int N, int SIGN, bool BY>
+template <int N, int SIGN, int BY>
 __device__ __forceinline__ void yfft(float2 (&v)[N / 64], float2* buf, const March3DArgs& a, int t)
 {
-    line_fft<N, SIGN, BY>(v, buf, a.twy, a.chy, a.bchy, a.nky, t);
+    line_fft<N, SIGN, BY>(v, buf, a.twy, a.chy, a.bchy, a.mry, a.nky, t);
 }
 
 }  // namespace
@@ -170,7 +175,7 @@ __device__ __forceinline__ void yfft(float2 (&v)[N / 64], float2* buf, const Mar
 // Initial wavefields of the chunk (adjoint: source delta at the shot position and receiver
 // spectra; forward: source delta, receiver zero), forward y-FFT, stored as the step state.
 // Workgroup: WAVES rows (one wave each); grid: nitem * 2 * nmx rows.
-template <int NY, bool BY>
+template <int NY, int BY>
 __global__ __launch_bounds__(256) void march3d_init(March3DArgs a)
 {
     constexpr int P = NY / 64;
@@ -218,7 +223,7 @@ __device__ __forceinline__ float2 kz_phase64(const March3DArgs& a, double c0, in
 // of C complex.  conj_src / conj_rec: use conj(L) (-dz).  Workgroups are renumbered so that each
 // XCD (consecutive workgroup ids go round robin over the 8 XCDs) takes a contiguous run of tiles:
 // neighbouring tiles share 128-byte lines, which then meet in one L2.
-template <int NX, int NWF, int C, bool BX>
+template <int NX, int NWF, int C, int BX>
 __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 1024 ? 2 : 4))) void march3d_cols(March3DArgs a)
 {
     constexpr int P = NX / 64;
@@ -325,7 +330,7 @@ __device__ __forceinline__ float2* rslice(const March3DArgs& a, int item, int wf
 
 // x-pass: as march3d_cols up to the forward x-FFT, then per vref in use: kz phase of
 // vref[ir][iz] (conj for a -dz step), inverse x-FFT, slice tile written through LDS.
-template <int NX, int NWF, int C, bool BX>
+template <int NX, int NWF, int C, int BX>
 __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 512 ? 2 : 4))) void march3d_cols_pspi(
     March3DArgs a)
 {
@@ -427,7 +432,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 51
 // vref in use at the depth, the slice R[ir] (the weighted wavefield, y transformed) is x
 // transformed and multiplied by its kz phase (conj for the -dz step), the products are summed,
 // and one inverse x-FFT gives the stepped receiver (y transformed) in the state.
-template <int NX, int C, bool BX>
+template <int NX, int C, int BX>
 __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 512 ? 2 : 4))) void march3d_cols_gather(
     March3DArgs a)
 {
@@ -513,7 +518,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 51
 
 // u = the pspiop selection of row x of wavefield wf from the slices of depth iz_sel: inverse
 // y-FFT of each slice in use, per point the slice of its iref1, real part where cut.
-template <int NY, bool BY>
+template <int NY, int BY>
 __device__ __forceinline__ void pspi_select_row(const March3DArgs& a, int item, int wf, int x, float2 (&u)[NY / 64],
                                                 float2* buf, int t)
 {
@@ -584,7 +589,7 @@ __device__ __forceinline__ float pspi3_weight(uint8_t s1, uint8_t s2, float w2, 
 //   phase 2 modelling receiver: select the previous step's result (zero at the first), inject
 //           m smig of this depth, forward y-FFT
 //   phase 3 modelling output: select the last step's result (zero when no step ran), spectra out
-template <int NY, bool BY>
+template <int NY, int BY>
 __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, int phase)
 {
     constexpr int P = NY / 64;
@@ -755,7 +760,7 @@ __global__ __launch_bounds__(512) void march3d_rows_pspi(March3DArgs a, int F, i
 // of frequency item g F + f.  ds / dr: the source / receiver was stepped at this depth; next_s /
 // next_r: it is stepped at the next depth (its forward y-FFT is needed); the imaging runs when dr.
 // The split-step phase is computed once and applied conjugated to the source.
-template <int NY, bool BY>
+template <int NY, int BY>
 __global__ __launch_bounds__(BY ? 512 : 1024) void march3d_rows_adj(March3DArgs a, int F)
 {
     constexpr int P = NY / 64;
@@ -851,7 +856,7 @@ __global__ __launch_bounds__(BY ? 512 : 1024) void march3d_rows_adj(March3DArgs 
 //           receiver's step; at the first call the state is the source trace, later IFFTy of
 //           the previous step's result is stored in smig at depth iz_sel; with ds the next step
 //           starts (taper, conj(SS), FFTy).
-template <int NY, bool BY>
+template <int NY, int BY>
 __global__ __launch_bounds__(BY ? 512 : 1024) void march3d_rows_fwd(March3DArgs a, int F, int phase)
 {
     constexpr int P = NY / 64;
@@ -946,11 +951,23 @@ __global__ __launch_bounds__(BY ? 512 : 1024) void march3d_rows_fwd(March3DArgs 
 // Array length of a 3-D line transform of n points: n for powers of two 128..1024, else the
 // Bluestein length (the power of two >= max(128, 2n - 1), up to 1024: n <= 512; n = 1 is the
 // nkx of a single inline, nmx = 1); 0 if neither.
+int march3d_kind(int n)
+{
+    if (n >= 128 && !(n & (n - 1))) return kLenPow2;
+    return mixed_radix_plan(n, nullptr) ? kLenMixed : kLenBluestein;
+}
+
 int march3d_len(int n)
 {
-    if (n >= 128 && n <= 1024 && (n & (n - 1)) == 0) return n;
-    if (n < 1 || n > 512) return 0;
+    if (n < 1) return 0;
+    const int kind = march3d_kind(n);
+    if (kind == kLenPow2) return n <= 1024 ? n : 0;
     int m = 128;
+    if (kind == kLenMixed) {
+        while (m < n) m <<= 1;
+        return m <= 1024 ? m : 0;
+    }
+    if (n > 512) return 0;
     while (m < 2 * n - 1) m <<= 1;
     return m;
 }
@@ -972,7 +989,7 @@ __device__ __forceinline__ float2* eslot(const March3DArgs& a, const Ewem3DArgs&
 
 // Initial state: the source delta at igx (slot 0) and the receiver spectra (slots 1 and 2, zero
 // when modelling), forward y-FFT.  One wave per row; grid: nitem * 3 * nmx rows.
-template <int NY, bool BY>
+template <int NY, int BY>
 __global__ __launch_bounds__(256) void ewem3d_init(March3DArgs a, Ewem3DArgs e)
 {
     constexpr int P = NY / 64;
@@ -1007,7 +1024,7 @@ __global__ __launch_bounds__(256) void ewem3d_init(March3DArgs a, Ewem3DArgs e)
 // receiver pair (slots 1, 2): e.mode 0 separation, 1 combination (the 2 x 2 map of kx,
 // ewem.c:593-630, 680-710), 2 kz phases of vp and vs (conj_rec).  Columns staged through LDS in
 // tiles of C.
-template <int NX, int NWF, int C, bool BX>
+template <int NX, int NWF, int C, int BX>
 __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 1024 ? 2 : 4))) void ewem3d_cols(
     March3DArgs ap, March3DArgs as, Ewem3DArgs e)
 {
@@ -1126,7 +1143,7 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
 //   phase 3 modelling receiver: truncation, injection up += smig mpp, us2 += smig mps2, taper,
 //           conj split-step phases (the forward branch of ssop, -dz), forward y-FFT
 //   phase 4 modelling output: the receiver pair's spectra
-template <int NY, bool BY>
+template <int NY, int BY>
 __global__ __launch_bounds__(512) void ewem3d_rows(March3DArgs ap, March3DArgs as, Ewem3DArgs e, int F, int phase)
 {
     constexpr int P = NY / 64;
@@ -1279,7 +1296,7 @@ __global__ __launch_bounds__(512) void ewem3d_rows(March3DArgs ap, March3DArgs a
     store_row(2, v2);
 }
 
-template <int N, bool BL>
+template <int N, int BL>
 static int launch_ewem3d_n(int which, const March3DArgs& ap, const March3DArgs& as, const Ewem3DArgs& e, int F,
                            int phase, hipStream_t st)
 {
@@ -1325,7 +1342,7 @@ static int launch3(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, 
     return SWEM_OK;
 }
 
-template <int N, bool BL>
+template <int N, int BL>
 static int launch3_n(int which, const March3DArgs& a, int F, int phase, hipStream_t st)
 {
     constexpr int L = LdsLen<N>::value;
@@ -1387,12 +1404,17 @@ int launch_ewem3d(int which, const March3DArgs& ap, const March3DArgs& as, const
         return set_error(SWEM_EINVAL, "3-D ewem: inconsistent launch (nkx=%d nky=%d)", ap.nkx, ap.nky);
     if (which == 2 && (F < 1 || F > 8))  // the y-pass holds three rows per wave: 512 threads
         return set_error(SWEM_EINVAL, "3-D ewem: %d frequencies per workgroup (1..8)", F);
-    const bool blue = m != n;
-    switch (m) {
-    case 128: return blue ? launch_ewem3d_n<128, true>(which, ap, as, e, F, phase, st) : launch_ewem3d_n<128, false>(which, ap, as, e, F, phase, st);
-    case 256: return blue ? launch_ewem3d_n<256, true>(which, ap, as, e, F, phase, st) : launch_ewem3d_n<256, false>(which, ap, as, e, F, phase, st);
-    case 512: return blue ? launch_ewem3d_n<512, true>(which, ap, as, e, F, phase, st) : launch_ewem3d_n<512, false>(which, ap, as, e, F, phase, st);
-    case 1024: return blue ? launch_ewem3d_n<1024, true>(which, ap, as, e, F, phase, st) : launch_ewem3d_n<1024, false>(which, ap, as, e, F, phase, st);
+    const int kind = march3d_kind(n);
+    if ((kind == kLenBluestein && (xpass ? !(ap.chx && ap.bchx) : !(ap.chy && ap.bchy))) ||
+        (kind == kLenMixed && (xpass ? ap.mrx.n : ap.mry.n) != n))
+        return set_error(SWEM_EINVAL, "3-D ewem: transform tables missing (n=%d)", n);
+    switch (kind * 16 + m / 128) {
+#define SWEM_E3(M, K) \
+    case K * 16 + M / 128: return launch_ewem3d_n<M, K>(which, ap, as, e, F, phase, st);
+        SWEM_E3(128, 0) SWEM_E3(256, 0) SWEM_E3(512, 0) SWEM_E3(1024, 0)
+        SWEM_E3(128, 1) SWEM_E3(256, 1) SWEM_E3(512, 1) SWEM_E3(1024, 1)
+        SWEM_E3(128, 2) SWEM_E3(256, 2) SWEM_E3(512, 2) SWEM_E3(1024, 2)
+#undef SWEM_E3
     }
     return set_error(SWEM_EUNSUPPORTED, "3-D ewem: n=%d", n);
 }
@@ -1403,23 +1425,27 @@ int launch_march3d(int which, const March3DArgs& a, int F, int phase, hipStream_
     const int n = xpass ? a.nkx : a.nky, m = march3d_len(n);
     if (!march3d_supported(a.nkx, a.nky) || a.nmx > a.nkx || a.nmy > a.nky || a.nitem < 1)
         return set_error(SWEM_EINVAL, "3-D march: inconsistent launch (nkx=%d nky=%d)", a.nkx, a.nky);
-    if (m != n && (xpass ? !(a.chx && a.bchx) : !(a.chy && a.bchy)))
+    const int kind = march3d_kind(n);
+    if (kind == kLenBluestein && (xpass ? !(a.chx && a.bchx) : !(a.chy && a.bchy)))
         return set_error(SWEM_EINVAL, "3-D march: Bluestein tables missing (n=%d)", n);
+    if (kind == kLenMixed && (xpass ? a.mrx.n : a.mry.n) != n)
+        return set_error(SWEM_EINVAL, "3-D march: mixed-radix plan missing (n=%d)", n);
     // y-passes: up to 16 frequencies per workgroup (8 on a Bluestein y axis, whose kernels are
     // compiled for 512 threads to keep their longer transforms in registers; 8 for PSPI)
-    const int fmax = which == 5 ? 8 : (!xpass && m != n ? 8 : 16);
+    const int fmax = which == 5 ? 8 : (!xpass && kind != kLenPow2 ? 8 : 16);
     if (((which == 2 || which == 3 || which == 5) && F > fmax) || F < 1)
         return set_error(SWEM_EINVAL, "3-D march: %d frequencies per workgroup (at most %d here)", F, fmax);
     if ((which == 4 || which == 5 || which == 6) && (!a.R || !a.sel || !a.used || a.nref < 1 || a.nref > 31))
         return set_error(SWEM_EINVAL, "3-D PSPI march: tables missing");
     if ((a.linear || which == 6) && (!a.sel2 || !a.w2))
         return set_error(SWEM_EINVAL, "3-D PSPI march: linear-mode tables missing");
-    const bool blue = m != n;
-    switch (m) {
-    case 128: return blue ? launch3_n<128, true>(which, a, F, phase, st) : launch3_n<128, false>(which, a, F, phase, st);
-    case 256: return blue ? launch3_n<256, true>(which, a, F, phase, st) : launch3_n<256, false>(which, a, F, phase, st);
-    case 512: return blue ? launch3_n<512, true>(which, a, F, phase, st) : launch3_n<512, false>(which, a, F, phase, st);
-    case 1024: return blue ? launch3_n<1024, true>(which, a, F, phase, st) : launch3_n<1024, false>(which, a, F, phase, st);
+    switch (kind * 16 + m / 128) {
+#define SWEM_M3(M, K) \
+    case K * 16 + M / 128: return launch3_n<M, K>(which, a, F, phase, st);
+        SWEM_M3(128, 0) SWEM_M3(256, 0) SWEM_M3(512, 0) SWEM_M3(1024, 0)
+        SWEM_M3(128, 1) SWEM_M3(256, 1) SWEM_M3(512, 1) SWEM_M3(1024, 1)
+        SWEM_M3(128, 2) SWEM_M3(256, 2) SWEM_M3(512, 2) SWEM_M3(1024, 2)
+#undef SWEM_M3
     }
     return set_error(SWEM_EUNSUPPORTED, "3-D march: n=%d", n);
 }

@@ -113,6 +113,7 @@ struct swem_plan {
     swem::DevBuf<float2> tw, twy, spec_s, ktab, stab;
     swem::DevBuf<float2> chirp, bchirp;  // x Bluestein: [nkx] chirp, [2][narr] kernel spectra
     swem::DevBuf<float2> chirp_y, bchirp_y;  // 3-D y Bluestein: [nky] chirp, [2][narr_y]
+    swem::MixedRadix mrx{}, mry{};           // mixed-radix plans of nkx, nky (n = 0: not mixed radix)
     swem::DevBuf<float> taper, taper_y, p2f, pdf, w2, wavd;
     swem::Bases own_bases;
     const float* basis_adj = nullptr;  // [nt][2nf] (own_bases or a shared cache entry)

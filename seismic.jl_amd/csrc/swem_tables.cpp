@@ -167,4 +167,28 @@ void taper_f32(int n, int l, float* t)
     }
 }
 
+// Passes of the mixed-radix transform of n: radix 8 while 8 divides the rest, then 4, 2, 5, 3.
+bool mixed_radix_plan(int n, MixedRadix* mr)
+{
+    if (n < 1 || n > 4096) return false;
+    MixedRadix m{};
+    m.n = n;
+    int rest = n, ns = 1;
+    static const int radices[] = {8, 4, 2, 5, 3};
+    for (int r : radices)
+        while (rest % r == 0) {
+            if (m.np == kMixedMaxPass) return false;
+            m.r[m.np] = (uint8_t)r;
+            m.ns[m.np] = (uint16_t)ns;
+            m.tws[m.np] = (uint16_t)(n / (ns * r));
+            m.magic[m.np] = ns > 1 ? (uint32_t)((1ull << 32) / (unsigned)ns + 1) : 0u;
+            ns *= r;
+            rest /= r;
+            m.np++;
+        }
+    if (rest != 1) return false;
+    if (mr) *mr = m;
+    return true;
+}
+
 }  // namespace swem

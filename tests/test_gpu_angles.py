@@ -28,8 +28,9 @@ def case(nx, nz, nt=512, blobs=False, seed=0):
     return g, vel, wav, rng
 
 
-# (nmx, nz, velocity with blobs): powers of two and Bluestein lengths on both axes
-SIZES = [(128, 100, False), (128, 128, True), (150, 90, True), (256, 200, False)]
+# (nmx, nz, velocity with blobs): powers of two, mixed-radix (100, 150, 90, 200) and Bluestein
+# (131, 97) lengths on both axes
+SIZES = [(128, 100, False), (128, 128, True), (150, 90, True), (256, 200, False), (131, 97, True)]
 
 
 @pytest.mark.parametrize("nx,nz,blobs", SIZES)
@@ -109,8 +110,8 @@ def test_compute_angles_3d_matches_oracle(gpu, nx, ny, nz):
 
 
 def test_compute_angles_3d_refused_off_grid(gpu):
-    """nmx = 600 is neither a power of two nor <= 512: SWEM_EUNSUPPORTED."""
-    g = seiswem.WemGeometry(nt=64, dt=0.004, nmx=600, dmx=10.0, nmy=16, dmy=10.0, nz=8, dz=10.0, pspi=False)
+    """nmx = 614 = 2 * 307 is neither a power of two, 2^a 3^b 5^c nor <= 512: SWEM_EUNSUPPORTED."""
+    g = seiswem.WemGeometry(nt=64, dt=0.004, nmx=614, dmx=10.0, nmy=16, dmy=10.0, nz=8, dz=10.0, pspi=False)
     vel = np.full((g.nm, g.nz), 2000.0, np.float32)
     with pytest.raises(seiswem.SwemError) as e:
         seiswem.compute_angles(vel, synth.wavelet_trace(64, 0.004, 20.0), g, 80.0, 80.0)

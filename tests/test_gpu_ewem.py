@@ -33,7 +33,7 @@ def run(g, vp, vs, wav, sx, adj, a, b, c):
                         g.fmax, g.padt, g.padx, adj)
 
 
-@pytest.mark.parametrize("nx", [64, 50, 150])  # nkx 128 (power of two), 100 and 300 (Bluestein)
+@pytest.mark.parametrize("nx", [64, 50, 150, 37])  # nkx 128 (power of two), 100 and 300 (mixed radix), 74 (Bluestein)
 def test_ewem_migration_matches_oracle(gpu, nx):
     g, p, vp, vs, wav, rng = case(nx)
     sx = g.dmx * (nx // 3)
@@ -46,7 +46,7 @@ def test_ewem_migration_matches_oracle(gpu, nx):
     assert e1 <= TOL and e2 <= TOL and np.all(mps1 == 0)
 
 
-@pytest.mark.parametrize("nx", [64, 50])
+@pytest.mark.parametrize("nx", [64, 50, 37])
 def test_ewem_modelling_matches_oracle(gpu, nx):
     g, p, vp, vs, wav, rng = case(nx, seed=3)
     sx = g.dmx * (nx // 2)
@@ -145,12 +145,14 @@ def run3(g, vp, vs, wav, sx, sy, adj, a, b, c):
                         g.fmax, g.padt, g.padx, adj)
 
 
-# (nx, ny): nkx x nky 128 x 128 (power of two), 60 x 40 (Bluestein on both axes); and receivers
-# and source below the surface (depths above gz and sz not stepped)
-CASES3 = [(64, 64, 0.0, 0.0), (30, 20, 0.0, 0.0), (32, 24, 25.0, 15.0)]
+# (nx, ny): nkx x nky 128 x 128 (power of two), 60 x 40 (mixed radix on both axes), 74 x 42
+# (Bluestein on both axes); and receivers and source below the surface (depths above gz and sz
+# not stepped)
+CASES3 = [(64, 64, 0.0, 0.0), (30, 20, 0.0, 0.0), (37, 21, 0.0, 0.0), (32, 24, 25.0, 15.0)]
+IDS3 = ["pow2", "mixed", "bluestein", "buried"]
 
 
-@pytest.mark.parametrize("nx,ny,gz,sz", CASES3, ids=["pow2", "bluestein", "buried"])
+@pytest.mark.parametrize("nx,ny,gz,sz", CASES3, ids=IDS3)
 def test_ewem_3d_migration_matches_oracle(gpu, nx, ny, gz, sz):
     g, p, vp, vs, wav, rng = case3d(nx, ny, gz=gz, sz=sz)
     sx, sy = g.dmx * (nx // 3), g.dmy * (ny // 2)
@@ -164,7 +166,7 @@ def test_ewem_3d_migration_matches_oracle(gpu, nx, ny, gz, sz):
         assert np.all(mpp.reshape(g.nm, g.nz)[:, : int(gz // g.dz) + (gz % g.dz > 0)] == 0)
 
 
-@pytest.mark.parametrize("nx,ny,gz,sz", CASES3, ids=["pow2", "bluestein", "buried"])
+@pytest.mark.parametrize("nx,ny,gz,sz", CASES3, ids=IDS3)
 def test_ewem_3d_modelling_matches_oracle(gpu, nx, ny, gz, sz):
     g, p, vp, vs, wav, rng = case3d(nx, ny, seed=4, gz=gz, sz=sz)
     sx, sy = g.dmx * (nx // 2), g.dmy * (ny // 3)

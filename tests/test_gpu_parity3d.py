@@ -196,13 +196,18 @@ def test_pspi3d_linear_dot_product(gpu):
     assert ec > 1e-3
 
 
-# 3-D grids whose padded lengths (padx 2) are not powers of two 128..1024: the x-pass and the
-# y-pass take those axes through Bluestein's identity on a power-of-two array length M >= 2n - 1
+# 3-D grids whose padded lengths (padx 2) are not powers of two 128..1024: an axis of length
+# 2^a 3^b 5^c takes the mixed-radix transform on the next power of two (up to 1024), another
+# length Bluestein's identity on a power-of-two array length M >= 2n - 1 (up to n = 512)
 BLUE3 = [
-    (50, 40, 12),   # nkx 100 (M 256), nky 80 (M 256)
-    (64, 37, 12),   # nkx 128 (a power of two), nky 74 (M 256)
-    (150, 32, 8),   # nkx 300 (M 1024, taper on x), nky 64 (a power of two below 128: M 128)
-    (30, 130, 8),   # nkx 60 (M 128), nky 260 (M 1024, taper on y)
+    pytest.param(37, 41, 12, id="blue74x82"),        # nkx 74 (M 256), nky 82 (M 256)
+    pytest.param(64, 37, 12, id="pow2x_blue74"),     # nkx 128 (a power of two), nky 74 (M 256)
+    pytest.param(151, 32, 8, id="blue302x_mixed64"), # nkx 302 (M 1024, taper on x), nky 64 (mixed, M 128)
+    pytest.param(30, 131, 8, id="mixed60x_blue262"), # nkx 60 (mixed, M 128), nky 262 (M 1024, taper on y)
+    pytest.param(50, 40, 12, id="mixed100x80"),      # nkx 100, nky 80 (mixed, M 128)
+    pytest.param(150, 32, 8, id="mixed300x64"),      # nkx 300 (M 512), nky 64
+    pytest.param(300, 20, 6, id="mixed600x40"),      # nkx 600 (M 1024): beyond Bluestein's 512
+    pytest.param(30, 125, 8, id="mixed60x250"),      # nkx 60, nky 250 (M 256)
 ]
 
 
@@ -214,7 +219,7 @@ def test_migrate3d_bluestein(gpu, nx, ny, nz):
     m_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), d=d, adj=True)
     m_gpu = seiswem.ShotProfileWEM(d[None], True, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)
     err = rel_l2(m_gpu, m_ref)
-    print(f"3-D Bluestein migrate {nx}x{ny}x{nz}: rel-L2 {err:.3e}", flush=True)
+    print(f"3-D non-power-of-two migrate {nx}x{ny}x{nz}: rel-L2 {err:.3e}", flush=True)
     assert err <= TOL
 
 
@@ -224,20 +229,22 @@ def test_model3d_bluestein(gpu, nx, ny, nz):
     d_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), m=refl, adj=False)
     d_gpu = seiswem.ShotProfileWEM(refl, False, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)[0]
     err = rel_l2(d_gpu, d_ref)
-    print(f"3-D Bluestein model {nx}x{ny}x{nz}: rel-L2 {err:.3e}", flush=True)
+    print(f"3-D non-power-of-two model {nx}x{ny}x{nz}: rel-L2 {err:.3e}", flush=True)
     assert err <= TOL
 
 
-def test_dot_product_3d_bluestein(gpu):
-    g, vel, refl, wav, sx, sy = small3d(50, 40, 12)
+@pytest.mark.parametrize("nx,ny", [(37, 41), (50, 40)], ids=["blue", "mixed"])
+def test_dot_product_3d_bluestein(gpu, nx, ny):
+    g, vel, refl, wav, sx, sy = small3d(nx, ny, 12)
     a, b, e = dot_test(g, vel, wav, sx, sy, 9)
-    print(f"3-D Bluestein dot: {a:.9e} {b:.9e} rel {e:.3e}", flush=True)
+    print(f"3-D {nx}x{ny} dot: {a:.9e} {b:.9e} rel {e:.3e}", flush=True)
     assert e <= DOT_TOL
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(50, 40, 10), (150, 32, 8)])
+@pytest.mark.parametrize("nx,ny,nz", [(37, 41, 10), (50, 40, 10), (150, 32, 8)])
 def test_pspi3d_bluestein(gpu, nx, ny, nz):
-    """3-D PSPI (compat) on Bluestein lengths, migration and modelling against the FP64 oracle."""
+    """3-D PSPI (compat) on Bluestein (74 x 82) and mixed-radix lengths, migration and modelling
+    against the FP64 oracle."""
     g, vel, refl, wav, sx, sy = pspi3d(nx, ny, nz)
     rng = np.random.default_rng(nx + ny)
     d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
@@ -246,13 +253,13 @@ def test_pspi3d_bluestein(gpu, nx, ny, nz):
     d_ref = O.wem(oparams(g), vel, wav, float(sx[0]), float(sy[0]), m=refl, adj=False)
     d_gpu = seiswem.ShotProfileWEM(refl, False, vel=vel, wav=wav, geom=g, sx=sx, sy=sy)[0]
     e1, e2 = rel_l2(m_gpu, m_ref), rel_l2(d_gpu, d_ref)
-    print(f"3-D PSPI Bluestein {nx}x{ny}x{nz}: migrate {e1:.3e} model {e2:.3e}", flush=True)
+    print(f"3-D PSPI {nx}x{ny}x{nz}: migrate {e1:.3e} model {e2:.3e}", flush=True)
     assert e1 <= TOL and e2 <= TOL
 
 
 def test_3d_lengths_beyond_bluestein_are_refused(gpu):
-    """nky = 2 * 300 = 600 is neither a power of two nor <= 512: SWEM_EUNSUPPORTED, named."""
-    g, vel, refl, wav, sx, sy = small3d(64, 300, 4)
+    """nky = 2 * 307 = 614 is neither a power of two, 2^a 3^b 5^c nor <= 512: SWEM_EUNSUPPORTED."""
+    g, vel, refl, wav, sx, sy = small3d(64, 307, 4)
     with pytest.raises(seiswem.SwemError) as e:
         seiswem.Plan(g, vel, wav)
     assert e.value.code == seiswem.SWEM_EUNSUPPORTED
@@ -260,8 +267,8 @@ def test_3d_lengths_beyond_bluestein_are_refused(gpu):
 
 @pytest.mark.parametrize("ny,pspi", [(40, False), (128, False), (40, True)])
 def test_single_inline_3d(gpu, ny, pspi):
-    """nmx = 1 with nmy > 1: nkx = 1 (wem.c:80), the x-pass transforms one point through the
-    Bluestein array; migration and modelling against the FP64 oracle."""
+    """nmx = 1 with nmy > 1: nkx = 1 (wem.c:80), the x-pass transforms one point (the mixed-radix
+    transform with no pass); migration and modelling against the FP64 oracle."""
     g, vel, refl, wav, sx, sy = pspi3d(1, ny, 8) if pspi else small3d(1, ny, 8)
     rng = np.random.default_rng(ny)
     d = rng.standard_normal((g.nm, g.nt)).astype(np.float32)
