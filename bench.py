@@ -255,6 +255,71 @@ def measure_serving(args):
     }
 
 
+def measure_survey(args):
+    """Serving the shots of one survey: the 2-D PSPI workload's 64 shots (nx 1024, nz 500, nt 2000,
+    0-80 Hz, one velocity and wavelet) submitted as 64 single-shot requests, Poisson arrivals at
+    --survey-rate (default 100/s) and all at once, on a server that fuses requests of one survey
+    queued together into one launch (max_fuse 32) and on one that does not (max_fuse 1); beside
+    them the plain per-request path on a sample."""
+    import threading
+
+    import seiswem
+    from seiswem import synth
+
+    g, vel, _, wav, sx = synth.config2(nshot=64)
+    rng = np.random.default_rng(7)
+    recs = rng.standard_normal((8, 1, g.nm, g.nt)).astype(np.float32)  # cost does not depend on values
+    reqs = [(g, vel, wav, recs[i % 8], sx[i:i + 1], None) for i in range(sx.size)]
+    arrivals = synth.poisson_arrivals(len(reqs), args.survey_rate, seed=5)
+
+    def run(srv, times):
+        ids = [None] * len(reqs)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, r in enumerate(reqs):
+                dt = times[i] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = srv.submit(*r)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat = []
+        for i in range(len(reqs)):
+            while ids[i] is None:
+                time.sleep(1e-4)
+            _, st = srv.wait(ids[i])
+            if st["state"] != seiswem.REQ_DONE:
+                raise RuntimeError(st["message"])
+            lat.append(st["done_ms"] - st["submit_ms"])
+        th.join()
+        return np.array(lat), time.perf_counter() - t0
+
+    out = {"workload": f"serving one survey: {len(reqs)} single-shot requests of the 2-D PSPI workload "
+                       f"(nx 1024, nz 500, nt 2000, 641 bins), Poisson {args.survey_rate:g}/s and all at once"}
+    for name, fuse in (("fused", 32), ("unfused", 1)):
+        srv = seiswem.Server(streams=4, max_inflight=8, max_fuse=fuse)
+        srv.wait(srv.submit(*reqs[0]))  # the band's bases resident, as in steady state
+        lat, wall = run(srv, arrivals)
+        _, wall_sat = run(srv, np.zeros(len(reqs)))
+        st = srv.stats()
+        srv.close()
+        out[name] = {"max_fuse": fuse, "served_shots_per_s": len(reqs) / wall_sat,
+                     "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+                     "poisson_run_s": wall, "saturated_run_s": wall_sat,
+                     "fused_launches": st["fused_launches"], "fused_requests": st["fused_requests"]}
+    k = min(args.plain_sample, 8)
+    t0 = time.perf_counter()
+    for r in reqs[:k]:
+        seiswem.ShotProfileWEM(r[3], True, vel=r[1], wav=r[2], geom=r[0], sx=r[4])
+    out["plain_shots_per_s"] = k / (time.perf_counter() - t0)
+    out["plain_sample"] = k
+    out["fused_vs_plain"] = out["fused"]["served_shots_per_s"] / out["plain_shots_per_s"]
+    out["fused_vs_unfused"] = out["fused"]["served_shots_per_s"] / out["unfused"]["served_shots_per_s"]
+    return out
+
+
 def bench_serving(args):
     r = measure_serving(args)
     out = {
@@ -268,6 +333,8 @@ def bench_serving(args):
         "plain_shots_per_s": r["plain_shots_per_s"], "plain_sample": r["plain_sample"],
         "served_vs_plain": r["served_vs_plain"],
     }
+    if not args.no_survey:
+        out["survey"] = measure_survey(args)
     print(json.dumps(out))
 
 
@@ -561,6 +628,9 @@ def main():
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--plain-sample", type=int, default=24, help="serving: requests timed on the plain path")
+    ap.add_argument("--survey-rate", type=float, default=100.0,
+                    help="serving: Poisson arrival rate (requests/s) of the one-survey scenario")
+    ap.add_argument("--no-survey", action="store_true", help="serving: leave out the one-survey scenario")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy a 1 GiB device buffer once (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -653,6 +723,8 @@ def main():
         out["ssf3d"] = measure_ssf3d(args, steps=args.steps_3d, warmup=1)
     if not args.no_serving:
         out["serving"] = measure_serving(args)
+        if not args.no_survey:
+            out["serving"]["survey"] = measure_survey(args)
     if args.copy_calibration:
         src = seiswem.DeviceArray((COPY_BYTES // 4,))
         dst = seiswem.DeviceArray((COPY_BYTES // 4,))

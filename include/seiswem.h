@@ -281,6 +281,10 @@ typedef struct swem_server_config {
     int max_pending;        /* queued requests before submit returns SWEM_EBUSY (default 4096) */
     double skip_window_ms;  /* a head-of-queue request that does not fit lets later ones pass
                                for this long after its submission (default 500) */
+    int max_fuse;           /* single-shot requests with the same geometry, options, velocity and
+                               wavelet that are queued together run as the shots of one launch on
+                               one plan, at most this many (default 32; 1 = never); each keeps its
+                               own image, bit-identical to the request served alone */
 } swem_server_config;
 
 typedef struct swem_request {   /* one migration request: the arguments of swem_shots, adj=1 */
@@ -306,6 +310,7 @@ typedef struct swem_server_stats {
     int pending, inflight;
     size_t device_bytes, device_bytes_used, device_bytes_peak, basis_bytes;
     double busy_ms;         /* time with at least one request on the device */
+    long long fused_launches, fused_requests;  /* launches that carried several requests, and those requests */
 } swem_server_stats;
 
 void swem_server_config_init(swem_server_config* cfg);

@@ -145,6 +145,10 @@ struct Request {
     int state = SWEM_REQ_PENDING;
     int error = SWEM_OK;
     bool claimed = false;  // a thread waits in swem_batch_wait (one waiter per request)
+    // fusion: single-shot requests with equal geometry, options, velocity and wavelet (equal
+    // key, then compared in full) run as the shots of one launch on one plan
+    uint64_t key = 0;                 // 0: not fusable (several shots)
+    std::vector<Request*> group;      // admitted: the requests of this launch, itself first
     std::string message;
     double t_submit = 0, t_admit = 0, t_done = 0;
     // resources while admitted
@@ -220,19 +224,51 @@ struct swem_server {
     long long next_id = 1;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     // statistics
-    long long submitted = 0, completed = 0, failed = 0;
+    long long submitted = 0, completed = 0, failed = 0, fused_launches = 0, fused_requests = 0;
     double busy_since = -1, busy_ms = 0, first_submit = -1, last_done = 0;
 };
 
 namespace {
 
-// Device and pinned footprint of a request (plan set up, nothing allocated yet).
-void footprint(const Request& r, const swem_plan* p, size_t& dev_bytes, size_t& pin_bytes)
+// Device and pinned footprint of a launch of nshot shots of r's shape, with nimg images (plan
+// set up, nothing allocated yet).
+void footprint(const Request& r, const swem_plan* p, int nshot, int nimg, size_t& dev_bytes, size_t& pin_bytes)
 {
     const size_t nm = p->nm;
-    const size_t dbytes = (size_t)r.nshot * nm * r.g.nt * sizeof(float), mbytes = nm * r.g.nz * sizeof(float);
-    dev_bytes = plan_migrate_bytes(p, r.nshot) + align_up(dbytes) + align_up(mbytes) + (1 << 20);
-    pin_bytes = plan_upload_bytes(p, r.nshot) + align_up(dbytes) + align_up(mbytes) + (1 << 16);
+    const size_t dbytes = (size_t)nshot * nm * r.g.nt * sizeof(float), mbytes = nimg * nm * r.g.nz * sizeof(float);
+    dev_bytes = plan_migrate_bytes(p, nshot) + align_up(dbytes) + align_up(mbytes) + (1 << 20);
+    pin_bytes = plan_upload_bytes(p, nshot) + align_up(dbytes) + align_up(mbytes) + (1 << 16);
+}
+
+// Fusion key of a single-shot request: a 64-bit hash of its geometry, options, velocity and
+// wavelet (equal keys are confirmed by a full comparison before two requests are fused).
+uint64_t fusion_key(const Request& r)
+{
+    uint64_t h = 0x9e3779b97f4a7c15ull;
+    auto mix = [&h](const void* data, size_t bytes) {
+        const unsigned char* b = static_cast<const unsigned char*>(data);
+        size_t i = 0;
+        for (; i + 8 <= bytes; i += 8) {
+            uint64_t w;
+            std::memcpy(&w, b + i, 8);
+            h = (h ^ w) * 0x100000001b3ull;
+            h ^= h >> 29;
+        }
+        for (; i < bytes; i++) h = (h ^ b[i]) * 0x100000001b3ull;
+    };
+    mix(&r.g, sizeof r.g);
+    mix(&r.opt.pspi_mode, sizeof r.opt.pspi_mode);
+    mix(&r.opt.precision, sizeof r.opt.precision);
+    mix(r.vel.data(), r.vel.size() * sizeof(float));
+    mix(r.wav.data(), r.wav.size() * sizeof(float));
+    return h ? h : 1;
+}
+
+bool fusable(const Request& a, const Request& b)
+{
+    return a.key && a.key == b.key && a.nshot == 1 && b.nshot == 1 &&
+           std::memcmp(&a.g, &b.g, sizeof a.g) == 0 && a.opt.pspi_mode == b.opt.pspi_mode &&
+           a.opt.precision == b.opt.precision && a.vel == b.vel && a.wav == b.wav;
 }
 
 // Frees what an admitted request holds (its work has completed or never started).  Worker only.
@@ -253,10 +289,15 @@ void release(swem_server* s, Request* r)
 
 enum AdmitResult { kAdmitted, kNoFit, kFailed };
 
-// Tries to admit r (worker thread, server lock not held).  kFailed leaves the status in
-// code / msg; kAdmitted may carry an error of the enqueue (code != 0), reported at completion.
-AdmitResult admit(swem_server* s, Request* r, int& code, std::string& msg)
+// Tries to admit the launch of grp (worker thread, server lock not held): grp[0] alone, or
+// several fusable single-shot requests as the shots of one launch with one image each.  kFailed
+// leaves the status in code / msg; kAdmitted may carry an error of the enqueue (code != 0),
+// reported at completion.
+AdmitResult admit(swem_server* s, const std::vector<Request*>& grp, int& code, std::string& msg)
 {
+    Request* r = grp[0];
+    const bool fused = grp.size() > 1;
+    const int nshot = fused ? (int)grp.size() : r->nshot, nimg = fused ? (int)grp.size() : 1;
     code = SWEM_OK;
     swem_plan* p = new swem_plan();
     p->served = true;
@@ -274,7 +315,7 @@ AdmitResult admit(swem_server* s, Request* r, int& code, std::string& msg)
         return kFailed;
     }
     size_t dev_bytes, pin_bytes;
-    footprint(*r, p, dev_bytes, pin_bytes);
+    footprint(*r, p, nshot, nimg, dev_bytes, pin_bytes);
     if (dev_bytes > s->dev.capacity() || pin_bytes > s->pin.capacity()) {
         plan_free(p);
         code = SWEM_ENOMEM;
@@ -292,6 +333,7 @@ AdmitResult admit(swem_server* s, Request* r, int& code, std::string& msg)
         return kNoFit;
     }
     r->plan = p;
+    r->group = grp;
     r->dev_off = doff;
     r->dev_bytes = dev_bytes;
     r->pin_off = poff;
@@ -326,13 +368,19 @@ AdmitResult admit(swem_server* s, Request* r, int& code, std::string& msg)
     stager.st = st;
     tl_dev_alloc = &bump;
     tl_stager = &stager;
-    const size_t nm = p->nm;
+    const size_t nm = p->nm, rec = nm * r->g.nt;
     DevBuf<float> d_dev, m_dev;
+    std::vector<float> sx, sy;
+    for (Request* q : grp) {
+        sx.insert(sx.end(), q->sx.begin(), q->sx.end());
+        sy.insert(sy.end(), q->sy.begin(), q->sy.end());
+    }
     rc = plan_build(p, r->wav.data());
-    if (!rc) rc = d_dev.alloc((size_t)r->nshot * nm * r->g.nt);
-    if (!rc) rc = stager.put(d_dev.p, r->d.data(), d_dev.bytes());
-    if (!rc) rc = m_dev.alloc(nm * r->g.nz);
-    if (!rc) rc = plan_migrate(p, d_dev.p, r->sx.data(), r->sy.data(), r->nshot, m_dev.p, 0, 0, st);
+    if (!rc) rc = d_dev.alloc((size_t)nshot * rec);
+    for (size_t i = 0; i < grp.size() && !rc; i++)
+        rc = stager.put(d_dev.p + (fused ? i * rec : 0), grp[i]->d.data(), grp[i]->d.size() * sizeof(float));
+    if (!rc) rc = m_dev.alloc((size_t)nimg * nm * r->g.nz);
+    if (!rc) rc = plan_migrate(p, d_dev.p, sx.data(), sy.data(), nshot, m_dev.p, 0, fused ? 1 : 0, st);
     if (!rc) {
         r->pin_out = static_cast<float*>(stager.take(m_dev.bytes()));
         if (!r->pin_out) rc = set_error(SWEM_ENOMEM, "request staging block exhausted (image)");
@@ -371,20 +419,31 @@ void worker_main(swem_server* s)
                     r->error = SWEM_EDEVICE;
                     r->message = hipGetErrorString(q);
                 }
-                if (!r->error) {
-                    r->m.assign(r->pin_out, r->pin_out + (size_t)r->g.nmx * r->g.nmy * r->g.nz);
-                    r->state = SWEM_REQ_DONE;
-                    s->completed++;
-                } else {
-                    r->state = SWEM_REQ_FAILED;
-                    s->failed++;
+                const double t = now_ms(s->t0);
+                const size_t nimg = (size_t)r->g.nmx * r->g.nmy * r->g.nz;
+                // every request of the launch (the leader last: release() uses it)
+                for (size_t k = r->group.size(); k-- > 0;) {
+                    Request* m = r->group[k];
+                    if (m != r) {
+                        m->error = r->error;
+                        m->message = r->message;
+                    }
+                    if (!r->error) {
+                        m->m.assign(r->pin_out + k * nimg, r->pin_out + (k + 1) * nimg);
+                        m->state = SWEM_REQ_DONE;
+                        s->completed++;
+                    } else {
+                        m->state = SWEM_REQ_FAILED;
+                        s->failed++;
+                    }
+                    m->t_done = t;
                 }
                 release(s, r);
-                r->t_done = now_ms(s->t0);
-                s->last_done = r->t_done;
+                r->group.clear();
+                s->last_done = t;
                 s->inflight.erase(s->inflight.begin() + i);
                 if (s->inflight.empty() && s->busy_since >= 0) {
-                    s->busy_ms += r->t_done - s->busy_since;
+                    s->busy_ms += t - s->busy_since;
                     s->busy_since = -1;
                 }
             }
@@ -405,12 +464,28 @@ void worker_main(swem_server* s)
                 }
             }
         }
+        std::vector<Request*> grp;
         for (size_t k = 0; k < cand.size(); k++) {
             Request* r = cand[k];
             if ((int)s->inflight.size() >= s->cfg.max_inflight) break;
+            // the pending requests this one fuses with (continuous batching: those already queued)
+            grp.assign(1, r);
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                if (r->state != SWEM_REQ_PENDING) continue;  // taken into an earlier launch
+                if (r->key && s->cfg.max_fuse > 1)
+                    for (Request* q : s->pending) {
+                        if ((int)grp.size() >= s->cfg.max_fuse) break;
+                        if (q != r && q->state == SWEM_REQ_PENDING && fusable(*r, *q)) grp.push_back(q);
+                    }
+            }
             int code = 0;
             std::string msg;
-            const AdmitResult res = admit(s, r, code, msg);
+            AdmitResult res = admit(s, grp, code, msg);
+            if (res == kNoFit && grp.size() > 1) {  // the launch of the group does not fit: r alone
+                grp.resize(1);
+                res = admit(s, grp, code, msg);
+            }
             std::unique_lock<std::mutex> lk(s->mu);
             const double t = now_ms(s->t0);
             if (res == kNoFit) {
@@ -422,20 +497,27 @@ void worker_main(swem_server* s)
                 code = SWEM_ENOMEM;  // does not fit even on an idle device
                 msg = "request does not fit the server's free arenas";
             }
-            s->pending.erase(std::find(s->pending.begin(), s->pending.end(), r));
+            for (Request* m : grp) {
+                s->pending.erase(std::find(s->pending.begin(), s->pending.end(), m));
+                m->error = code;
+                m->message = msg;
+                if (res == kAdmitted) {
+                    m->state = SWEM_REQ_RUNNING;
+                    m->t_admit = t;
+                } else {
+                    m->state = SWEM_REQ_FAILED;
+                    m->t_done = t;
+                    s->failed++;
+                }
+            }
             if (res == kAdmitted) {
-                r->state = SWEM_REQ_RUNNING;
-                r->t_admit = t;
-                r->error = code;
-                r->message = msg;
                 if (s->inflight.empty()) s->busy_since = t;
                 s->inflight.push_back(r);
+                if (grp.size() > 1) {
+                    s->fused_launches++;
+                    s->fused_requests += (long long)grp.size();
+                }
             } else {
-                r->state = SWEM_REQ_FAILED;
-                r->error = code;
-                r->message = msg;
-                r->t_done = t;
-                s->failed++;
                 lk.unlock();
                 s->cv_done.notify_all();
             }
@@ -457,6 +539,7 @@ void swem_server_config_init(swem_server_config* c)
     c->max_inflight = 8;
     c->max_pending = 4096;
     c->skip_window_ms = 500.0;
+    c->max_fuse = 32;
 }
 
 int swem_server_create(const swem_server_config* cfg, swem_server** out)
@@ -466,8 +549,10 @@ int swem_server_create(const swem_server_config* cfg, swem_server** out)
     swem_server_config c;
     swem_server_config_init(&c);
     if (cfg) c = *cfg;
-    if (c.streams < 1 || c.streams > 16 || c.max_inflight < 1 || c.max_pending < 1)
-        return set_error(SWEM_EINVAL, "server config: streams 1..16, max_inflight >= 1, max_pending >= 1");
+    if (c.streams < 1 || c.streams > 16 || c.max_inflight < 1 || c.max_pending < 1 || c.max_fuse < 1 ||
+        c.max_fuse > 1024)
+        return set_error(SWEM_EINVAL,
+                         "server config: streams 1..16, max_inflight >= 1, max_pending >= 1, max_fuse 1..1024");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || c.device < 0 || c.device >= count)
         return set_error(SWEM_EDEVICE, "server: device %d not available", c.device);
@@ -537,6 +622,7 @@ int swem_batch_submit(swem_server* s, const swem_request* q, long long* id)
     r->sy.assign(q->nshot, 0.f);
     if (q->sy) r->sy.assign(q->sy, q->sy + q->nshot);
     r->nshot = q->nshot;
+    if (r->nshot == 1) r->key = fusion_key(*r);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if ((int)s->pending.size() >= s->cfg.max_pending) return set_error(SWEM_EBUSY, "serving queue full");
@@ -612,6 +698,8 @@ int swem_server_get_stats(swem_server* s, swem_server_stats* out)
     out->device_bytes_peak = s->dev.peak();
     out->busy_ms = s->busy_ms + (s->busy_since >= 0 ? now_ms(s->t0) - s->busy_since : 0.0);
     out->basis_bytes = s->bases.bytes.load();
+    out->fused_launches = s->fused_launches;
+    out->fused_requests = s->fused_requests;
     return SWEM_OK;
 }
 

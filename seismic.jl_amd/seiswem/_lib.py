@@ -89,7 +89,7 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [
         ("device", ctypes.c_int), ("device_bytes", ctypes.c_size_t), ("pinned_bytes", ctypes.c_size_t),
         ("streams", ctypes.c_int), ("max_inflight", ctypes.c_int), ("max_pending", ctypes.c_int),
-        ("skip_window_ms", ctypes.c_double),
+        ("skip_window_ms", ctypes.c_double), ("max_fuse", ctypes.c_int),
     ]
 
 
@@ -114,7 +114,8 @@ class ServerStats(ctypes.Structure):
         ("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong),
         ("pending", ctypes.c_int), ("inflight", ctypes.c_int), ("device_bytes", ctypes.c_size_t),
         ("device_bytes_used", ctypes.c_size_t), ("device_bytes_peak", ctypes.c_size_t),
-        ("basis_bytes", ctypes.c_size_t), ("busy_ms", ctypes.c_double),
+        ("basis_bytes", ctypes.c_size_t), ("busy_ms", ctypes.c_double), ("fused_launches", ctypes.c_longlong),
+        ("fused_requests", ctypes.c_longlong),
     ]
 
 

@@ -15,11 +15,14 @@ from .wem import WemGeometry
 
 class Server:
     def __init__(self, device_bytes: int = 0, pinned_bytes: int = 0, streams: int = 4, max_inflight: int = 8,
-                 max_pending: int = 4096, skip_window_ms: float = 500.0, device: int = 0):
+                 max_pending: int = 4096, skip_window_ms: float = 500.0, device: int = 0, max_fuse: int = 32):
+        """max_fuse: single-shot requests over the same velocity, wavelet and geometry that are
+        queued together run as the shots of one launch (at most max_fuse; 1 = never)."""
         c = _lib.ServerConfig()
         load().swem_server_config_init(ctypes.byref(c))
         c.device, c.device_bytes, c.pinned_bytes = device, device_bytes, pinned_bytes
         c.streams, c.max_inflight, c.max_pending, c.skip_window_ms = streams, max_inflight, max_pending, skip_window_ms
+        c.max_fuse = max_fuse
         p = ctypes.c_void_p()
         check(load().swem_server_create(ctypes.byref(c), ctypes.byref(p)))
         self._s = p.value
