@@ -2003,11 +2003,11 @@ static int eshots3d_impl(swem_plan* pp, swem_plan* ps, int nshot, const float* s
     }
     std::vector<int> isrc(nshot);
     for (int i = 0; i < nshot; i++) {
-        const int ix = (int)std::trunc((sx[i] - g.omx) / g.dmx);  // ewem.c:99-100
-        const int iy = (int)std::trunc(((sy ? sy[i] : 0.f) - g.omy) / g.dmy);
-        if (ix < 0 || ix >= nmx || iy < 0 || iy >= nmy)
+        const float fx = std::trunc((sx[i] - g.omx) / g.dmx);  // ewem.c:99-100
+        const float fy = std::trunc(((sy ? sy[i] : 0.f) - g.omy) / g.dmy);
+        if (!(fx >= 0.f && fx < (float)nmx && fy >= 0.f && fy < (float)nmy))
             return set_error(SWEM_EINVAL, "ewem: source (%g, %g) outside the grid", (double)sx[i], sy ? (double)sy[i] : 0.0);
-        isrc[i] = ix * nmy + iy;
+        isrc[i] = (int)fx * nmy + (int)fy;
     }
     if (nf < 1) return SWEM_OK;
     EwemTables tb;
@@ -2095,22 +2095,27 @@ static int eshots3d_impl(swem_plan* pp, swem_plan* ps, int nshot, const float* s
                 return launch_ewem3d(1, ap, as, e, F, 0, st);
             };
             const int ngr = (ap.nitem + F - 1) / F;
+            // The truncation to y < nmy commutes with the separation and the combination (maps of
+            // kx alone), so the combination of one depth and the separation of the next share one
+            // x-pass and one truncation; an x-pass on a state another x-pass wrote scales by 1 / nkx.
+            const float inv_nk = ap.inv_nk, inv_nkx = (float)(1.0 / (double)ap.nkx);
             if (adj) {
                 e.conj_src = 1;
                 e.conj_rec = 0;
+                bool comb_due = false;  // the receiver pair of the previous depth awaits its combination
                 for (int iz = 0; iz < nz; iz++) {
                     at(iz);
                     if (e.ds) SWEM_TRY(xpass(3));
                     if (e.dr) {
-                        SWEM_TRY(xpass(0));
+                        e.s1 = inv_nk;
+                        e.s2 = inv_nkx;
+                        e.izc = iz - 1;
+                        SWEM_TRY(xpass(comb_due ? 4 : 0));
                         SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 1, st));
                         SWEM_TRY(xpass(2));
                     }
                     SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 0, st));
-                    if (e.dr) {
-                        SWEM_TRY(xpass(1));
-                        SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 1, st));
-                    }
+                    comb_due = e.dr;  // the last depth's combination is never used
                     const int acc = !first_image || f0 > 0;
                     SWEM_TRY(launch_reduce_depth(ppart.p, ngr, nm, dmpp.p + iz, nz, acc, st));
                     SWEM_TRY(launch_reduce_depth(spart.p, ngr, nm, dmps.p + iz, nz, acc, st));
@@ -2123,21 +2128,26 @@ static int eshots3d_impl(swem_plan* pp, swem_plan* ps, int nshot, const float* s
                     if (e.ds) SWEM_TRY(xpass(3));
                     SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 2, st));
                 }
-                int top = -1;  // the last (uppermost) receiver depth
-                for (int iz = 0; iz < nz; iz++)
-                    if (stepped_at(pp, iz, g.gz)) {
-                        top = iz;
-                        break;
-                    }
+                int comb_at = -1;  // depth whose combination is due
                 for (int iz = nz - 1; iz >= 0; iz--) {
                     at(iz);
                     if (!e.dr) continue;
-                    SWEM_TRY(xpass(0));
+                    if (comb_at >= 0) {
+                        e.izc = comb_at;
+                        e.s1 = e.s2 = inv_nkx;
+                        SWEM_TRY(xpass(4));
+                    } else {
+                        e.s1 = inv_nk;  // the initial (zero) state
+                        SWEM_TRY(xpass(0));
+                    }
                     SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 3, st));
                     SWEM_TRY(xpass(2));
-                    SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 1, st));
+                    comb_at = iz;
+                }
+                if (comb_at >= 0) {  // the uppermost combination
+                    at(comb_at);
+                    e.s1 = inv_nkx;
                     SWEM_TRY(xpass(1));
-                    if (iz != top) SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 1, st));
                 }
                 // the modelled spectra: the last combination's state (or the zero initial state)
                 SWEM_TRY(launch_ewem3d(2, ap, as, e, F, 4, st));
@@ -2171,10 +2181,12 @@ int swem_eshots(const swem_geometry* g, const float* vp, const float* vs, const 
     int ntfft, ifmin, ifmax;
     ewem_sampling(ge, ntfft, ifmin, ifmax);
     const bool three_d = ge.nmy > 1;
+    if (!(ge.dmx > 0.f)) return set_error(SWEM_EINVAL, "ewem: dmx must be positive");
     std::vector<int> igx(nshot);
     for (int i = 0; i < nshot && !three_d; i++) {
-        igx[i] = (int)std::trunc((sx[i] - ge.omx) / ge.dmx);  // ewem.c:99
-        if (igx[i] < 0 || igx[i] >= ge.nmx) return set_error(SWEM_EINVAL, "ewem: source x %g outside the grid", (double)sx[i]);
+        const float fx = std::trunc((sx[i] - ge.omx) / ge.dmx);  // ewem.c:99
+        if (!(fx >= 0.f && fx < (float)ge.nmx)) return set_error(SWEM_EINVAL, "ewem: source x %g outside the grid", (double)sx[i]);
+        igx[i] = (int)fx;
     }
     swem_options o;
     if (opt) o = *opt;

@@ -236,7 +236,12 @@ struct Ewem3DArgs {
     int sep_adj;            // matrix branch of the separation / combination (1: migration)
     int ds, dr, next_s;     // source / receiver stepped at this depth; source stepped at the next
     int conj_src, conj_rec; // x-pass: conj(L) for the -dz steps
-    int mode;               // x-pass: 0 separation, 1 combination, 2 kz phases of the pair, 3 of the source
+    int mode;               // x-pass: 0 separation, 1 combination, 2 kz phases of the pair, 3 of the source,
+                            // 4 combination at depth izc then separation at iz (one x-pass, the
+                            // truncation to x < nmx between them)
+    int izc;                // mode 4: depth of the combination
+    float s1, s2;           // scales of the x-pass's first and second 2 x 2 maps: 1 / (nkx nky) on a
+                            // state a y-pass wrote, 1 / nkx on one an x-pass wrote
     float dwf, dkxf;        // float dw and dkx, as ewem.c forms w and the separation's kx
     float inv_maxs;         // 1 / (max|wav| / sqrt(ntfft)) (ewem.c:146)
     const float* vp0;       // [nz] 1 / po_p (separation)

@@ -1021,11 +1021,12 @@ __global__ __launch_bounds__(256) void ewem3d_init(March3DArgs a, Ewem3DArgs e)
 }
 
 // x-pass.  NWF = 1 (e.mode 3): the source (slot 0), kz phase of vp (conj_src).  NWF = 2: the
-// receiver pair (slots 1, 2): e.mode 0 separation, 1 combination (the 2 x 2 map of kx,
-// ewem.c:593-630, 680-710), 2 kz phases of vp and vs (conj_rec).  Columns staged through LDS in
-// tiles of C.
-template <int NX, int NWF, int C, int BX>
-__global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 1024 ? 2 : 4))) void ewem3d_cols(
+// receiver pair (slots 1, 2): MAPS, the 2 x 2 maps of kx (ewem.c:593-630, 680-710): e.mode 0
+// separation, 1 combination, 4 combination of depth izc then separation of depth iz with the
+// truncation to x < nmx between them; else e.mode 2, the kz phases of vp and vs (conj_rec).
+// Columns staged through LDS in tiles of C.
+template <int NX, int NWF, int C, int BX, bool MAPS>
+__global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 1024 ? 2 : (MAPS && NX >= 512 ? 3 : 4)))) void ewem3d_cols(
     March3DArgs ap, March3DArgs as, Ewem3DArgs e)
 {
     constexpr int P = NX / 64;
@@ -1060,61 +1061,91 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
         }
     }
     __syncthreads();
-    float2 v[NWF][P];
-#pragma unroll
-    for (int k = 0; k < NWF; ++k) {
+    const int iw = a.ifmin + a.f0 + item;
+    const int iky = ky0 + w;
+    // a column of wavefield k: from the wave's buffer into registers, forward x-FFT
+    auto fetch = [&](int k, float2 (&u)[P]) {
         float2* buf = smem + (w * NWF + k) * L;
 #pragma unroll
         for (int q = 0; q < P; ++q) {
             const int x = t + 64 * q;
-            v[k][q] = x < a.nmx ? buf[lds_pad(x)] : make_float2(0.f, 0.f);
+            u[q] = x < a.nmx ? buf[lds_pad(x)] : make_float2(0.f, 0.f);
         }
         wave_sync_lds();
-        xfft<NX, -1, BX>(v[k], buf, a, t);
-    }
-    const int iw = a.ifmin + a.f0 + item;
-    const int iky = ky0 + w;
-    if (NWF == 2 && e.mode != 2) {
-        const float wf = (float)iw * e.dwf;
-        const float vp = e.vp0[a.iz], vs = e.vs0[a.iz];
-        const int n = a.nkx;
+        xfft<NX, -1, BX>(u, buf, a, t);
+    };
+    // inverse x-FFT and back into the buffer (x < nmx)
+    auto put = [&](int k, float2 (&u)[P]) {
+        float2* buf = smem + (w * NWF + k) * L;
+        wave_sync_lds();
+        xfft<NX, +1, BX>(u, buf, a, t);
+        wave_sync_lds();
 #pragma unroll
         for (int q = 0; q < P; ++q) {
-            const int k = t + 64 * q;
-            const float kx = (float)k < (float)n / 2.f ? e.dkxf * (float)k : -(e.dkxf * (float)n - e.dkxf * (float)k);
-            float2 r1, r2;
-            esep_k(kx, wf, vp, vs, e.mode == 0, e.sep_adj != 0, v[0][q], v[NWF - 1][q], r1, r2);
-            v[0][q] = k < n ? cscale(r1, a.inv_nk) : make_float2(0.f, 0.f);
-            v[NWF - 1][q] = k < n ? cscale(r2, a.inv_nk) : make_float2(0.f, 0.f);
+            const int x = t + 64 * q;
+            if (x < a.nmx) buf[lds_pad(x)] = u[q];
         }
+    };
+    if constexpr (MAPS) {
+        static_assert(NWF == 2, "the 2 x 2 maps act on the receiver pair");
+        float2 v[2][P];
+        fetch(0, v[0]);
+        fetch(1, v[1]);
+        const float wf = (float)iw * e.dwf;
+        const int n = a.nkx;
+        // the 2 x 2 map of kx (separation or combination) with the layer velocities of depth iz
+        auto map = [&](bool sep, int iz, float sc) {
+            const float vp = e.vp0[iz], vs = e.vs0[iz];
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const int k = t + 64 * q;
+                const float kx = (float)k < (float)n / 2.f ? e.dkxf * (float)k : -(e.dkxf * (float)n - e.dkxf * (float)k);
+                float2 r1, r2;
+                esep_k(kx, wf, vp, vs, sep, e.sep_adj != 0, v[0][q], v[1][q], r1, r2);
+                v[0][q] = k < n ? cscale(r1, sc) : make_float2(0.f, 0.f);
+                v[1][q] = k < n ? cscale(r2, sc) : make_float2(0.f, 0.f);
+            }
+        };
+        if (e.mode == 4) {
+            // combination of depth izc, back to x (truncated to x < nmx), separation of depth iz
+            map(false, e.izc, e.s1);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                float2* buf = smem + (w * NWF + k) * L;
+                wave_sync_lds();
+                xfft<NX, +1, BX>(v[k], buf, a, t);
+#pragma unroll
+                for (int q = 0; q < P; ++q)
+                    if (t + 64 * q >= a.nmx) v[k][q] = make_float2(0.f, 0.f);
+                wave_sync_lds();
+                xfft<NX, -1, BX>(v[k], buf, a, t);
+            }
+            map(true, a.iz, e.s2);
+        } else {
+            map(e.mode == 0, a.iz, e.s1);
+        }
+        put(0, v[0]);
+        put(1, v[1]);
     } else {
+        // kz phases, one wavefield at a time (the source with vp; the pair with vp and vs)
         const double wd = (double)iw * a.dw;
         const int kyi = iky <= a.nky / 2 ? iky : a.nky - iky;
         const double ky = kyi * a.dky;
+        const bool cj = NWF == 1 ? e.conj_src : e.conj_rec;
 #pragma unroll
         for (int k = 0; k < NWF; ++k) {
             const March3DArgs& pl = (NWF == 2 && k == 1) ? as : ap;  // us2 steps with vs
+            float2 v[P];
+            fetch(k, v);
             const double p2 = pl.p2[a.iz];
             const double c0 = wd * wd * p2 - ky * ky;
-            const bool cj = NWF == 1 ? e.conj_src : e.conj_rec;
 #pragma unroll
             for (int q = 0; q < P; ++q) {
                 const float2 ph = a.prec == SWEM_PREC_KZ64 ? kz_phase64(pl, c0, t + 64 * q)
                                                            : kz_phase(pl, wd, p2, pl.p2f[a.iz], t + 64 * q, iky);
-                v[k][q] = cmul(v[k][q], make_float2(ph.x, cj ? -ph.y : ph.y));
+                v[q] = cmul(v[q], make_float2(ph.x, cj ? -ph.y : ph.y));
             }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < NWF; ++k) {
-        float2* buf = smem + (w * NWF + k) * L;
-        wave_sync_lds();
-        xfft<NX, +1, BX>(v[k], buf, a, t);
-        wave_sync_lds();
-#pragma unroll
-        for (int q = 0; q < P; ++q) {
-            const int x = t + 64 * q;
-            if (x < a.nmx) buf[lds_pad(x)] = v[k][q];
+            put(k, v);
         }
     }
     __syncthreads();
@@ -1308,12 +1339,15 @@ static int launch_ewem3d_n(int which, const March3DArgs& ap, const March3DArgs& 
         return SWEM_OK;
     }
     if (which == 1) {
-        const bool src = e.mode == 3;  // the source's kz x-pass
-        constexpr int C2 = N >= 1024 ? 4 : 8, C1 = 2 * C2;
-        const int C = src ? C1 : C2, nwf = src ? 1 : 2;
+        const bool src = e.mode == 3, maps = e.mode != 2 && !src;  // the source's kz; the 2 x 2 maps
+        // the maps' kernel holds both wavefields through two transforms each: at N = 512 it runs
+        // 4 columns per workgroup at 3 waves per SIMD rather than spill at 4
+        constexpr int C2 = N >= 1024 ? 4 : 8, C1 = 2 * C2, CM = N >= 512 ? 4 : 8;
+        const int C = src ? C1 : (maps ? CM : C2), nwf = src ? 1 : 2;
         const size_t lds = (size_t)C * nwf * L * sizeof(float2);
         const dim3 grid(ap.nitem * ((ap.nky + C - 1) / C)), block(64 * C);
-        auto k = src ? ewem3d_cols<N, 1, C1, BL> : ewem3d_cols<N, 2, C2, BL>;
+        auto k = src ? ewem3d_cols<N, 1, C1, BL, false>
+                     : (maps ? ewem3d_cols<N, 2, CM, BL, true> : ewem3d_cols<N, 2, C2, BL, false>);
         if (lds > 48 * 1024) SWEM_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k, grid, block, lds, st, ap, as, e);
         SWEM_HIP(hipGetLastError());
