@@ -157,6 +157,7 @@ struct Request {
     hipEvent_t done = nullptr;
     int stream = -1;
     float* pin_out = nullptr;
+    std::atomic<bool> enqueued{false};  // a preparation thread has enqueued the launch and its event
     // allocator of the request's device block: the plan's buffers refer to it until released
     BumpAlloc bump;
     std::vector<float> m;
@@ -165,9 +166,11 @@ struct Request {
 // Fourier bases per band, resident for the server's lifetime (a handful of distinct bands).
 struct ServerBases : BasisCache {
     std::map<std::vector<long long>, std::unique_ptr<Bases>> entries;
-    std::atomic<size_t> bytes{0};  // written by the worker, read by swem_server_get_stats
+    std::mutex mu;                 // the preparation threads look up and build concurrently
+    std::atomic<size_t> bytes{0};  // read by swem_server_get_stats
     const Bases* find_or_build(const swem_geometry& g, const Sampling& s, int nf) override
     {
+        std::lock_guard<std::mutex> lk(mu);
         long long dtbits = 0;
         std::memcpy(&dtbits, &g.dt, sizeof(float));
         std::vector<long long> key{g.nt, dtbits, s.ntfft, s.ifmin, nf, s.fop_n};
@@ -221,6 +224,12 @@ struct swem_server {
     std::unordered_map<long long, std::unique_ptr<Request>> all;
     std::thread worker;
     bool stop = false;
+    // preparation threads: tables, uploads and the launch of admitted requests (host work that
+    // would otherwise serialise admissions on the worker)
+    std::vector<std::thread> prep;
+    std::deque<Request*> jobs;
+    std::condition_variable cv_prep;
+    bool prep_stop = false;
     long long next_id = 1;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     // statistics
@@ -240,8 +249,9 @@ void footprint(const Request& r, const swem_plan* p, int nshot, int nimg, size_t
     pin_bytes = plan_upload_bytes(p, nshot) + align_up(dbytes) + align_up(mbytes) + (1 << 16);
 }
 
-// Fusion key of a single-shot request: a 64-bit hash of its geometry, options, velocity and
-// wavelet (equal keys are confirmed by a full comparison before two requests are fused).
+// Fusion key of a single-shot request: a 64-bit hash of its geometry, options and wavelet and of
+// 64 velocity samples spread over the model, so that a submission does not read a 3-D velocity
+// through (equal keys are confirmed by a full comparison before two requests are fused).
 uint64_t fusion_key(const Request& r)
 {
     uint64_t h = 0x9e3779b97f4a7c15ull;
@@ -259,7 +269,9 @@ uint64_t fusion_key(const Request& r)
     mix(&r.g, sizeof r.g);
     mix(&r.opt.pspi_mode, sizeof r.opt.pspi_mode);
     mix(&r.opt.precision, sizeof r.opt.precision);
-    mix(r.vel.data(), r.vel.size() * sizeof(float));
+    const size_t nv = r.vel.size(), step = std::max<size_t>(1, nv / 64);
+    for (size_t i = 0; i < nv; i += step) mix(&r.vel[i], sizeof(float));
+    mix(&r.vel[nv - 1], sizeof(float));
     mix(r.wav.data(), r.wav.size() * sizeof(float));
     return h ? h : 1;
 }
@@ -290,9 +302,10 @@ void release(swem_server* s, Request* r)
 enum AdmitResult { kAdmitted, kNoFit, kFailed };
 
 // Tries to admit the launch of grp (worker thread, server lock not held): grp[0] alone, or
-// several fusable single-shot requests as the shots of one launch with one image each.  kFailed
-// leaves the status in code / msg; kAdmitted may carry an error of the enqueue (code != 0),
-// reported at completion.
+// several fusable single-shot requests as the shots of one launch with one image each.  kAdmitted
+// reserves the plan (set up, nothing built), the arena blocks, a stream and an event; a preparation
+// thread then builds and enqueues the launch (enqueue_launch).  kFailed leaves the status in
+// code / msg.
 AdmitResult admit(swem_server* s, const std::vector<Request*>& grp, int& code, std::string& msg)
 {
     Request* r = grp[0];
@@ -358,16 +371,31 @@ AdmitResult admit(swem_server* s, const std::vector<Request*>& grp, int& code, s
     r->done = s->event_pool.back();
     s->event_pool.pop_back();
 
+    return kAdmitted;
+}
+
+// Builds the plan of an admitted launch (its tables into the request's arena block), stages the
+// records, enqueues the migration and the image download on the request's stream and records its
+// event (a preparation thread).  Returns the status of the enqueue; the event is recorded either
+// way, so the arena blocks are released only after the stream has passed whatever was enqueued.
+int enqueue_launch(swem_server* s, Request* r, std::string& msg)
+{
+    const std::vector<Request*>& grp = r->group;
+    const bool fused = grp.size() > 1;
+    const int nshot = fused ? (int)grp.size() : r->nshot, nimg = fused ? (int)grp.size() : 1;
+    swem_plan* p = r->plan;
+    hipStream_t st = s->streams[r->stream];
     BumpAlloc& bump = r->bump;
-    bump.base = s->dev_base + doff;
-    bump.cap = dev_bytes;
+    bump.base = s->dev_base + r->dev_off;
+    bump.cap = r->dev_bytes;
     bump.used = 0;
     PinnedStager stager;
-    stager.base = s->pin_base + poff;
-    stager.cap = pin_bytes;
+    stager.base = s->pin_base + r->pin_off;
+    stager.cap = r->pin_bytes;
     stager.st = st;
     tl_dev_alloc = &bump;
     tl_stager = &stager;
+    int rc = SWEM_OK;
     const size_t nm = p->nm, rec = nm * r->g.nt;
     DevBuf<float> d_dev, m_dev;
     std::vector<float> sx, sy;
@@ -390,13 +418,32 @@ AdmitResult admit(swem_server* s, const std::vector<Request*>& grp, int& code, s
     tl_dev_alloc = nullptr;
     tl_stager = nullptr;
     d_dev.p = m_dev.p = nullptr;  // both live in the request block
-    if (rc) {
-        // work may be enqueued already: the memory is kept until the stream has passed it
-        code = rc;
-        msg = swem_last_error();
-    }
+    if (rc) msg = swem_last_error();
     (void)hipEventRecord(r->done, st);
-    return kAdmitted;
+    return rc;
+}
+
+void prep_main(swem_server* s)
+{
+    (void)hipSetDevice(s->cfg.device);
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_prep.wait(lk, [s] { return s->prep_stop || !s->jobs.empty(); });
+            if (s->jobs.empty()) return;
+            r = s->jobs.front();
+            s->jobs.pop_front();
+        }
+        std::string msg;
+        const int rc = enqueue_launch(s, r, msg);
+        if (rc) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            r->error = rc;
+            r->message = msg;
+        }
+        r->enqueued.store(true, std::memory_order_release);
+    }
 }
 
 void worker_main(swem_server* s)
@@ -408,6 +455,10 @@ void worker_main(swem_server* s)
         // completions
         for (size_t i = 0; i < s->inflight.size();) {
             Request* r = s->inflight[i];
+            if (!r->enqueued.load(std::memory_order_acquire)) {  // still being prepared
+                ++i;
+                continue;
+            }
             const hipError_t q = hipEventQuery(r->done);
             if (q == hipErrorNotReady) {
                 ++i;
@@ -517,6 +568,8 @@ void worker_main(swem_server* s)
                     s->fused_launches++;
                     s->fused_requests += (long long)grp.size();
                 }
+                s->jobs.push_back(r);
+                s->cv_prep.notify_one();
             } else {
                 lk.unlock();
                 s->cv_done.notify_all();
@@ -579,6 +632,7 @@ int swem_server_create(const swem_server_config* cfg, swem_server** out)
         s->stream_load.push_back(0);
     }
     swem_server* raw = s.release();
+    for (int i = 0; i < c.streams; i++) raw->prep.emplace_back(prep_main, raw);
     raw->worker = std::thread(worker_main, raw);
     *out = raw;
     return SWEM_OK;
@@ -592,7 +646,13 @@ int swem_server_destroy(swem_server* s)
         s->stop = true;
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    if (s->worker.joinable()) s->worker.join();  // returns when nothing is pending or in flight
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->prep_stop = true;
+    }
+    s->cv_prep.notify_all();
+    for (std::thread& t : s->prep) t.join();
     (void)hipSetDevice(s->cfg.device);
     for (hipStream_t st : s->streams) (void)hipStreamSynchronize(st);
     for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
