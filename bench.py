@@ -700,7 +700,8 @@ def main():
                 "kernel": "march2d_adj_packed<2048,16>" if packed else "march2d_adj (PSPI, nkx=2048)",
                 "compute_unit": "FP32 vector ALU (no stage is a matrix product); peak = MI355X FP32 157.3 TFLOP/s",
                 "flops_basis": "reference operation count: 2 (1 + nref) transforms per depth and frequency"
-                               + ("; the packed kernel executes 3 + nref" if packed else ""),
+                               + ("; the packed kernel executes 3 + nref, and nref where at most four "
+                                  "points select the top vref" if packed else ""),
                 "avg_launch_ms": march_ms / launches, "flops_per_launch": flops_per_launch}
     out = {
         "metric": "migrated shots/sec (2D & 3D) at <=1e-4 rel-L2 vs FP64; p50/p99 request latency",

@@ -363,8 +363,8 @@ static int build_tables(swem_plan* p, const float* wav)
         SWEM_TRY(p->used.upload(used.data(), used.size()));
         // Packed compat migration: every depth steps both wavefields (z >= sz and z >= gz,
         // wem.c:235-241) and the uncut points of a depth share one vref.
-        // Per depth: vref of the uncut points (-1: none), their count and the first kUncutSparse
-        // positions.
+        // Per depth: vref of the uncut points (-1: none), their count, 1 when no cut point selects
+        // that vref, and the first kUncutSparse positions.
         std::vector<int> cref((size_t)nz * kCrefStride, 0);
         bool packed = p->opt.pspi_mode == SWEM_PSPI_COMPAT && !p->three_d && narr == nkx;
         for (int iz = 0; iz < nz; iz++) {
@@ -379,9 +379,17 @@ static int build_tables(swem_plan* p, const float* wav)
                 int* row = cref.data() + (size_t)iz * kCrefStride;
                 if (row[0] < 0) row[0] = i1[i];
                 else if (row[0] != i1[i]) packed = false;
-                if (row[1] < kUncutSparse) row[2 + row[1]] = ix;
+                if (row[1] < kUncutSparse) row[3 + row[1]] = ix;
                 row[1]++;
             }
+        for (int iz = 0; iz < nz; iz++) {  // does a cut point select the uncut points' vref?
+            int* row = cref.data() + (size_t)iz * kCrefStride;
+            row[2] = row[0] >= 0;
+            for (int ix = 0; ix < nm && row[2]; ix++) {
+                const size_t i = (size_t)ix * nz + iz;
+                if (cut[i] && i1[i] == row[0]) row[2] = 0;
+            }
+        }
         SWEM_TRY(p->cref.upload(cref.data(), cref.size()));
         p->packed_ok = packed;
     }

@@ -75,9 +75,9 @@ int launch_phase_tables(const PhaseTableArgs& a, bool ss, hipStream_t st);
 // Uncut-point table of the packed PSPI compat kernel: depths with at most kUncutSparse uncut points
 // use direct sums over them instead of two transforms.
 constexpr int kUncutSparse = 4;
-constexpr int kCrefStride = 2 + kUncutSparse;
+constexpr int kCrefStride = 3 + kUncutSparse;
 // Complex LDS scratch after the transform buffers of a packed-kernel workgroup.
-constexpr int kPackedScratch = 32;
+constexpr int kPackedScratch = 48;
 
 struct March2DArgs {
     int nmx, nz, nf, nref, ngroups, ifmin, nshot;
@@ -101,7 +101,8 @@ struct March2DArgs {
     const float* w2;        // [nz][nkx] weight of iref2 (PSPI linear)
     const uint32_t* used;   // [nz] bit ir set when some x selects vref ir at iz
     const int* cref;        // [nz][kCrefStride] uncut points of depth iz (PSPI compat): vref (-1: none),
-                            // count, the first kUncutSparse positions x
+                            // count, 1 when no cut point selects that vref, the first kUncutSparse
+                            // positions x
     float* partial;         // [nshot][ngroups][nz][nmx] migration partial images
     const float* mT;        // [nz][nmx] image (modelling input, depth-major)
     float2* smig;           // [nshot][nfc][nz][nmx] source wavefield scratch (modelling)

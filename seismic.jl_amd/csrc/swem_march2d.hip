@@ -699,7 +699,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
     constexpr int T = N / P;
     constexpr int K = kUncutSparse;
     constexpr int NW = T / 64;  // waves of the subgroup
-    static_assert(4 + K + NW * K <= kPackedScratch, "packed scratch");
+    static_assert(4 + K + 2 * NW * K <= kPackedScratch, "packed scratch");
     extern __shared__ float2 smem[];
     int g, shot;
     decode_block(a.ngroups, a.nshot, g, shot);
@@ -759,7 +759,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (j >= prev_cnt) break;
-                const int xj = prev_row[2 + j];
+                const int xj = prev_row[3 + j];
 #pragma unroll
                 for (int q = 0; q < P; ++q)
                     if (t + q * T == xj) scr_e[j] = e[q];
@@ -770,7 +770,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if (j >= prev_cnt) break;
-                const int xj = prev_row[2 + j];
+                const int xj = prev_row[3 + j];
                 const float2 ev = scr_e[j];
                 float2 w = a.tw[(t * xj) & (N - 1)];
                 const float2 st = a.tw[(T * xj) & (N - 1)];
@@ -793,8 +793,12 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
         }
         auto s1 = [&](int q) { return (s1w[q >> 2] >> (8 * (q & 3))) & 0xffu; };
         const uint32_t used = a.used[iz];
+        // Few uncut points, and no cut point selecting their vref cr: their output under cr, like
+        // their new e, comes from direct sums instead of an inverse transform of cr (HALF kernels:
+        // the others have no registers left for the second set of sums).
+        const bool direct_cr = HALF && cnt > 0 && cnt <= K && crow[2] != 0;
         for (int ir = 0; ir < a.nref; ++ir) {
-            if (!((used >> ir) & 1u)) continue;
+            if (!((used >> ir) & 1u) || (direct_cr && ir == cr)) continue;
             const float2* row = krow(a, c, ir);
             float2 w[P];
             kload_row<N, P>(w, row, c.t);
@@ -814,12 +818,12 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
             const float2* row = krow(a, c, cr);
             float2 lv[P];
             kload_row<N, P>(lv, row, c.t);
-#pragma unroll
-            for (int q = 0; q < P; ++q) {
-                const float2 L = lv[q];
-                e[q] = make_float2(L.x * e[q].x - L.y * z[q].x, L.x * e[q].y - L.y * z[q].y);
-            }
             if (cnt > K) {
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const float2 L = lv[q];
+                    e[q] = make_float2(L.x * e[q].x - L.y * z[q].x, L.x * e[q].y - L.y * z[q].y);
+                }
                 xfft<N, P, +1, WAVE>(e, smem, a.tw, t, pp);
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
@@ -827,28 +831,45 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
                     if ((HALF && q >= P / 2) || !(x < a.nmx && !(s1(q) >> 7))) e[q] = make_float2(0.f, 0.f);
                 }
             } else {
-                // e_new(x_j) = sum_k U(k) conj(w^(k x_j)) at the few uncut points: partial sums
-                // per thread, then over the subgroup in a fixed order
-                float2 acc[K];
+                // e_new(x_j) = sum_k U(k) conj(w^(k x_j)) at the few uncut points and, when direct_cr,
+                // out(x_j) = sum_k W(k) conj(w^(k x_j)) with W = Re L Z + Im L E: partial sums per
+                // thread, then over the subgroup in a fixed order
+                // U into e and W into lv, in place (the spectra z are not needed after them)
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const float2 L = lv[q];
+                    const float2 u = make_float2(L.x * e[q].x - L.y * z[q].x, L.x * e[q].y - L.y * z[q].y);
+                    lv[q] = make_float2(z[q].x * L.x + e[q].x * L.y, z[q].y * L.x + e[q].y * L.y);
+                    e[q] = u;
+                }
+                float2 acc[K], accw[K];
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    acc[j] = make_float2(0.f, 0.f);
+                    acc[j] = accw[j] = make_float2(0.f, 0.f);
                     if (j >= cnt) continue;
-                    const int xj = crow[2 + j];
+                    const int xj = crow[3 + j];
                     float2 w = a.tw[(t * xj) & (N - 1)];
                     const float2 st = a.tw[(T * xj) & (N - 1)];
 #pragma unroll
                     for (int q = 0; q < P; ++q) {
                         acc[j] = cadd(acc[j], cmulc(e[q], w));
+                        if (direct_cr) accw[j] = cadd(accw[j], cmulc(lv[q], w));
                         w = cmul(w, st);
                     }
 #pragma unroll
                     for (int o = 32; o > 0; o >>= 1) {
                         acc[j].x += __shfl_xor(acc[j].x, o, 64);
                         acc[j].y += __shfl_xor(acc[j].y, o, 64);
+                        if (direct_cr) {
+                            accw[j].x += __shfl_xor(accw[j].x, o, 64);
+                            accw[j].y += __shfl_xor(accw[j].y, o, 64);
+                        }
                     }
                     if constexpr (NW > 1)
-                        if ((t & 63) == 0) scr_v[(t >> 6) * K + j] = acc[j];
+                        if ((t & 63) == 0) {
+                            scr_v[(t >> 6) * K + j] = acc[j];
+                            scr_v[(NW + (t >> 6)) * K + j] = accw[j];
+                        }
                 }
                 if constexpr (NW > 1) {
                     subgroup_sync<T>();
@@ -856,8 +877,12 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
                     for (int j = 0; j < K; ++j) {
                         if (j >= cnt) break;
                         acc[j] = scr_v[j];
+                        accw[j] = scr_v[NW * K + j];
 #pragma unroll
-                        for (int wv = 1; wv < NW; ++wv) acc[j] = cadd(acc[j], scr_v[wv * K + j]);
+                        for (int wv = 1; wv < NW; ++wv) {
+                            acc[j] = cadd(acc[j], scr_v[wv * K + j]);
+                            accw[j] = cadd(accw[j], scr_v[(NW + wv) * K + j]);
+                        }
                     }
                 }
 #pragma unroll
@@ -865,10 +890,13 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     if (j >= cnt) break;
-                    const int xj = crow[2 + j];
+                    const int xj = crow[3 + j];
 #pragma unroll
                     for (int q = 0; q < (HALF ? P / 2 : P); ++q)
-                        if (t + q * T == xj) e[q] = acc[j];
+                        if (t + q * T == xj) {
+                            e[q] = acc[j];
+                            if (direct_cr) out[q] = accw[j];
+                        }
                 }
             }
         }
