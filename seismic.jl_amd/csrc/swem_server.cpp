@@ -527,9 +527,13 @@ void worker_main(swem_server* s)
                 if (r->key && s->cfg.max_fuse > 1)
                     for (Request* q : s->pending) {
                         if ((int)grp.size() >= s->cfg.max_fuse) break;
-                        if (q != r && q->state == SWEM_REQ_PENDING && fusable(*r, *q)) grp.push_back(q);
+                        if (q != r && q->state == SWEM_REQ_PENDING && q->key == r->key) grp.push_back(q);
                     }
             }
+            // equal keys confirmed in full outside the lock: a request's inputs do not change after
+            // submission, and only this thread takes requests out of the pending state
+            grp.erase(std::remove_if(grp.begin() + 1, grp.end(), [r](const Request* q) { return !fusable(*r, *q); }),
+                      grp.end());
             int code = 0;
             std::string msg;
             AdmitResult res = admit(s, grp, code, msg);
