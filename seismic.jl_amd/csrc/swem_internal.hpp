@@ -77,7 +77,7 @@ int launch_phase_tables(const PhaseTableArgs& a, bool ss, hipStream_t st);
 constexpr int kUncutSparse = 4;
 constexpr int kCrefStride = 3 + kUncutSparse;
 // Complex LDS scratch after the transform buffers of a packed-kernel workgroup.
-constexpr int kPackedScratch = 48;
+constexpr int kPackedScratch = 80;
 
 struct March2DArgs {
     int nmx, nz, nf, nref, ngroups, ifmin, nshot;

@@ -28,6 +28,8 @@
 //   pspiop             wem.c:380-471  FFT, per vref (phase, IFFT), per-x selection with the
 //                                     real-part cut of linear_interp (wem.c:473-478)
 // A step of -dz uses the complex conjugate of the +dz phases (the evanescent factor is real).
+#include <type_traits>
+
 #include "swem_esep.hpp"
 #include "swem_fft.hpp"
 #include "swem_internal.hpp"
@@ -693,12 +695,17 @@ __device__ __forceinline__ void subgroup_sync()
 
 // HALF: nmx <= N / 2, so the points q >= P / 2 (x >= N / 2) are zero in the state and never selected:
 // those constants let the first forward pass and the last inverse pass drop their work.
-template <int N, int P, bool WAVE, bool HALF>
+// X: the transform policy (XPow2: nkx = N; XMixed: a mixed-radix nkx <= N, whose twiddle table
+// holds nkx entries and whose spectra are zero at k >= nkx).
+template <int N, int P, bool WAVE, bool HALF, class X = XPow2<N, P>>
 __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
 {
     constexpr int T = N / P;
     constexpr int K = kUncutSparse;
     constexpr int NW = T / 64;  // waves of the subgroup
+    constexpr bool POW2 = std::is_same<X, XPow2<N, P>>::value;
+    // exp(-2 pi i m / nkx) for the direct sums (m >= 0)
+    auto twid = [&](int m) { return POW2 ? a.tw[m & (N - 1)] : a.tw[m % a.nkx]; };
     static_assert(4 + K + 2 * NW * K <= kPackedScratch, "packed scratch");
     extern __shared__ float2 smem[];
     int g, shot;
@@ -732,7 +739,18 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
     }
     nr = subgroup_sum<T>(nr, red);
     const float ns = sqrtf(ss.x * ss.x + ss.y * ss.y);
-    float alpha = (nr > 0.f && ns > 0.f) ? ns / sqrtf(nr) : 1.f;
+    float* part = a.partial + ((size_t)shot * a.ngroups + g) * (size_t)a.nz * a.nmx;
+    if (!(nr > 0.f && ns > 0.f)) {
+        // a zero receiver or source spectrum images to zero at every depth (the march is linear
+        // in each): the packed pair would leave rounding noise of the other wavefield instead
+        for (int iz = 0; iz < a.nz; ++iz) {
+            const auto pr = buf_rsrc(part + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
+#pragma unroll
+            for (int q = 0; q < P; ++q) buf_store1_nt(pr, (uint32_t)(t + q * T) * 4u, 0.f);
+        }
+        return;
+    }
+    float alpha = ns / sqrtf(nr);
     float inv_alpha = 1.f / alpha;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
@@ -742,7 +760,6 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
         e[q] = make_float2(us.y, -alpha * e[q].y);
         if (HALF && q >= P / 2) z[q] = e[q] = make_float2(0.f, 0.f);
     }
-    float* part = a.partial + ((size_t)shot * a.ngroups + g) * (size_t)a.nz * a.nmx;
     // e before the step: dense (the initial imaginary parts), sparse (nonzero only at the
     // listed uncut points of the previous depth: prev_cnt <= K) or zero
     int prev_cnt = -1;  // -1: dense
@@ -751,9 +768,9 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
         c.iz = iz;
         const int* crow = a.cref + (size_t)iz * kCrefStride;
         const int cr = crow[0], cnt = crow[1];
-        xfft<N, P, -1, WAVE>(z, smem, a.tw, t, pp);
+        X::template fft<-1, WAVE>(z, smem, a, t, pp);
         if (prev_cnt < 0 || prev_cnt > K) {
-            xfft<N, P, -1, WAVE>(e, smem, a.tw, t, pp);
+            X::template fft<-1, WAVE>(e, smem, a, t, pp);
         } else if (prev_cnt > 0) {
             // E(k) = sum_j e(x_j) w^(k x_j), w = exp(-2 pi i / N): the owners publish e(x_j)
 #pragma unroll
@@ -772,13 +789,18 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
                 if (j >= prev_cnt) break;
                 const int xj = prev_row[3 + j];
                 const float2 ev = scr_e[j];
-                float2 w = a.tw[(t * xj) & (N - 1)];
-                const float2 st = a.tw[(T * xj) & (N - 1)];
+                float2 w = twid(t * xj);
+                const float2 st = twid(T * xj);
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
                     e[q] = cadd(e[q], cmul(ev, w));
                     w = cmul(w, st);
                 }
+            }
+            if constexpr (!POW2) {  // the spectrum of a length nkx < N ends at nkx
+#pragma unroll
+                for (int q = 0; q < P; ++q)
+                    if (t + q * T >= a.nkx) e[q] = make_float2(0.f, 0.f);
             }
         }  // prev_cnt == 0: e == 0, E == 0
         // selection bytes packed four to a register: (sel(q) = s1w[q / 4] >> 8 (q % 4)) & 0xff
@@ -801,13 +823,13 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
             if (!((used >> ir) & 1u) || (direct_cr && ir == cr)) continue;
             const float2* row = krow(a, c, ir);
             float2 w[P];
-            kload_row<N, P>(w, row, c.t);
+            X::kload_row(w, row, a, c.t);
 #pragma unroll
             for (int q = 0; q < P; ++q) {
                 const float2 L = w[q];
                 w[q] = make_float2(z[q].x * L.x + e[q].x * L.y, z[q].y * L.x + e[q].y * L.y);
             }
-            xfft<N, P, +1, WAVE>(w, smem, a.tw, t, pp);
+            X::template fft<+1, WAVE>(w, smem, a, t, pp);
 #pragma unroll
             for (int q = 0; q < (HALF ? P / 2 : P); ++q)
                 if ((int)(s1(q) & 0x7fu) == ir) out[q] = w[q];
@@ -817,14 +839,14 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
             // e_new = IFFT(Re L E - Im L Z), the conjugate of IFFT(V) by L(-k) = L(k)
             const float2* row = krow(a, c, cr);
             float2 lv[P];
-            kload_row<N, P>(lv, row, c.t);
+            X::kload_row(lv, row, a, c.t);
             if (cnt > K) {
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
                     const float2 L = lv[q];
                     e[q] = make_float2(L.x * e[q].x - L.y * z[q].x, L.x * e[q].y - L.y * z[q].y);
                 }
-                xfft<N, P, +1, WAVE>(e, smem, a.tw, t, pp);
+                X::template fft<+1, WAVE>(e, smem, a, t, pp);
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
                     const int x = t + q * T;
@@ -848,8 +870,8 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
                     acc[j] = accw[j] = make_float2(0.f, 0.f);
                     if (j >= cnt) continue;
                     const int xj = crow[3 + j];
-                    float2 w = a.tw[(t * xj) & (N - 1)];
-                    const float2 st = a.tw[(T * xj) & (N - 1)];
+                    float2 w = twid(t * xj);
+                    const float2 st = twid(T * xj);
 #pragma unroll
                     for (int q = 0; q < P; ++q) {
                         acc[j] = cadd(acc[j], cmulc(e[q], w));
@@ -947,6 +969,14 @@ template <int N, bool HALF>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_packed_w(March2DArgs a)
 {
     adj_body_packed<N, N / 64, true, HALF>(a);
+}
+
+// Packed compat migration of a mixed-radix nkx on the array length M (the subgroup shape of the
+// other non-power-of-two kernels: M / 8 threads from M = 1024, one wave below).
+template <int M, int P, bool HALF>
+__global__ __launch_bounds__(M / P) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_packed_mixed(March2DArgs a)
+{
+    adj_body_packed<M, P, false, HALF, XMixed<M, P>>(a);
 }
 
 #define SWEM_SUBGROUP_BOUNDS(P) __launch_bounds__((P) == 16 ? 512 : kMaxBlock)
@@ -1054,6 +1084,7 @@ March2DShape march2d_shape(int nkx, int variant, int adj)
         sh.blue = m;
         sh.mixed = march2d_kind(nkx) == kLenMixed;
         sh.variant = adj ? SWEM_VARIANT_PAIR : SWEM_VARIANT_SUBGROUP;
+        if (sh.mixed && adj && variant == SWEM_VARIANT_PACKED) sh.variant = SWEM_VARIANT_PACKED;
         sh.T = m >= 1024 ? m / 8 : 64;
         sh.per_freq = 1;
         sh.max_block = adj ? sh.T : 512;
@@ -1157,6 +1188,13 @@ static int launch_blue_mode(const March2DArgs& a, const March2DShape& sh, int F,
     constexpr int P = M >= 1024 ? 8 : M / 64;
     const size_t lds = march2d_lds_bytes(a.nkx, sh, F);
     dim3 block(sh.T * sh.per_freq * F);
+    if constexpr (MODE == kModePspiCompat) {
+        if (sh.mixed && adj && sh.variant == SWEM_VARIANT_PACKED) {
+            const size_t lds_p = lds + kPackedScratch * sizeof(float2);
+            return 2 * a.nmx <= M ? launch_kernel(march2d_adj_packed_mixed<M, P, true>, a, block, lds_p, st)
+                                  : launch_kernel(march2d_adj_packed_mixed<M, P, false>, a, block, lds_p, st);
+        }
+    }
     if (sh.mixed)
         return adj ? launch_kernel(march2d_adj_mixed<M, MODE, P>, a, block, lds, st)
                    : launch_kernel(march2d_fwd_mixed<M, MODE, P>, a, block, lds, st);
