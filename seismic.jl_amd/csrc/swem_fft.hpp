@@ -199,11 +199,81 @@ constexpr int fft_passes()
     return p;
 }
 
+// Twiddle seeds of fft<N,P>: for each pass after the first (NS > 1, radix R) and each butterfly
+// b of a thread, the powers w^e, w = exp(-2 pi i k / (NS R)), k = (t + b T) mod NS, for the
+// seeded exponents e (1, 2, 3 and the multiples of 4 below R), read from the table at the start
+// of a transform.  The other powers are one product of two seeds, w^(4a + c) = w^(4a) w^c, so
+// a twiddle carries at most one rounding beyond the table's (powers by longer chains of
+// products drift: their systematic error, repeated over every transform of a march, grows the
+// image error tenfold).  Radix 16: 6 loads per butterfly instead of 15, radix 8: 4 instead of 7.
+template <int R>
+constexpr int seed_count()
+{
+    return R <= 4 ? R - 1 : 3 + (R / 4 - 1);
+}
+// the i-th seeded exponent: 1, 2, 3, then 4, 8, 12, ...
+constexpr int seed_exp(int i) { return i < 3 ? i + 1 : 4 * (i - 2); }
+
+template <int N, int P>
+constexpr int fft_seeds_before(int ns_end)
+{
+    int n = 1, c = 0;
+    while (n < N && n < ns_end) {
+        const int rem = N / n, r = rem >= P ? P : rem;
+        if (n > 1) c += (P / r) * (r <= 4 ? r - 1 : 3 + (r / 4 - 1));
+        n *= r;
+    }
+    return c;
+}
+template <int N, int P>
+struct FftSeeds {
+    static constexpr int count = fft_seeds_before<N, P>(N);
+    float2 w[count > 0 ? count : 1];
+};
+
+// The seeds of pass NS (all passes from NS on when ALL).
+template <int N, int P, int NS, bool ALL>
+__device__ __forceinline__ void fft_seeds_load(FftSeeds<N, P>& s, const float2* __restrict__ tw, int t)
+{
+    constexpr int T = N / P;
+    constexpr int REM = N / NS;
+    constexpr int R = REM >= P ? P : REM;
+    if constexpr (NS > 1) {
+        constexpr int off = fft_seeds_before<N, P>(NS);
+        constexpr int S = seed_count<R>();
+#pragma unroll
+        for (int b = 0; b < P / R; ++b) {
+            // opaque: the loads stay in the transform instead of pinning registers across the
+            // caller's loops
+            int ti = ((t + b * T) & (NS - 1)) * (N / (NS * R));
+            asm volatile("" : "+v"(ti));
+#pragma unroll
+            for (int i = 0; i < S; ++i) s.w[off + b * S + i] = tw[seed_exp(i) * ti];
+        }
+    }
+    if constexpr (ALL && NS * R < N) fft_seeds_load<N, P, NS * R, true>(s, tw, t);
+}
+
+// pw[r] = w^r, r = 1 .. R-1, from the seeds sd[i] = w^seed_exp(i) (conjugated for SIGN > 0)
+template <int R, int SIGN>
+__device__ __forceinline__ void twiddle_powers(const float2* sd, float2 (&pw)[R])
+{
+    constexpr int S = seed_count<R>();
+#pragma unroll
+    for (int i = 0; i < S; ++i) pw[seed_exp(i)] = SIGN > 0 ? cconj(sd[i]) : sd[i];
+#pragma unroll
+    for (int r = 5; r < R; ++r)
+        if (r % 4 != 0) pw[r] = cmul(pw[r & ~3], pw[r & 3]);
+}
+
 // One Stockham pass and the passes after it on one wavefield v in the strided mapping.
 // lds: the subgroup's two buffers (2 * LdsLen<N> complex); pp: buffer parity, toggled per
-// round trip.  tw[m] = exp(-2 pi i m/N).
-template <int N, int P, int SIGN, int NS = 1>
-__device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* __restrict__ tw, int t, int& pp)
+// round trip; s: the twiddle seeds, loaded at the start of the transform (PRE) or of each pass
+// (every load of a pass in flight before the first use: under register pressure the scheduler
+// would otherwise pair each load with its use and serialise the latencies).
+template <int N, int P, int SIGN, int NS, bool PRE>
+__device__ __forceinline__ void fft_pass(float2 (&v)[P], float2* lds, const float2* __restrict__ tw,
+                                         FftSeeds<N, P>& s, int t, int& pp)
 {
     constexpr int T = N / P;
     constexpr int REM = N / NS;
@@ -213,6 +283,10 @@ __device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* _
     constexpr int L = LdsLen<N>::value;
     constexpr int SH = PassPad<NS, R>::value;
     float2* buf = lds + pp * L;
+    if constexpr (NS > 1 && !PRE) {
+        fft_seeds_load<N, P, NS, false>(s, tw, t);
+        __builtin_amdgcn_sched_barrier(0);
+    }
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = t + b * T;
@@ -221,19 +295,10 @@ __device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* _
 #pragma unroll
         for (int r = 0; r < R; ++r) x[r] = v[b + r * B];
         if constexpr (NS > 1) {
-            // The twiddle index is made opaque to the optimiser so the loads stay in this pass
-            // instead of being hoisted out of the depth loop, where 21 complex twiddles per
-            // transform would pin 42 registers for the whole kernel.
-            int ti = k * (N / (NS * R));
-            asm volatile("" : "+v"(ti));
-            // every twiddle load in flight before the first use: under register pressure the
-            // scheduler would otherwise pair each load with its use and serialise the latencies
-            float2 w[R];
+            float2 pw[R];
+            twiddle_powers<R, SIGN>(&s.w[fft_seeds_before<N, P>(NS) + b * seed_count<R>()], pw);
 #pragma unroll
-            for (int r = 1; r < R; ++r) w[r] = tw[r * ti];
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int r = 1; r < R; ++r) x[r] = cmul(x[r], SIGN > 0 ? make_float2(w[r].x, -w[r].y) : w[r]);
+            for (int r = 1; r < R; ++r) x[r] = cmul(x[r], pw[r]);
         }
         Dft<R, SIGN>::run(x);
         if constexpr (LAST) {
@@ -250,8 +315,19 @@ __device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* _
 #pragma unroll
         for (int q = 0; q < P; ++q) v[q] = buf[padsh<SH>(t + q * T)];
         pp ^= 1;
-        fft<N, P, SIGN, NS * R>(v, lds, tw, t, pp);
+        fft_pass<N, P, SIGN, NS * R, PRE>(v, lds, tw, s, t, pp);
     }
+}
+
+// The transform of one wavefield v in the strided mapping.  tw[m] = exp(-2 pi i m/N).  PRE:
+// every seed loaded at the start (their latency overlaps the first pass; 2 registers per seed
+// held through it), else per pass.
+template <int N, int P, int SIGN, bool PRE = false>
+__device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* __restrict__ tw, int t, int& pp)
+{
+    FftSeeds<N, P> s;
+    if constexpr (PRE) fft_seeds_load<N, P, 1, true>(s, tw, t);
+    fft_pass<N, P, SIGN, 1, PRE>(v, lds, tw, s, t, pp);
 }
 
 

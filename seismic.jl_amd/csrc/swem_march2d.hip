@@ -115,29 +115,30 @@ __device__ __forceinline__ void kload_row(float2 (&L)[P], const float2* __restri
 
 // One x-transform of the wavefield: a wave-level transform (WAVE, P = N/64, no barriers) or a
 // subgroup transform through ping-pong buffers.
-template <int N, int P, int SIGN, bool WAVE>
+template <int N, int P, int SIGN, bool WAVE, bool PRE = false>
 __device__ __forceinline__ void xfft(float2 (&v)[P], float2* lds, const float2* __restrict__ tw, int t, int& pp)
 {
     if constexpr (WAVE) {
         static_assert(P * 64 == N, "wave transforms hold N/64 points per lane");
         wfft<N, SIGN>(v, lds, tw, t);
     } else
-        fft<N, P, SIGN>(v, lds, tw, t, pp);
+        fft<N, P, SIGN, PRE>(v, lds, tw, t, pp);
 }
 
 // Transform policies of the step functions.  XPow2: the transform length is the register array
-// length N (a power of two).  XBlue: a transform of runtime length n = nkx (any length) through
+// length N (a power of two); PRE: every twiddle seed of a transform loaded at its start (the
+// packed kernel: fastest, two registers per seed held through the first pass).  XBlue: a transform of runtime length n = nkx (any length) through
 // two power-of-two transforms of the array length M >= 2 n - 1 (Bluestein): with
 // w_j = exp(SIGN i pi j^2 / n),  X_k = w_k sum_j (x_j w_j) conj(w_(k-j)),  the sum a cyclic
 // convolution of length M taken as IFFT_M(FFT_M(x w) B) with B = FFT_M(conj w, wrapped) / M
 // built in FP64.  Entries j, k >= n of the array are zero on input and on output.  The PSPI
 // selection tables keep the array length as their row stride (0x7f padding beyond nmx) in both.
-template <int N, int P>
+template <int N, int P, bool PRE = false>
 struct XPow2 {
     template <int SIGN, bool WAVE>
     __device__ __forceinline__ static void fft(float2 (&v)[P], float2* lds, const March2DArgs& a, int t, int& pp)
     {
-        xfft<N, P, SIGN, WAVE>(v, lds, a.tw, t, pp);
+        xfft<N, P, SIGN, WAVE, PRE>(v, lds, a.tw, t, pp);
     }
     __device__ __forceinline__ static void kload_row(float2 (&L)[P], const float2* __restrict__ row, const March2DArgs&, int t)
     {
@@ -697,13 +698,13 @@ __device__ __forceinline__ void subgroup_sync()
 // those constants let the first forward pass and the last inverse pass drop their work.
 // X: the transform policy (XPow2: nkx = N; XMixed: a mixed-radix nkx <= N, whose twiddle table
 // holds nkx entries and whose spectra are zero at k >= nkx).
-template <int N, int P, bool WAVE, bool HALF, class X = XPow2<N, P>>
+template <int N, int P, bool WAVE, bool HALF, class X = XPow2<N, P, true>>
 __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
 {
     constexpr int T = N / P;
     constexpr int K = kUncutSparse;
     constexpr int NW = T / 64;  // waves of the subgroup
-    constexpr bool POW2 = std::is_same<X, XPow2<N, P>>::value;
+    constexpr bool POW2 = std::is_same<X, XPow2<N, P, true>>::value;
     // exp(-2 pi i m / nkx) for the direct sums (m >= 0)
     auto twid = [&](int m) { return POW2 ? a.tw[m & (N - 1)] : a.tw[m % a.nkx]; };
     static_assert(4 + K + 2 * NW * K <= kPackedScratch, "packed scratch");
