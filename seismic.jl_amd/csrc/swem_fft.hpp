@@ -344,10 +344,11 @@ struct WPass {
     static constexpr int R = REM >= RMAX ? RMAX : REM;
     static constexpr int B = P / R;
     static constexpr bool LAST = (NS * R == N);
-    static constexpr int NTW = NS > 1 ? B * (R - 1) : 1;  // twiddles per lane (1: unused slot)
+    static constexpr int S = seed_count<R>();
+    static constexpr int NTW = NS > 1 ? B * S : 1;  // twiddle seeds per lane (1: unused slot)
 };
 
-// The twiddles of pass NS of wfft<N>: w[b (R - 1) + r - 1] = tw[r k N / (NS R)].
+// The twiddle seeds of pass NS of wfft<N> (as in fft): w[b S + i] = tw[seed_exp(i) k N / (NS R)].
 template <int N, int NS>
 __device__ __forceinline__ void wfft_twiddles(const float2* __restrict__ tw, int t, float2 (&w)[WPass<N, NS>::NTW])
 {
@@ -359,7 +360,7 @@ __device__ __forceinline__ void wfft_twiddles(const float2* __restrict__ tw, int
             int ti = k * (N / (NS * S::R));
             asm volatile("" : "+v"(ti));  // keep the loads in the transform (see fft)
 #pragma unroll
-            for (int r = 1; r < S::R; ++r) w[b * (S::R - 1) + r - 1] = tw[r * ti];
+            for (int i = 0; i < S::S; ++i) w[b * S::S + i] = tw[seed_exp(i) * ti];
         }
     }
 }
@@ -376,8 +377,8 @@ __device__ __forceinline__ void wfft_pass(float2 (&v)[N / 64], float2* buf, cons
     constexpr int SH = PassPad<NS, R>::value;
     static_assert(P % R == 0, "radix must divide the points per lane");
     constexpr int NSN = S::LAST ? NS : NS * R;
-    // prefetch only where it costs at most 7 complex registers (N <= 512); larger transforms load
-    // their twiddles in the pass that uses them
+    // prefetch only where it costs at most 7 complex registers; larger transforms load their
+    // seeds in the pass that uses them
     constexpr bool PRE = WPass<N, NSN>::NTW <= 7;
     float2 wn[WPass<N, NSN>::NTW];
     if constexpr (!S::LAST && PRE) wfft_twiddles<N, NSN>(tw, t, wn);
@@ -389,11 +390,10 @@ __device__ __forceinline__ void wfft_pass(float2 (&v)[N / 64], float2* buf, cons
 #pragma unroll
         for (int r = 0; r < R; ++r) x[r] = v[b + r * B];
         if constexpr (NS > 1) {
+            float2 pw[R];
+            twiddle_powers<R, SIGN>(&w[b * S::S], pw);
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const float2 wr = w[b * (R - 1) + r - 1];
-                x[r] = cmul(x[r], SIGN > 0 ? make_float2(wr.x, -wr.y) : wr);
-            }
+            for (int r = 1; r < R; ++r) x[r] = cmul(x[r], pw[r]);
         }
         Dft<R, SIGN>::run(x);
         if constexpr (S::LAST) {
