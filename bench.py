@@ -96,7 +96,7 @@ def traffic_per_launch(nshot, kernel):
 def measure_ssf3d(args, steps, warmup, grid=256, nz=0, dot=True, graphs=True):
     """3-D split-step migration of one shot (nx=ny=256, dx=dy=12.5 m; nz=300, dz=10 m; nt=1500,
     dt=4 ms; 3-50 Hz = 565 bins; nkx=nky=512).  The passes are bound by their vector-ALU issue
-    (the x-pass keeps the VALU 66 % busy, its state stays in the last-level cache between the
+    (the x-pass keeps the VALU 88 % busy, its state stays in the last-level cache between the
     passes), so the roofline is the FP32 one on the algorithmic flops per (w, iz), 2 F + 4 nm with
     F = 10 nk log2 nk + 6 nk + 8 nm; the HBM figure on 64 nk algorithmic bytes per (w, iz) (two
     wavefields x two passes x read+write of the padded slice) is reported beside it.  Returns the
@@ -155,7 +155,7 @@ def measure_ssf3d(args, steps, warmup, grid=256, nz=0, dot=True, graphs=True):
                         "frac": tflops / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "march3d x-pass + y-pass",
                         "algorithmic_flops_per_shot": flops,
                         "note": "algorithmic flops 2 F + 4 nm per (w, iz), F = 10 nk log2 nk + 6 nk + 8 nm; "
-                                "the x-pass keeps the VALU 66 % busy (profiles/march3d_ssf_512x512_sq_counters.md)"},
+                                "the x-pass keeps the VALU 88 % busy (profiles/march3d_ssf_512x512_sq_counters.md)"},
            "hbm": {"achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                    "algorithmic_bytes_per_shot": byts,
                    "note": "64 nk bytes per (w, iz); the kernels skip the pad rows and move half"},

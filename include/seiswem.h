@@ -226,7 +226,8 @@ int swem_plan_timing(swem_plan* plan, swem_timing* out);
 /* Measured launch shapes of a plan (no reference counterpart; the reference plans FFTW with
  * FFTW_ESTIMATE, wem.c:103-104).  2-D: every migration kernel variant that fits the plan's nkx,
  * each with every frequency group size (frequencies per workgroup, 1..16) its workgroup and LDS
- * allow, is timed on nshot shots of zero records (the work does not depend on the data).  3-D:
+ * allow, is timed on nshot shots of pseudo-random records (the work is the same for any records
+ * with nonzero spectra).  3-D:
  * frequencies per y-pass workgroup (2..16, 8 at most for PSPI or a Bluestein y axis) times the step
  * state of one launch (64, 128, 256 MiB, i.e. frequencies marched together against the 256 MB
  * last-level cache), timed on the plan's first 16 depths.  The fastest candidate becomes the

@@ -165,8 +165,8 @@ class Plan:
         return {k: getattr(t, k) for k, _ in _lib.Timing._fields_}
 
     def autotune(self, nshot: int = 1) -> dict:
-        """Times every 2-D migration kernel variant that fits this plan on nshot zero records and
-        keeps the fastest (swem_plan_autotune).  Returns {"best": variant, "ms": {variant: ms}}."""
+        """Times every 2-D migration kernel variant that fits this plan on nshot pseudo-random
+        records and keeps the fastest (swem_plan_autotune).  Returns {"best": variant, "ms": {variant: ms}}."""
         best = ctypes.c_int(-1)
         times = (ctypes.c_float * (_lib.VARIANT_PACKED + 1))()
         check(load().swem_plan_autotune(self._p, int(nshot), ctypes.byref(best), times))
