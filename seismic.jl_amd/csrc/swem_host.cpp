@@ -366,8 +366,7 @@ static int build_tables(swem_plan* p, const float* wav)
         // Per depth: vref of the uncut points (-1: none), their count, 1 when no cut point selects
         // that vref, and the first kUncutSparse positions.
         std::vector<int> cref((size_t)nz * kCrefStride, 0);
-        bool packed = p->opt.pspi_mode == SWEM_PSPI_COMPAT && !p->three_d &&
-                      (narr == nkx || march2d_kind(nkx) == kLenMixed);
+        bool packed = p->opt.pspi_mode == SWEM_PSPI_COMPAT && !p->three_d;  // any 2-D length
         for (int iz = 0; iz < nz; iz++) {
             const float z = g.oz + g.dz * (float)iz;
             if (!(z >= g.sz && z >= g.gz)) packed = false;

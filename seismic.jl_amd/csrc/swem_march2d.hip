@@ -706,6 +706,9 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
     constexpr int K = kUncutSparse;
     constexpr int NW = T / 64;  // waves of the subgroup
     constexpr bool POW2 = std::is_same<X, XPow2<N, P, true>>::value;
+    // Bluestein lengths: the table tw is of the array length, not of nkx, so no direct sums (the
+    // uncut points' e and output always come from transforms)
+    constexpr bool BLUE = std::is_same<X, XBlue<N, P>>::value;
     // exp(-2 pi i m / nkx) for the direct sums (m >= 0)
     auto twid = [&](int m) { return POW2 ? a.tw[m & (N - 1)] : a.tw[m % a.nkx]; };
     static_assert(4 + K + 2 * NW * K <= kPackedScratch, "packed scratch");
@@ -771,7 +774,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
         const int* crow = a.cref + (size_t)iz * kCrefStride;
         const int cr = crow[0], cnt = crow[1];
         X::template fft<-1, WAVE>(z, smem, a, t, pp);
-        if (prev_cnt < 0 || prev_cnt > K) {
+        if (prev_cnt < 0 || prev_cnt > (BLUE ? 0 : K)) {
             X::template fft<-1, WAVE>(e, smem, a, t, pp);
         } else if (prev_cnt > 0) {
             // E(k) = sum_j e(x_j) w^(k x_j), w = exp(-2 pi i / N): the owners publish e(x_j)
@@ -820,7 +823,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
         // Few uncut points, and no cut point selecting their vref cr: their output under cr, like
         // their new e, comes from direct sums instead of an inverse transform of cr (HALF kernels:
         // the others have no registers left for the second set of sums).
-        const bool direct_cr = HALF && cnt > 0 && cnt <= K && crow[2] != 0;
+        const bool direct_cr = !BLUE && HALF && cnt > 0 && cnt <= K && crow[2] != 0;
         for (int ir = 0; ir < a.nref; ++ir) {
             if (!((used >> ir) & 1u) || (direct_cr && ir == cr)) continue;
             const float2* row = krow(a, c, ir);
@@ -842,7 +845,7 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
             const float2* row = krow(a, c, cr);
             float2 lv[P];
             X::kload_row(lv, row, a, c.t);
-            if (cnt > K) {
+            if (BLUE || cnt > K) {
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
                     const float2 L = lv[q];
@@ -973,12 +976,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ma
     adj_body_packed<N, N / 64, true, HALF>(a);
 }
 
-// Packed compat migration of a mixed-radix nkx on the array length M (the subgroup shape of the
-// other non-power-of-two kernels: M / 8 threads from M = 1024, one wave below).
+// Packed compat migration of a mixed-radix or Bluestein nkx on the array length M (the subgroup
+// shape of the other non-power-of-two kernels: M / 8 threads from M = 1024, one wave below).
 template <int M, int P, bool HALF>
 __global__ __launch_bounds__(M / P) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_packed_mixed(March2DArgs a)
 {
     adj_body_packed<M, P, false, HALF, XMixed<M, P>>(a);
+}
+template <int M, int P, bool HALF>
+__global__ __launch_bounds__(M / P) __attribute__((amdgpu_waves_per_eu(2))) void march2d_adj_packed_blue(March2DArgs a)
+{
+    adj_body_packed<M, P, false, HALF, XBlue<M, P>>(a);
 }
 
 #define SWEM_SUBGROUP_BOUNDS(P) __launch_bounds__((P) == 16 ? 512 : kMaxBlock)
@@ -1086,7 +1094,7 @@ March2DShape march2d_shape(int nkx, int variant, int adj)
         sh.blue = m;
         sh.mixed = march2d_kind(nkx) == kLenMixed;
         sh.variant = adj ? SWEM_VARIANT_PAIR : SWEM_VARIANT_SUBGROUP;
-        if (sh.mixed && adj && variant == SWEM_VARIANT_PACKED) sh.variant = SWEM_VARIANT_PACKED;
+        if (adj && variant == SWEM_VARIANT_PACKED) sh.variant = SWEM_VARIANT_PACKED;
         sh.T = m >= 1024 ? m / 8 : 64;
         sh.per_freq = 1;
         sh.max_block = adj ? sh.T : 512;
@@ -1191,10 +1199,13 @@ static int launch_blue_mode(const March2DArgs& a, const March2DShape& sh, int F,
     const size_t lds = march2d_lds_bytes(a.nkx, sh, F);
     dim3 block(sh.T * sh.per_freq * F);
     if constexpr (MODE == kModePspiCompat) {
-        if (sh.mixed && adj && sh.variant == SWEM_VARIANT_PACKED) {
+        if (adj && sh.variant == SWEM_VARIANT_PACKED) {
             const size_t lds_p = lds + kPackedScratch * sizeof(float2);
-            return 2 * a.nmx <= M ? launch_kernel(march2d_adj_packed_mixed<M, P, true>, a, block, lds_p, st)
-                                  : launch_kernel(march2d_adj_packed_mixed<M, P, false>, a, block, lds_p, st);
+            if (sh.mixed)
+                return 2 * a.nmx <= M ? launch_kernel(march2d_adj_packed_mixed<M, P, true>, a, block, lds_p, st)
+                                      : launch_kernel(march2d_adj_packed_mixed<M, P, false>, a, block, lds_p, st);
+            // Bluestein: M >= 2 nkx - 1 >= 2 nmx - 1, and M even, so the upper half is always zero
+            return launch_kernel(march2d_adj_packed_blue<M, P, true>, a, block, lds_p, st);
         }
     }
     if (sh.mixed)
