@@ -629,7 +629,7 @@ def main():
     ap.add_argument("--grid", type=int, default=256,
                     help="ssf3d: nx = ny of the 3-D grid (256: the 3-D configuration; others, e.g. 250, take "
                          "the Bluestein kernels)")
-    ap.add_argument("--plain-shots", type=int, default=4,
+    ap.add_argument("--plain-shots", type=int, default=8,
                     help="pspi2d: shots timed one per call on the plain and resident single-shot paths (0: skip)")
     ap.add_argument("--workload", choices=("pspi2d", "ssf3d", "serving", "lsm", "angles", "elastic", "elastic3d"),
                     default="pspi2d")
