@@ -1346,10 +1346,10 @@ int swem_plan_timing(swem_plan* p, swem_timing* out)
 // timed on nshot records of pseudo-random values and the fastest becomes the plan's variant.
 // times[SWEM_VARIANT_PACKED + 1] receives each variant's march time in ms (0: not a distinct
 // candidate).
-// Pseudo-random records of nshot shots spread over the line and an image buffer, for timing
-// candidate launch shapes.  The march does the same work for any records whose spectra are
-// nonzero; zero records would not do: the packed kernel writes a zero image without marching a
-// frequency whose receiver spectrum is zero.
+// Records of nshot shots spread over the line and an image buffer, for timing candidate launch
+// shapes.  The march does the same work for any records whose spectra are nonzero; 2-D records
+// are pseudo-random, since the packed kernel writes a zero image without marching a frequency
+// whose receiver spectrum is zero; 3-D ones are zero (no 3-D kernel skips work on them).
 struct TuneBuffers {
     float* d = nullptr;
     float* m = nullptr;
@@ -1365,15 +1365,19 @@ struct TuneBuffers {
         const size_t nm = p->nm, nt = p->g.nt, nz = p->g.nz;
         SWEM_HIP(hipMalloc((void**)&d, (size_t)nshot * nm * nt * sizeof(float)));
         SWEM_HIP(hipMalloc((void**)&m, nm * nz * sizeof(float)));
-        std::vector<float> h((size_t)nshot * nm * nt);
-        uint32_t x = 2463534242u;  // xorshift32
-        for (float& v : h) {
-            x ^= x << 13;
-            x ^= x >> 17;
-            x ^= x << 5;
-            v = (float)(int32_t)x * (1.f / 2147483648.f);
+        if (p->three_d) {
+            SWEM_HIP(hipMemset(d, 0, (size_t)nshot * nm * nt * sizeof(float)));
+        } else {
+            std::vector<float> h((size_t)nshot * nm * nt);
+            uint32_t x = 2463534242u;  // xorshift32
+            for (float& v : h) {
+                x ^= x << 13;
+                x ^= x >> 17;
+                x ^= x << 5;
+                v = (float)(int32_t)x * (1.f / 2147483648.f);
+            }
+            SWEM_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
         }
-        SWEM_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
         sx.resize(nshot);
         sy.assign(nshot, p->g.omy + p->g.dmy * (float)(p->g.nmy / 2));
         for (int i = 0; i < nshot; i++)

@@ -745,9 +745,10 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
     nr = subgroup_sum<T>(nr, red);
     const float ns = sqrtf(ss.x * ss.x + ss.y * ss.y);
     float* part = a.partial + ((size_t)shot * a.ngroups + g) * (size_t)a.nz * a.nmx;
-    if (!(nr > 0.f && ns > 0.f)) {
+    if (nr == 0.f || ns == 0.f) {
         // a zero receiver or source spectrum images to zero at every depth (the march is linear
         // in each): the packed pair would leave rounding noise of the other wavefield instead
+        // (a NaN in either takes the march and reaches the image)
         for (int iz = 0; iz < a.nz; ++iz) {
             const auto pr = buf_rsrc(part + (size_t)iz * a.nmx, (uint32_t)a.nmx * 4u);
 #pragma unroll
