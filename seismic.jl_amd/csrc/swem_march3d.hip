@@ -44,13 +44,26 @@ __device__ __forceinline__ double reduce_rev(double phase)
     return r - rint(r);
 }
 
+// FP64 square root of s >= 0 by the hardware reciprocal square root (5e-8 relative) and one
+// Newton step: 4e-15 relative at most (measured over 1e-20 .. 1), against about 17 instructions
+// of the library's correctly rounded expansion; 0 below 1e-300 (the estimate is not defined
+// at 0).
+__device__ __forceinline__ double sqrt_nr(double s)
+{
+    const double r = __builtin_amdgcn_rsq(s);
+    double y = s * r;
+    const double h = 0.5 * r;
+    y = fma(y, fma(-y, h, 0.5), y);
+    return s > 1e-300 ? y : 0.0;
+}
+
 // kz phase from s = w^2 p^2 - kx^2 - ky^2 evaluated in FP64 (the cancellation near the
 // evanescent cutoff): square root, phase and its reduction to revolutions in FP64, rounded to
 // float only for the hardware sine / cosine of the reduced angle; the evanescent factor
 // exp(-0.2 sqrt(-s) |dz|) from the FP64 root (wem.c:341-343).  1/nk folded in.
 __device__ __forceinline__ float2 kz_phase_s64(const March3DArgs& a, double s)
 {
-    const double q = sqrt(fabs(s));
+    const double q = sqrt_nr(fabs(s));
     if (s >= 0.0) {
         const float2 c = cis_rev(reduce_rev(q * a.dzd));
         return make_float2(c.x * a.inv_nk, c.y * a.inv_nk);
