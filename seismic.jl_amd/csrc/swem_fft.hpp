@@ -159,6 +159,57 @@ struct Dft<5, SIGN> {
     }
 };
 
+// x * exp(SIGN i pi M / 8) for a compile-time M in 0..7 (the twiddles W_R^n of R <= 16).
+template <int M, int SIGN>
+__device__ __forceinline__ float2 cmul_pi8(float2 x)
+{
+    constexpr float C[5] = {1.f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f, 0.f};
+    if constexpr (M == 0) return x;
+    else if constexpr (M == 4) return mul_si<SIGN>(x);
+    else if constexpr (M == 2) return cscale(cadd(x, mul_si<SIGN>(x)), C[2]);
+    else if constexpr (M == 6) return cscale(csub(mul_si<SIGN>(x), x), C[2]);
+    else {
+        // cos and sin of M pi / 8 (M odd): cos = +-C[M or 8 - M], sin = C[4 - M or M - 4]
+        constexpr float c = M < 4 ? C[M] : -C[8 - M];
+        constexpr float sn = M < 4 ? C[4 - M] : C[M - 4];
+        return cmul(x, make_float2(c, SIGN * sn));
+    }
+}
+
+// R-point DFT whose inputs r >= R/2 are zero: X_2m = DFT_{R/2}(x_n)_m, X_2m+1 = DFT_{R/2}(x_n W_R^n)_m
+// over n < R/2 (W_R = exp(SIGN 2 pi i / R)); v[r] for r >= R/2 is not read.
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_half_in(float2* v)
+{
+    constexpr int H = R / 2;
+    if constexpr (R == 2) {
+        v[1] = v[0];
+    } else {
+        float2 e[H], o[H];
+#pragma unroll
+        for (int n = 0; n < H; ++n) e[n] = v[n];
+        o[0] = v[0];
+        if constexpr (H > 1) o[1] = cmul_pi8<16 / R, SIGN>(v[1]);
+        if constexpr (H > 2) {
+            o[2] = cmul_pi8<32 / R, SIGN>(v[2]);
+            o[3] = cmul_pi8<48 / R, SIGN>(v[3]);
+        }
+        if constexpr (H > 4) {
+            o[4] = cmul_pi8<64 / R, SIGN>(v[4]);
+            o[5] = cmul_pi8<80 / R, SIGN>(v[5]);
+            o[6] = cmul_pi8<96 / R, SIGN>(v[6]);
+            o[7] = cmul_pi8<112 / R, SIGN>(v[7]);
+        }
+        Dft<H, SIGN>::run(e);
+        Dft<H, SIGN>::run(o);
+#pragma unroll
+        for (int m = 0; m < H; ++m) {
+            v[2 * m] = e[m];
+            v[2 * m + 1] = o[m];
+        }
+    }
+}
+
 // Bank-conflict padding: one complex inserted every 2^SH.  The LDS has 64 banks of 4 bytes (32
 // complex per row).  A Stockham pass writes at lane stride R (first pass) or in groups of NS
 // lanes, then reads back at unit lane stride; SH is chosen per pass so both the writes and the
@@ -271,7 +322,7 @@ __device__ __forceinline__ void twiddle_powers(const float2* sd, float2 (&pw)[R]
 // round trip; s: the twiddle seeds, loaded at the start of the transform (PRE) or of each pass
 // (every load of a pass in flight before the first use: under register pressure the scheduler
 // would otherwise pair each load with its use and serialise the latencies).
-template <int N, int P, int SIGN, int NS, bool PRE>
+template <int N, int P, int SIGN, int NS, bool PRE, bool HIN>
 __device__ __forceinline__ void fft_pass(float2 (&v)[P], float2* lds, const float2* __restrict__ tw,
                                          FftSeeds<N, P>& s, int t, int& pp)
 {
@@ -300,7 +351,8 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[P], float2* lds, const floa
 #pragma unroll
             for (int r = 1; r < R; ++r) x[r] = cmul(x[r], pw[r]);
         }
-        Dft<R, SIGN>::run(x);
+        if constexpr (HIN && NS == 1 && B == 1) dft_half_in<R, SIGN>(x);
+        else Dft<R, SIGN>::run(x);
         if constexpr (LAST) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b + r * B] = x[r];
@@ -315,19 +367,20 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[P], float2* lds, const floa
 #pragma unroll
         for (int q = 0; q < P; ++q) v[q] = buf[padsh<SH>(t + q * T)];
         pp ^= 1;
-        fft_pass<N, P, SIGN, NS * R, PRE>(v, lds, tw, s, t, pp);
+        fft_pass<N, P, SIGN, NS * R, PRE, false>(v, lds, tw, s, t, pp);
     }
 }
 
 // The transform of one wavefield v in the strided mapping.  tw[m] = exp(-2 pi i m/N).  PRE:
 // every seed loaded at the start (their latency overlaps the first pass; 2 registers per seed
-// held through it), else per pass.
-template <int N, int P, int SIGN, bool PRE = false>
+// held through it), else per pass.  HIN: the points N/2 .. N-1 (slots q >= P/2) are zero on
+// input and are not read (the first pass runs half-input DFTs).
+template <int N, int P, int SIGN, bool PRE = false, bool HIN = false>
 __device__ __forceinline__ void fft(float2 (&v)[P], float2* lds, const float2* __restrict__ tw, int t, int& pp)
 {
     FftSeeds<N, P> s;
     if constexpr (PRE) fft_seeds_load<N, P, 1, true>(s, tw, t);
-    fft_pass<N, P, SIGN, 1, PRE>(v, lds, tw, s, t, pp);
+    fft_pass<N, P, SIGN, 1, PRE, HIN>(v, lds, tw, s, t, pp);
 }
 
 

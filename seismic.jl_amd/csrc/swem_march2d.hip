@@ -115,14 +115,14 @@ __device__ __forceinline__ void kload_row(float2 (&L)[P], const float2* __restri
 
 // One x-transform of the wavefield: a wave-level transform (WAVE, P = N/64, no barriers) or a
 // subgroup transform through ping-pong buffers.
-template <int N, int P, int SIGN, bool WAVE, bool PRE = false>
+template <int N, int P, int SIGN, bool WAVE, bool PRE = false, bool HIN = false>
 __device__ __forceinline__ void xfft(float2 (&v)[P], float2* lds, const float2* __restrict__ tw, int t, int& pp)
 {
     if constexpr (WAVE) {
         static_assert(P * 64 == N, "wave transforms hold N/64 points per lane");
         wfft<N, SIGN>(v, lds, tw, t);
     } else
-        fft<N, P, SIGN, PRE>(v, lds, tw, t, pp);
+        fft<N, P, SIGN, PRE, HIN>(v, lds, tw, t, pp);
 }
 
 // Transform policies of the step functions.  XPow2: the transform length is the register array
@@ -136,10 +136,11 @@ __device__ __forceinline__ void xfft(float2 (&v)[P], float2* lds, const float2* 
 // selection tables keep the array length as their row stride (0x7f padding beyond nmx) in both.
 template <int N, int P, bool PRE = true>
 struct XPow2 {
-    template <int SIGN, bool WAVE>
+    // HIN: slots q >= P/2 are zero on input (see fft)
+    template <int SIGN, bool WAVE, bool HIN = false>
     __device__ __forceinline__ static void fft(float2 (&v)[P], float2* lds, const March2DArgs& a, int t, int& pp)
     {
-        xfft<N, P, SIGN, WAVE, PRE>(v, lds, a.tw, t, pp);
+        xfft<N, P, SIGN, WAVE, PRE, HIN>(v, lds, a.tw, t, pp);
     }
     __device__ __forceinline__ static void kload_row(float2 (&L)[P], const float2* __restrict__ row, const March2DArgs&, int t)
     {
@@ -150,7 +151,7 @@ struct XPow2 {
 template <int M, int P>
 struct XBlue {
     static constexpr int T = M / P;
-    template <int SIGN, bool WAVE>
+    template <int SIGN, bool WAVE, bool HIN = false>
     __device__ __forceinline__ static void fft(float2 (&v)[P], float2* lds, const March2DArgs& a, int t, int& pp)
     {
         static_assert(!WAVE, "Bluestein transforms use the subgroup FFT");
@@ -200,7 +201,7 @@ struct XBlue {
 // transformed in the subgroup's two LDS buffers (mixed_fft); entries j >= n stay zero.
 template <int M, int P>
 struct XMixed {
-    template <int SIGN, bool WAVE>
+    template <int SIGN, bool WAVE, bool HIN = false>
     __device__ __forceinline__ static void fft(float2 (&v)[P], float2* lds, const March2DArgs& a, int t, int& pp)
     {
         static_assert(!WAVE, "mixed-radix transforms use the subgroup buffers");
@@ -774,9 +775,10 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
         c.iz = iz;
         const int* crow = a.cref + (size_t)iz * kCrefStride;
         const int cr = crow[0], cnt = crow[1];
-        X::template fft<-1, WAVE>(z, smem, a, t, pp);
+        // HALF: slots q >= P/2 of z and e are zero (x >= N/2 >= nmx is never selected)
+        X::template fft<-1, WAVE, HALF>(z, smem, a, t, pp);
         if (prev_cnt < 0 || prev_cnt > (BLUE ? 0 : K)) {
-            X::template fft<-1, WAVE>(e, smem, a, t, pp);
+            X::template fft<-1, WAVE, HALF>(e, smem, a, t, pp);
         } else if (prev_cnt > 0) {
             // E(k) = sum_j e(x_j) w^(k x_j), w = exp(-2 pi i / N): the owners publish e(x_j)
 #pragma unroll
@@ -943,13 +945,16 @@ __device__ __forceinline__ void adj_body_packed(const March2DArgs& a)
                 continue;  // x >= nmx: nothing to image
             }
             buf_store1_nt(pr, (uint32_t)(t + q * T) * 4u, factor * (z[q].x * z[q].y - e[q].x * e[q].y) * inv_alpha);
-            n2s += z[q].x * z[q].x + e[q].x * e[q].x;
-            n2r += z[q].y * z[q].y + e[q].y * e[q].y;
         }
         // keep the two packed wavefields of comparable size (the rounding of each transform is
         // relative to their sum): every 8 depths, rescale the receiver part when the ratio of
         // their energies leaves [1/4, 4]
         if ((iz & 7) != 7) continue;
+#pragma unroll
+        for (int q = 0; q < (HALF ? P / 2 : P); ++q) {
+            n2s += z[q].x * z[q].x + e[q].x * e[q].x;
+            n2r += z[q].y * z[q].y + e[q].y * e[q].y;
+        }
         n2s = subgroup_sum<T>(n2s, red);
         n2r = subgroup_sum<T>(n2r, red);
         if (n2s > 0.f && n2r > 0.f && (n2s > 4.f * n2r || n2r > 4.f * n2s)) {
