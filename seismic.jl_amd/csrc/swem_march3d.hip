@@ -46,15 +46,15 @@ __device__ __forceinline__ double reduce_rev(double phase)
 
 // FP64 square root of s >= 0 by the hardware reciprocal square root (5e-8 relative) and one
 // Newton step: 4e-15 relative at most (measured over 1e-20 .. 1), against about 17 instructions
-// of the library's correctly rounded expansion; 0 below 1e-300 (the estimate is not defined
-// at 0).
+// of the library's correctly rounded expansion; 0 up to 1e-300 (the estimate is not defined at
+// 0); a NaN stays NaN.
 __device__ __forceinline__ double sqrt_nr(double s)
 {
     const double r = __builtin_amdgcn_rsq(s);
     double y = s * r;
     const double h = 0.5 * r;
     y = fma(y, fma(-y, h, 0.5), y);
-    return s > 1e-300 ? y : 0.0;
+    return s <= 1e-300 ? 0.0 : y;
 }
 
 // kz phase from s = w^2 p^2 - kx^2 - ky^2 evaluated in FP64 (the cancellation near the
