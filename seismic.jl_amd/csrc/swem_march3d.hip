@@ -230,6 +230,24 @@ __device__ __forceinline__ float2 kz_phase64(const March3DArgs& a, double c0, in
     return kz_phase_s64(a, fma(-(double)(kxi * kxi), a.dkx2, c0));
 }
 
+// kz phases of a lane's points ikx = t + 64 q of one ky column on a power-of-two nkx = N (KZ64):
+// q <= P/2 directly, the others from their mirror nkx - ikx (the same |kx|, so the same phase bit
+// for bit), which lane 64 - t holds at slot P - 1 - q (lane 0: its own slot P - q).
+template <int N>
+__device__ __forceinline__ void kz_phases64_col(const March3DArgs& a, double c0, int t, float2 (&ph)[N / 64])
+{
+    constexpr int P = N / 64;
+#pragma unroll
+    for (int q = 0; q <= P / 2 && q < P; ++q) ph[q] = kz_phase64(a, c0, t + 64 * q);
+    const int src = (64 - t) & 63;
+#pragma unroll
+    for (int q = P / 2 + 1; q < P; ++q) {
+        const float2 m = ph[P - 1 - q];
+        const float x = __shfl(m.x, src, 64), y = __shfl(m.y, src, 64);
+        ph[q] = t == 0 ? ph[P - q] : make_float2(x, y);
+    }
+}
+
 // x-pass.  Workgroup: C waves, wave w owns column ky0 + w of every stepped wavefield (NWF = 2:
 // source and receiver, which share the kz phase; NWF = 1: the one wmask names).  The tile of C
 // consecutive ky columns is staged through LDS so the global reads and writes are row segments
@@ -295,9 +313,15 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
     bool cj[NWF];
 #pragma unroll
     for (int k = 0; k < NWF; ++k) cj[k] = wf_of(k) == 0 ? a.conj_src : a.conj_rec;
+    float2 phm[P];
+    const bool mirror = BX == kLenPow2 && a.prec == SWEM_PREC_KZ64;
+    if constexpr (BX == kLenPow2)
+        if (mirror) kz_phases64_col<NX>(a, c0, t, phm);
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-        const float2 ph = a.prec == SWEM_PREC_KZ64 ? kz_phase64(a, c0, t + 64 * q) : kz_phase(a, wd, p2, a.p2f[a.iz], t + 64 * q, iky);
+        const float2 ph = mirror ? phm[q]
+                          : a.prec == SWEM_PREC_KZ64 ? kz_phase64(a, c0, t + 64 * q)
+                                                     : kz_phase(a, wd, p2, a.p2f[a.iz], t + 64 * q, iky);
 #pragma unroll
         for (int k = 0; k < NWF; ++k) v[k][q] = cmul(v[k][q], make_float2(ph.x, cj[k] ? -ph.y : ph.y));
     }
