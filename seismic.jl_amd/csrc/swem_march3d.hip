@@ -1176,10 +1176,15 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 10
             fetch(k, v);
             const double p2 = pl.p2[a.iz];
             const double c0 = wd * wd * p2 - ky * ky;
+            float2 phm[P];
+            const bool mirror = BX == kLenPow2 && a.prec == SWEM_PREC_KZ64;
+            if constexpr (BX == kLenPow2)
+                if (mirror) kz_phases64_col<NX>(pl, c0, t, phm);
 #pragma unroll
             for (int q = 0; q < P; ++q) {
-                const float2 ph = a.prec == SWEM_PREC_KZ64 ? kz_phase64(pl, c0, t + 64 * q)
-                                                           : kz_phase(pl, wd, p2, pl.p2f[a.iz], t + 64 * q, iky);
+                const float2 ph = mirror ? phm[q]
+                                  : a.prec == SWEM_PREC_KZ64 ? kz_phase64(pl, c0, t + 64 * q)
+                                                             : kz_phase(pl, wd, p2, pl.p2f[a.iz], t + 64 * q, iky);
                 v[q] = cmul(v[q], make_float2(ph.x, cj ? -ph.y : ph.y));
             }
             put(k, v);
