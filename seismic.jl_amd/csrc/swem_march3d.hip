@@ -431,13 +431,20 @@ __global__ __launch_bounds__(64 * C) __attribute__((amdgpu_waves_per_eu(NX >= 51
         const double p2 = a.p2[(size_t)a.iz * a.nref + ir];
         const float p2f = a.p2f[(size_t)a.iz * a.nref + ir];
         const double c0 = wd * wd * p2 - ky * ky;
+        // the two wavefields of a migration share each phase (the one-wavefield shapes keep
+        // computing it per point: the mirrored set would spill there)
+        float2 phm[P];
+        const bool mirror = NWF == 2 && BX == kLenPow2 && a.prec == SWEM_PREC_KZ64;
+        if constexpr (NWF == 2 && BX == kLenPow2)
+            if (mirror) kz_phases64_col<NX>(a, c0, t, phm);
 #pragma unroll
         for (int k = 0; k < NWF; ++k) {
             float2 bq[P];
 #pragma unroll
             for (int q = 0; q < P; ++q) {
-                const float2 ph = a.prec == SWEM_PREC_KZ64 ? kz_phase64(a, c0, t + 64 * q)
-                                                           : kz_phase(a, wd, p2, p2f, t + 64 * q, iky);
+                const float2 ph = mirror ? phm[q]
+                                  : a.prec == SWEM_PREC_KZ64 ? kz_phase64(a, c0, t + 64 * q)
+                                                             : kz_phase(a, wd, p2, p2f, t + 64 * q, iky);
                 bq[q] = cmul(v[k][q], make_float2(ph.x, cj[k] ? -ph.y : ph.y));
             }
             float2* buf = smem + (w * NWF + k) * L;
