@@ -1,6 +1,6 @@
 """GPU edge cases against the FP64 oracle: the smallest grids (one trace, nmx 2 and 3, nz 1), the largest
 power-of-two transform (nkx 4096), receivers and sources below the surface with a shifted depth
-origin, and zero records (an exactly zero image)."""
+origin, and zero records or a zero wavelet (an exactly zero image, for each packed PSPI body)."""
 import numpy as np
 import pytest
 
@@ -71,9 +71,21 @@ def test_receivers_and_source_below_the_surface(gpu, pspi):
     assert rel_l2(dd, O.wem(oparams(g), vel, wav, 300.0, m=r, adj=False)) <= TOL
 
 
-def test_zero_records_give_a_zero_image(gpu):
-    g = geom(100, 20, pspi=True)
+# nkx 200 (mixed radix), 202 (Bluestein), 256 (power of two): each packed PSPI body
+@pytest.mark.parametrize("nx", [100, 101, 128])
+def test_zero_records_give_a_zero_image(gpu, nx):
+    g = geom(nx, 20, pspi=True)
     vel = np.full((g.nmx, g.nz), 2000.0, np.float32)
     wav = synth.wavelet_trace(g.nt, g.dt, 15.0)
     m = seiswem.ShotProfileWEM(np.zeros((1, g.nm, g.nt), np.float32), True, vel=vel, wav=wav, geom=g, sx=[500.0])
+    assert np.all(m == 0)
+
+
+@pytest.mark.parametrize("nx", [100, 101, 128])
+def test_zero_wavelet_gives_a_zero_image(gpu, nx):
+    g = geom(nx, 20, pspi=True)
+    rng = np.random.default_rng(nx)
+    vel = synth.blob_velocity(g.nmx, g.nz, g.dmx, g.dz, rng, nblobs=3, v0=2000.0)
+    d = rng.standard_normal((1, g.nm, g.nt)).astype(np.float32)
+    m = seiswem.ShotProfileWEM(d, True, vel=vel, wav=np.zeros(g.nt, np.float32), geom=g, sx=[500.0])
     assert np.all(m == 0)
